@@ -1,0 +1,118 @@
+// Vector kernels of the Newton-Krylov loop: wavefront-shuffle reductions
+// (norms, fused multi-dots for Gram-Schmidt), axpy forms, and the
+// finite-difference Jacobian-vector product.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "fvp_blas.h"
+#include "fvp_internal.h"
+
+namespace fvp {
+
+constexpr int kRedBlocks = 512;   // partial sums per reduction (>= 2 per CU)
+constexpr int kRedThreads = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// block sum of v into out (thread 0 holds the result); 256 threads = 4 wavefronts
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double r = 0.;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += lds[w];
+    __syncthreads();
+    return r;
+}
+
+// partials[b] = sum over this block's grid-stride slice of x*y (x == y gives ||x||^2)
+__global__ void __launch_bounds__(kRedThreads) dot_partial(long n, const double *__restrict__ x,
+                                                          const double *__restrict__ y, double *__restrict__ part) {
+    __shared__ double lds[kRedThreads / 64];
+    double s = 0.;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        s += x[i] * y[i];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// out[0] = sum of np partials (fixed order: deterministic)
+__global__ void __launch_bounds__(kRedThreads) sum_partials(int np, const double *__restrict__ part,
+                                                           double *__restrict__ out) {
+    __shared__ double lds[kRedThreads / 64];
+    double s = 0.;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+// FD increment sigma = sqrt(eps) * ||psi|| / ||v|| from two squared norms
+__global__ void __launch_bounds__(kRedThreads) jv_sigma(int np, const double *__restrict__ part_psi,
+                                                       const double *__restrict__ part_v, double *__restrict__ sig) {
+    __shared__ double lds[kRedThreads / 64];
+    double a = 0., b = 0.;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) { a += part_psi[i]; b += part_v[i]; }
+    a = block_sum(a, lds);
+    b = block_sum(b, lds);
+    if (threadIdx.x == 0) {
+        const double sq_eps = 1.4901161193847656e-08;  // sqrt(2^-52)
+        double np_ = sqrt(a), nv = sqrt(b);
+        if (np_ == 0.) np_ = 1.;
+        sig[0] = nv > 0. ? sq_eps * np_ / nv : 1.;
+    }
+}
+
+__global__ void jv_perturb(long n, const double *__restrict__ sig, double cj, const double *__restrict__ psi,
+                           const double *__restrict__ psip, const double *__restrict__ v, double *__restrict__ y,
+                           double *__restrict__ yp) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = sig[0];
+    y[i] = psi[i] + s * v[i];
+    yp[i] = psip[i] + cj * (s * v[i]);
+}
+
+__global__ void jv_finish(long n, const double *__restrict__ sig, const double *__restrict__ F1,
+                          const double *__restrict__ F0, double *__restrict__ Jv) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Jv[i] = (F1[i] - F0[i]) / sig[0];
+}
+
+static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+int dot_async(hipStream_t s, long n, const double *x, const double *y, double *part, double *out) {
+    dot_partial<<<kRedBlocks, kRedThreads, 0, s>>>(n, x, y, part);
+    sum_partials<<<1, kRedThreads, 0, s>>>(kRedBlocks, part, out);
+    FVP_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace fvp
+
+using namespace fvp;
+
+extern "C" int fvp_jv(fvp_ctx *c, double t, const double *psi, const double *psip, const double *F0, double c_j,
+                      const double *v, double *Jv) {
+    if (!c || !psi || !psip || !F0 || !v || !Jv) { set_error("null argument"); return -1; }
+    FVP_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const long N = c->N;
+    double *part = c->red.p;  // [0, 512) psi partials, [512, 1024) v partials, [1024] sigma
+    dot_partial<<<kRedBlocks, kRedThreads, 0, s>>>(N, psi, psi, part);
+    dot_partial<<<kRedBlocks, kRedThreads, 0, s>>>(N, v, v, part + kRedBlocks);
+    jv_sigma<<<1, kRedThreads, 0, s>>>(kRedBlocks, part, part + kRedBlocks, part + 2 * kRedBlocks);
+    jv_perturb<<<nblk(N, 256), 256, 0, s>>>(N, part + 2 * kRedBlocks, c_j, psi, psip, v, c->tmp1.p, c->tmp2.p);
+    FVP_HIP(hipGetLastError());
+    if (residual_impl(c, t, c->tmp1.p, c->tmp2.p, c->tmp3.p, c->averaging)) return -1;
+    jv_finish<<<nblk(N, 256), 256, 0, s>>>(N, part + 2 * kRedBlocks, c->tmp3.p, F0, Jv);
+    FVP_HIP(hipGetLastError());
+    return 0;
+}

@@ -1,0 +1,98 @@
+// Internal declarations of libfvp (context layout, error helpers, kernel launchers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fvp.h"
+
+namespace fvp {
+
+void set_error(const std::string &msg);
+
+constexpr int kMaxZones = 16;
+constexpr int kMaxTags = 64;
+
+// Zone parameters passed by value to kernels: K_xx, K_yy, K_zz, phi, alpha, n, m, S_r, S_s.
+struct ZoneTable {
+    int nz;
+    double p[kMaxZones][10];
+};
+enum { ZK_X = 0, ZK_Y, ZK_Z, Z_PHI, Z_ALPHA, Z_N, Z_M, Z_SR, Z_SS };
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int alloc(size_t count);
+    int upload(const T *h, size_t count);
+    void release();
+};
+
+}  // namespace fvp
+
+struct fvp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    int dim = 0, N = 0, T = 0, NE = 0, npe = 0, ned = 0, nz = 0, ntags = 0;
+    double rho0 = 1000.0;
+    fvp::ZoneTable zt{};
+    std::vector<int> bc_type;
+    std::vector<double> bc;  // ntags x FVP_BC_FIELDS
+
+    // element data (device): connectivity (int4, 2D pads the 4th slot with -1),
+    // zone index, conductivity multiplier, upwind bits, front-orientation bits
+    fvp::DevBuf<int4> conn;
+    fvp::DevBuf<uint8_t> ezone, emask, eorient;
+    fvp::DevBuf<double> kmul;
+    // node data
+    fvp::DevBuf<double> xyz;     // N x 3
+    fvp::DevBuf<double> invV;    // 1 / CV volume
+    fvp::DevBuf<uint8_t> zmask;  // bit z: node touches zone z
+    fvp::DevBuf<double> zw;      // nz x N SCV-volume fraction of zone z in CV i
+    fvp::DevBuf<double> kr;      // nz x N relative permeability per (zone, node)
+    fvp::DevBuf<double> cap;     // N capacity coefficient ahh
+    fvp::DevBuf<double> acc;     // N flux accumulator
+    // boundary flux: nodes with a flux condition and their rho0 * sum(q.n A)
+    int nb = 0;
+    fvp::DevBuf<int> bnode;
+    fvp::DevBuf<double> bval;
+    std::vector<int> bnode_h;
+    std::vector<double> barea_h;   // nb x ntags summed CV-face area per tag
+    std::vector<double> bnrm_h;    // nb x ntags x 3 summed area^2 * unit normal per tag
+    // Dirichlet rows
+    int ndir = 0;
+    fvp::DevBuf<int> dnode;
+    fvp::DevBuf<double> dval;
+    // time-step upwinding
+    int averaging = 1;             // 1: all edge weights 0.5
+    fvp::DevBuf<int> eedge;        // T x ned global edge of each local edge
+    fvp::DevBuf<int> slot_ptr;     // NE + 1
+    fvp::DevBuf<int> slot;         // T x ned (e*ned + j) sorted by (edge, element)
+    fvp::DevBuf<double> qface;     // T x ned averaged q.n per (element, local edge), back->front
+    fvp::DevBuf<double> aface;     // T x ned CV face area
+    fvp::DevBuf<double> eflux;     // NE
+    fvp::DevBuf<double> wf;        // NE front weight
+    // scratch for Jv / reductions
+    fvp::DevBuf<double> tmp1, tmp2, tmp3;
+    fvp::DevBuf<double> red;       // reduction partials
+    std::vector<double> red_h;
+};
+
+namespace fvp {
+int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, double *F, int averaging);
+int upload_bc_values(fvp_ctx *c);
+}  // namespace fvp
+
+#define FVP_HIP(call)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            fvp::set_error(std::string(#call) + ": " + hipGetErrorString(e_));                \
+            return -1;                                                                        \
+        }                                                                                     \
+    } while (0)

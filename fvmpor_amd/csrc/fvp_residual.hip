@@ -1,0 +1,332 @@
+// CVFE Richards residual on gfx950.
+//
+// The reference evaluates the residual as a chain of CSR SpMVs over per-CV-face
+// geometry tables plus ~13 elementwise passes over 36 M face arrays
+// (fvmpor_ODE_impl.cpp:62-115, fvmpor.h:646-945).  Here the CV-face geometry
+// is rebuilt per element from node coordinates: on a linear simplex the
+// median-dual face between nodes a and b inside element e has the area vector
+//     S_ab = |e| / (d+1) * (grad N_b - grad N_a)            (oriented a -> b)
+// which equals the reference's quad / segment normal times area
+// (cvface_shape.h:46-78, mesh.cpp:844-1010), so the Darcy flux across it is
+//     q . S_ab = |e| / (d+1) * (g_b - g_a),   g_k = grad N_k . q,
+// with q = -K (grad psi + e_gravity) constant on the element (fvmpor.h:681-717).
+// Kernels:
+//   vg_kernel      per node: van Genuchten Sw, dSw/dpsi, k_r for every zone the
+//                  CV touches, CV-averaged capacity ahh (fvmpor.h:874-945)
+//   elem_kernel    per element: gradient, 3 or 6 face fluxes, upwinded k_r,
+//                  scatter to the element's nodes
+//   finalize       F = acc / V - psi' ahh, boundary fluxes, Dirichlet rows
+//                  (fvmpor_ODE_impl.cpp:99-115)
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "fvp_internal.h"
+
+namespace fvp {
+
+// Host twin of the reference's saturation kernel (fvmpor.h:302-343), same operation order.
+__device__ __forceinline__ void vg_eval(double h, const double *zp, double &Sw, double &dSw, double &krw) {
+    if (h >= 0.) { Sw = 1.; dSw = 0.; krw = 1.; return; }
+    const double nVG = zp[Z_N], mVG = zp[Z_M], S_r = zp[Z_SR];
+    double d = pow(-zp[Z_ALPHA] * h, nVG);  // a = (alpha |h|)^n
+    double s = d + 1.;                      // b
+    double k = -1. / s;
+    d /= s;
+    s = pow(s, -mVG);                       // Se
+    d *= s;
+    d /= h;
+    d *= -(1 - S_r) * (nVG - 1);
+    k += 1.;
+    k = pow(k, mVG);
+    k -= 1.;
+    k = k * k;
+    k *= sqrt(s);
+    s *= (1 - S_r);
+    s += S_r;
+    Sw = s; dSw = d; krw = k;
+}
+
+__global__ void __launch_bounds__(256) vg_kernel(int N, ZoneTable zt, double rho0, const double *__restrict__ psi,
+                                                 const uint8_t *__restrict__ zmask, const double *__restrict__ zw,
+                                                 double *__restrict__ kr, double *__restrict__ cap) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double h = psi[i];
+    unsigned m = zmask[i];
+    double phi = 0., Sw = 0., dSw = 0., Ss = 0.;
+    for (int z = 0; z < zt.nz; ++z) {
+        if (!(m & (1u << z))) continue;
+        double w = zw[(size_t)z * N + i];
+        double s, d, k;
+        vg_eval(h, zt.p[z], s, d, k);
+        kr[(size_t)z * N + i] = k;
+        phi += zt.p[z][Z_PHI] * w;
+        Sw += s * w;
+        dSw += d * w;
+        Ss += zt.p[z][Z_SS] * w;
+    }
+    cap[i] = (phi * dSw + Ss * Sw) * rho0;
+}
+
+__device__ __forceinline__ void load_pt(const double *__restrict__ xyz, int i, double &x, double &y, double &z) {
+    x = xyz[3 * i]; y = xyz[3 * i + 1]; z = xyz[3 * i + 2];
+}
+
+// Local edge tables (mesh.cpp:409-500 element edge order).
+__constant__ int kTetA[6] = {0, 1, 2, 0, 1, 2};
+__constant__ int kTetB[6] = {1, 2, 0, 3, 3, 3};
+__constant__ int kTriA[3] = {0, 1, 2};
+__constant__ int kTriB[3] = {1, 2, 0};
+
+// Per-element face fluxes.  MODE 0: upwind k_r from emask; MODE 1: averaged
+// k_r (0.5 front + 0.5 back) and the averaged face fluxes are also written per
+// (element, local edge) in back->front orientation for the edge-flux sign test.
+template <int DIM, int MODE>
+__global__ void __launch_bounds__(256) elem_kernel(int T, int N, ZoneTable zt, double rho0,
+                                                   const int4 *__restrict__ conn, const uint8_t *__restrict__ ezone,
+                                                   const double *__restrict__ kmul, const uint8_t *__restrict__ emask,
+                                                   const uint8_t *__restrict__ eorient,
+                                                   const double *__restrict__ xyz, const double *__restrict__ psi,
+                                                   const double *__restrict__ kr, double *__restrict__ acc,
+                                                   double *__restrict__ qface, double *__restrict__ aface) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= T) return;
+    constexpr int NPE = DIM + 1;
+    constexpr int NED = DIM == 2 ? 3 : 6;
+    int4 c4 = conn[e];
+    int n[4] = {c4.x, c4.y, c4.z, c4.w};
+    double px[NPE], py[NPE], pz[NPE], h[NPE];
+#pragma unroll
+    for (int k = 0; k < NPE; ++k) {
+        load_pt(xyz, n[k], px[k], py[k], pz[k]);
+        h[k] = psi[n[k]];
+    }
+    const int z = ezone[e];
+    const double *zp = zt.p[z];
+    const double km = kmul[e];
+    double G[NPE][3];  // adjugate rows: grad N_k * det
+    double det;
+    if (DIM == 3) {
+        double ux = px[1] - px[0], uy = py[1] - py[0], uz = pz[1] - pz[0];
+        double vx = px[2] - px[0], vy = py[2] - py[0], vz = pz[2] - pz[0];
+        double wx = px[3] - px[0], wy = py[3] - py[0], wz = pz[3] - pz[0];
+        G[1][0] = vy * wz - vz * wy; G[1][1] = vz * wx - vx * wz; G[1][2] = vx * wy - vy * wx;
+        G[2][0] = wy * uz - wz * uy; G[2][1] = wz * ux - wx * uz; G[2][2] = wx * uy - wy * ux;
+        G[3][0] = uy * vz - uz * vy; G[3][1] = uz * vx - ux * vz; G[3][2] = ux * vy - uy * vx;
+        det = ux * G[1][0] + uy * G[1][1] + uz * G[1][2];
+    } else {
+        double ux = px[1] - px[0], uy = py[1] - py[0];
+        double vx = px[2] - px[0], vy = py[2] - py[0];
+        G[1][0] = vy; G[1][1] = -vx; G[1][2] = 0.;
+        G[2][0] = -uy; G[2][1] = ux; G[2][2] = 0.;
+        det = ux * vy - uy * vx;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) G[0][d] = -(G[1][d] + G[2][d] + (DIM == 3 ? G[NPE - 1][d] : 0.));
+    const double inv = 1.0 / det;
+    // gradient of psi
+    double gx = 0., gy = 0., gz = 0.;
+#pragma unroll
+    for (int k = 1; k < NPE; ++k) {
+        double dh = h[k] - h[0];
+        gx += dh * G[k][0]; gy += dh * G[k][1]; gz += dh * G[k][2];
+    }
+    gx *= inv; gy *= inv; gz *= inv;
+    // Darcy flux without k_r: q = -K (grad psi + e_gravity), gravity on y (2D) / z (3D)
+    double qx = gx * -(zp[ZK_X] * km);
+    double qy, qz;
+    if (DIM == 2) { qy = (gy + 1.) * -(zp[ZK_Y] * km); qz = 0.; }
+    else { qy = gy * -(zp[ZK_Y] * km); qz = (gz + 1.) * -(zp[ZK_Z] * km); }
+    double g[NPE];
+#pragma unroll
+    for (int k = 0; k < NPE; ++k) g[k] = G[k][0] * qx + G[k][1] * qy + G[k][2] * qz;
+    // |e|/(d+1) * (grad N_b - grad N_a) . q  =  sign(det) / (d! (d+1)) * (g_b - g_a)
+    const double sc = (det > 0 ? 1.0 : -1.0) / (DIM == 3 ? 24.0 : 6.0);
+    double f[NPE];
+#pragma unroll
+    for (int k = 0; k < NPE; ++k) f[k] = 0.;
+    const unsigned mask = MODE == 0 ? emask[e] : 0u;
+    const unsigned orient = MODE == 1 ? eorient[e] : 0u;
+#pragma unroll
+    for (int j = 0; j < NED; ++j) {
+        const int a = DIM == 3 ? kTetA[j] : kTriA[j];
+        const int b = DIM == 3 ? kTetB[j] : kTriB[j];
+        double flux = sc * (g[b] - g[a]);  // flow a -> b
+        double krf;
+        if (MODE == 0) {
+            int up = (mask >> j) & 1u ? n[a] : n[b];
+            krf = kr[(size_t)z * N + up];
+        } else {
+            const bool a_front = (orient >> j) & 1u;
+            double kf = kr[(size_t)z * N + (a_front ? n[a] : n[b])];
+            double kb = kr[(size_t)z * N + (a_front ? n[b] : n[a])];
+            krf = kf * 0.5;
+            krf += kb * 0.5;
+            double qn = krf * (a_front ? -flux : flux);  // back -> front
+            qface[(size_t)e * NED + j] = qn;
+            double sx = G[b][0] - G[a][0], sy = G[b][1] - G[a][1], sz = G[b][2] - G[a][2];
+            aface[(size_t)e * NED + j] = fabs(sc) * sqrt(sx * sx + sy * sy + sz * sz);
+        }
+        double M = rho0 * (krf * flux);
+        f[a] -= M;
+        f[b] += M;
+    }
+#pragma unroll
+    for (int k = 0; k < NPE; ++k) atomicAdd(&acc[n[k]], f[k]);
+}
+
+__global__ void boundary_kernel(int nb, const int *__restrict__ bnode, const double *__restrict__ bval,
+                                double *__restrict__ acc) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nb) acc[bnode[k]] -= bval[k];
+}
+
+__global__ void finalize_kernel(int N, const double *__restrict__ acc, const double *__restrict__ invV,
+                                const double *__restrict__ psip, const double *__restrict__ cap,
+                                double *__restrict__ F) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    F[i] = acc[i] * invV[i] - psip[i] * cap[i];
+}
+
+__global__ void dirichlet_kernel(int nd, const int *__restrict__ dnode, const double *__restrict__ dval,
+                                 const double *__restrict__ psi, double *__restrict__ F) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nd) {
+        int i = dnode[k];
+        F[i] = dval[k] - psi[i];
+    }
+}
+
+// edge flux = sum over the edge's CV faces of q.n / sum of their areas
+// (flux_lim_matrix, fvmpor.h:1033-1071), then the upwind test (fvmpor.h:783-794)
+__global__ void edge_flux_kernel(int NE, const int *__restrict__ slot_ptr, const int *__restrict__ slot,
+                                 const double *__restrict__ qface, const double *__restrict__ aface,
+                                 double *__restrict__ eflux, double *__restrict__ wf) {
+    int ed = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ed >= NE) return;
+    double tot = 0.;
+    for (int s = slot_ptr[ed]; s < slot_ptr[ed + 1]; ++s) tot += aface[slot[s]];
+    double w = 1. / tot;
+    double sum = 0.;
+    for (int s = slot_ptr[ed]; s < slot_ptr[ed + 1]; ++s) sum += w * qface[slot[s]];
+    eflux[ed] = sum;
+    wf[ed] = sum < 0. ? 1. : 0.;
+}
+
+// per-element upwind bits: bit j set when the first local node of edge j is upwind
+template <int DIM>
+__global__ void mask_kernel(int T, const int *__restrict__ eedge, const uint8_t *__restrict__ eorient,
+                            const double *__restrict__ wf, uint8_t *__restrict__ emask) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= T) return;
+    constexpr int NED = DIM == 2 ? 3 : 6;
+    unsigned o = eorient[e], m = 0;
+#pragma unroll
+    for (int j = 0; j < NED; ++j) {
+        bool front_up = wf[eedge[(size_t)e * NED + j]] == 1.0;
+        bool a_front = (o >> j) & 1u;
+        if (front_up == a_front) m |= 1u << j;
+    }
+    emask[e] = (uint8_t)m;
+}
+
+static inline int blocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, double *F, int averaging) {
+    (void)t;
+    hipStream_t s = c->stream;
+    const int N = c->N, T = c->T;
+    FVP_HIP(hipMemsetAsync(c->acc.p, 0, sizeof(double) * N, s));
+    vg_kernel<<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p, c->cap.p);
+    if (c->dim == 3) {
+        if (averaging)
+            elem_kernel<3, 1><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
+                                                           c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p,
+                                                           c->acc.p, c->qface.p, c->aface.p);
+        else
+            elem_kernel<3, 0><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
+                                                           c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p,
+                                                           c->acc.p, c->qface.p, c->aface.p);
+    } else {
+        if (averaging)
+            elem_kernel<2, 1><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
+                                                           c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p,
+                                                           c->acc.p, c->qface.p, c->aface.p);
+        else
+            elem_kernel<2, 0><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
+                                                           c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p,
+                                                           c->acc.p, c->qface.p, c->aface.p);
+    }
+    if (c->nb) boundary_kernel<<<blocks(c->nb, 256), 256, 0, s>>>(c->nb, c->bnode.p, c->bval.p, c->acc.p);
+    finalize_kernel<<<blocks(N, 256), 256, 0, s>>>(N, c->acc.p, c->invV.p, psip, c->cap.p, F);
+    if (c->ndir) dirichlet_kernel<<<blocks(c->ndir, 256), 256, 0, s>>>(c->ndir, c->dnode.p, c->dval.p, psi, F);
+    FVP_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace fvp
+
+using namespace fvp;
+
+extern "C" int fvp_residual(fvp_ctx *c, double t, const double *psi, const double *psip, double *F) {
+    if (!c || !psi || !psip || !F) { set_error("null argument"); return -1; }
+    FVP_HIP(hipSetDevice(c->device));
+    return residual_impl(c, t, psi, psip, F, c->averaging);
+}
+
+extern "C" int fvp_begin_step(fvp_ctx *c, double t, const double *psi, const double *psip) {
+    if (!c || !psi || !psip) { set_error("null argument"); return -1; }
+    FVP_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    // averaged-weight evaluation writes the per-face fluxes; its F is discarded
+    if (residual_impl(c, t, psi, psip, c->tmp3.p, 1)) return -1;
+    edge_flux_kernel<<<blocks(c->NE, 256), 256, 0, s>>>(c->NE, c->slot_ptr.p, c->slot.p, c->qface.p, c->aface.p,
+                                                        c->eflux.p, c->wf.p);
+    if (c->dim == 3)
+        mask_kernel<3><<<blocks(c->T, 256), 256, 0, s>>>(c->T, c->eedge.p, c->eorient.p, c->wf.p, c->emask.p);
+    else
+        mask_kernel<2><<<blocks(c->T, 256), 256, 0, s>>>(c->T, c->eedge.p, c->eorient.p, c->wf.p, c->emask.p);
+    FVP_HIP(hipGetLastError());
+    c->averaging = 0;
+    return 0;
+}
+
+extern "C" int fvp_get_upwind(fvp_ctx *c, double *w_front, double *edge_flux) {
+    if (!c) { set_error("null context"); return -1; }
+    FVP_HIP(hipSetDevice(c->device));
+    FVP_HIP(hipStreamSynchronize(c->stream));
+    if (w_front) FVP_HIP(hipMemcpy(w_front, c->wf.p, sizeof(double) * c->NE, hipMemcpyDeviceToHost));
+    if (edge_flux) FVP_HIP(hipMemcpy(edge_flux, c->eflux.p, sizeof(double) * c->NE, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int fvp_set_upwind(fvp_ctx *c, const double *w_front) {
+    if (!c || !w_front) { set_error("null argument"); return -1; }
+    int halves = 0;
+    for (int i = 0; i < c->NE; ++i) {
+        double w = w_front[i];
+        if (w == 0.5) ++halves;
+        else if (w != 0.0 && w != 1.0) { set_error("edge weights must be 0, 0.5 or 1"); return -1; }
+    }
+    if (halves && halves != c->NE) { set_error("averaged and upwind edge weights cannot be mixed"); return -1; }
+    FVP_HIP(hipSetDevice(c->device));
+    FVP_HIP(hipStreamSynchronize(c->stream));
+    FVP_HIP(hipMemcpy(c->wf.p, w_front, sizeof(double) * c->NE, hipMemcpyHostToDevice));
+    if (halves) { c->averaging = 1; return 0; }
+    if (c->dim == 3)
+        mask_kernel<3><<<blocks(c->T, 256), 256, 0, c->stream>>>(c->T, c->eedge.p, c->eorient.p, c->wf.p, c->emask.p);
+    else
+        mask_kernel<2><<<blocks(c->T, 256), 256, 0, c->stream>>>(c->T, c->eedge.p, c->eorient.p, c->wf.p, c->emask.p);
+    FVP_HIP(hipGetLastError());
+    c->averaging = 0;
+    return 0;
+}
+
+extern "C" int fvp_capacity(fvp_ctx *c, double *ahh) {
+    if (!c || !ahh) { set_error("null argument"); return -1; }
+    FVP_HIP(hipSetDevice(c->device));
+    FVP_HIP(hipMemcpyAsync(ahh, c->cap.p, sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}

@@ -1,0 +1,378 @@
+// Context construction: mesh ingest and the per-node / per-element tables the
+// residual kernels read.  This replaces the host-side set-up of the reference
+// (mesh::Mesh construction, mesh.cpp:70-80, 369-703, 786-1154;
+// VarSatPhysicsImpl::initialise_vectors / initialise_shape_functions,
+// fvmpor.h:346-644, 947-1124) with compact arrays: CV-face geometry is not
+// stored, the kernels rebuild it from node coordinates on every call.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+
+#include "fvp_internal.h"
+
+namespace fvp {
+
+static std::mutex g_err_mu;
+static std::string g_err;
+
+void set_error(const std::string &msg) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_err = msg;
+}
+
+template <class T>
+int DevBuf<T>::alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) return 0;
+    hipError_t e = hipMalloc(&p, sizeof(T) * count);
+    if (e != hipSuccess) {
+        p = nullptr;
+        set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return -1;
+    }
+    return 0;
+}
+template <class T>
+int DevBuf<T>::upload(const T *h, size_t count) {
+    if (alloc(count)) return -1;
+    if (count == 0) return 0;
+    hipError_t e = hipMemcpy(p, h, sizeof(T) * count, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_error(std::string("hipMemcpy: ") + hipGetErrorString(e));
+        return -1;
+    }
+    return 0;
+}
+template <class T>
+void DevBuf<T>::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+}
+template struct DevBuf<int>;
+template struct DevBuf<int4>;
+template struct DevBuf<uint8_t>;
+template struct DevBuf<double>;
+
+static const int TRI_EDGE[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+static const int TET_EDGE[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+static const int TRI_FACE[3][3] = {{0, 1, -1}, {1, 2, -1}, {2, 0, -1}};
+static const int TET_FACE[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+
+int upload_bc_values(fvp_ctx *c) {
+    // rho0 * sum over the node's boundary CV faces of q.n A (fvmpor.h:721-753)
+    std::vector<double> val(c->nb, 0.0);
+    for (int k = 0; k < c->nb; ++k) {
+        double s = 0.0;
+        for (int tag = 0; tag < c->ntags; ++tag) {
+            const double *b = &c->bc[tag * FVP_BC_FIELDS];
+            double a = c->barea_h[(size_t)k * c->ntags + tag];
+            if (a == 0.0) continue;
+            switch (c->bc_type[tag]) {
+            case 3:
+            case 7: s += b[0] * a; break;
+            case 6: {
+                const double *nv = &c->bnrm_h[((size_t)k * c->ntags + tag) * 3];
+                s += nv[0] * b[3] + nv[1] * b[4] + nv[2] * b[5];
+                break;
+            }
+            default: break;
+            }
+        }
+        val[k] = c->rho0 * s;
+    }
+    return c->bval.upload(val.data(), val.size());
+}
+
+}  // namespace fvp
+
+using namespace fvp;
+
+extern "C" const char *fvp_last_error(void) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    return g_err.c_str();
+}
+
+static int validate(const fvp_problem *p) {
+    if (!p) { set_error("null problem"); return -1; }
+    if (p->dim != 2 && p->dim != 3) { set_error("dim must be 2 or 3"); return -1; }
+    if (p->n_nodes <= 0 || p->n_elements <= 0) { set_error("empty mesh"); return -1; }
+    if (!p->xyz || !p->conn || !p->face_tags || !p->elem_zone || !p->zones) {
+        set_error("missing mesh arrays"); return -1;
+    }
+    if (p->n_zones <= 0 || p->n_zones > kMaxZones) { set_error("n_zones must be 1..16"); return -1; }
+    if (p->n_tags < 0 || p->n_tags > kMaxTags) { set_error("n_tags must be 0..64"); return -1; }
+    int npe = p->dim + 1;
+    for (long i = 0; i < (long)p->n_elements * npe; ++i)
+        if (p->conn[i] < 0 || p->conn[i] >= p->n_nodes) { set_error("connectivity out of range"); return -1; }
+    for (int e = 0; e < p->n_elements; ++e)
+        if (p->elem_zone[e] < 0 || p->elem_zone[e] >= p->n_zones) { set_error("element zone out of range"); return -1; }
+    for (long i = 0; i < (long)p->n_elements * npe; ++i)
+        if (p->face_tags[i] < 0 || (p->face_tags[i] > 0 && p->face_tags[i] >= p->n_tags)) {
+            set_error("face tag without a boundary condition"); return -1;
+        }
+    return 0;
+}
+
+extern "C" int fvp_create(const fvp_problem *p, int device, fvp_ctx **out) {
+    if (!out) { set_error("null output"); return -1; }
+    *out = nullptr;
+    if (validate(p)) return -1;
+    FVP_HIP(hipSetDevice(device));
+    fvp_ctx *c = new fvp_ctx();
+    c->device = device;
+    c->dim = p->dim; c->N = p->n_nodes; c->T = p->n_elements;
+    c->npe = p->dim + 1; c->ned = p->dim == 2 ? 3 : 6;
+    c->nz = p->n_zones; c->ntags = p->n_tags; c->rho0 = p->rho0;
+    const int N = c->N, T = c->T, npe = c->npe, ned = c->ned, dim = c->dim;
+    auto fail = [&]() { fvp_destroy(c); return -1; };
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("hipStreamCreate failed"); return fail();
+    }
+    c->stream = c->own_stream;
+
+    c->zt.nz = c->nz;
+    for (int z = 0; z < c->nz; ++z) {
+        const double *r = p->zones + z * FVP_ZONE_FIELDS;
+        double *d = c->zt.p[z];
+        d[ZK_X] = r[0]; d[ZK_Y] = r[1]; d[ZK_Z] = r[2]; d[Z_PHI] = r[3];
+        d[Z_ALPHA] = r[4]; d[Z_N] = r[5]; d[Z_M] = 1. - 1. / r[5]; d[Z_SR] = r[6]; d[Z_SS] = r[7];
+    }
+    c->bc_type.assign(p->bc_type, p->bc_type + c->ntags);
+    c->bc.assign(p->bc, p->bc + (size_t)c->ntags * FVP_BC_FIELDS);
+
+    // ---- element arrays
+    std::vector<int4> conn4(T);
+    std::vector<uint8_t> ez(T);
+    std::vector<double> km(T, 1.0);
+    for (int e = 0; e < T; ++e) {
+        const int *q = p->conn + (size_t)e * npe;
+        conn4[e] = make_int4(q[0], q[1], q[2], dim == 3 ? q[3] : -1);
+        ez[e] = (uint8_t)p->elem_zone[e];
+        if (p->elem_kmul) km[e] = p->elem_kmul[e];
+    }
+    if (c->conn.upload(conn4.data(), T) || c->ezone.upload(ez.data(), T) || c->kmul.upload(km.data(), T))
+        return fail();
+    if (c->xyz.upload(p->xyz, (size_t)N * 3)) return fail();
+
+    // ---- volumes: |T|/(d+1) per SCV (exact for simplices), zone fractions
+    std::vector<double> V(N, 0.0);
+    std::vector<double> evol(T);
+    for (int e = 0; e < T; ++e) {
+        const int *q = p->conn + (size_t)e * npe;
+        const double *a = p->xyz + 3 * q[0], *b = p->xyz + 3 * q[1], *cc = p->xyz + 3 * q[2];
+        double vol;
+        if (dim == 2) {
+            vol = 0.5 * std::fabs((b[0] - a[0]) * (cc[1] - a[1]) - (b[1] - a[1]) * (cc[0] - a[0]));
+        } else {
+            const double *d = p->xyz + 3 * q[3];
+            double ux = b[0] - a[0], uy = b[1] - a[1], uz = b[2] - a[2];
+            double vx = cc[0] - a[0], vy = cc[1] - a[1], vz = cc[2] - a[2];
+            double wx = d[0] - a[0], wy = d[1] - a[1], wz = d[2] - a[2];
+            vol = std::fabs(ux * (vy * wz - vz * wy) - uy * (vx * wz - vz * wx) + uz * (vx * wy - vy * wx)) / 6.0;
+        }
+        if (!(vol > 0.0)) { set_error("degenerate element " + std::to_string(e)); return fail(); }
+        evol[e] = vol / npe;
+        for (int k = 0; k < npe; ++k) V[q[k]] += evol[e];
+    }
+    std::vector<double> invV(N);
+    for (int i = 0; i < N; ++i) {
+        if (!(V[i] > 0.0)) { set_error("node " + std::to_string(i) + " is in no element"); return fail(); }
+        invV[i] = 1.0 / V[i];
+    }
+    std::vector<double> zw((size_t)c->nz * N, 0.0);
+    std::vector<uint8_t> zmask(N, 0);
+    for (int e = 0; e < T; ++e) {
+        const int *q = p->conn + (size_t)e * npe;
+        int z = p->elem_zone[e];
+        for (int k = 0; k < npe; ++k) {
+            zw[(size_t)z * N + q[k]] += evol[e] / V[q[k]];
+            zmask[q[k]] |= (uint8_t)(1u << (z & 7));
+        }
+    }
+    if (c->nz > 8) { set_error("more than 8 zones need a wider zone mask"); return fail(); }
+    if (c->invV.upload(invV.data(), N) || c->zw.upload(zw.data(), zw.size()) ||
+        c->zmask.upload(zmask.data(), N))
+        return fail();
+    if (c->kr.alloc((size_t)c->nz * N) || c->cap.alloc(N) || c->acc.alloc(N)) return fail();
+    FVP_HIP(hipMemset(c->kr.p, 0, sizeof(double) * c->kr.n));
+
+    // ---- edges: sort (min, max, insertion index); first insertion fixes front/back
+    {
+        const size_t ns = (size_t)T * ned;
+        std::vector<uint64_t> key(ns);
+        std::vector<uint32_t> idx(ns);
+        const int(*tab)[2] = dim == 2 ? TRI_EDGE : TET_EDGE;
+        for (int e = 0; e < T; ++e)
+            for (int j = 0; j < ned; ++j) {
+                int n0 = p->conn[(size_t)e * npe + tab[j][0]], n1 = p->conn[(size_t)e * npe + tab[j][1]];
+                uint32_t lo = (uint32_t)std::min(n0, n1), hi = (uint32_t)std::max(n0, n1);
+                key[(size_t)e * ned + j] = ((uint64_t)lo << 32) | hi;
+                idx[(size_t)e * ned + j] = (uint32_t)((size_t)e * ned + j);
+            }
+        std::vector<uint32_t> order(ns);
+        std::iota(order.begin(), order.end(), 0u);
+        std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+            return key[a] != key[b] ? key[a] < key[b] : a < b;
+        });
+        std::vector<int> eedge(ns), slot(ns), slot_ptr;
+        std::vector<uint8_t> orient(T, 0);
+        slot_ptr.reserve(ns / 4 + 2);
+        int ne = -1;
+        int front = -1;
+        for (size_t r = 0; r < ns; ++r) {
+            uint32_t s = order[r];
+            if (r == 0 || key[s] != key[order[r - 1]]) {
+                ++ne;
+                slot_ptr.push_back((int)r);
+                int e = (int)(s / ned), j = (int)(s % ned);
+                front = p->conn[(size_t)e * npe + tab[j][0]];  // first inserter's first node
+            }
+            int e = (int)(s / ned), j = (int)(s % ned);
+            eedge[s] = ne;
+            slot[r] = (int)s;
+            if (p->conn[(size_t)e * npe + tab[j][0]] == front) orient[e] |= (uint8_t)(1u << j);
+        }
+        slot_ptr.push_back((int)ns);
+        c->NE = ne + 1;
+        if (c->eedge.upload(eedge.data(), ns) || c->slot.upload(slot.data(), ns) ||
+            c->slot_ptr.upload(slot_ptr.data(), slot_ptr.size()) || c->eorient.upload(orient.data(), T))
+            return fail();
+    }
+    if (c->emask.alloc(T) || c->qface.alloc((size_t)T * ned) || c->aface.alloc((size_t)T * ned) ||
+        c->eflux.alloc(c->NE) || c->wf.alloc(c->NE))
+        return fail();
+    FVP_HIP(hipMemset(c->emask.p, 0, T));
+    FVP_HIP(hipMemset(c->eflux.p, 0, sizeof(double) * c->NE));
+    {
+        std::vector<double> half(c->NE, 0.5);
+        FVP_HIP(hipMemcpy(c->wf.p, half.data(), sizeof(double) * c->NE, hipMemcpyHostToDevice));
+    }
+    c->averaging = 1;
+
+    // ---- boundary CV faces aggregated per (node, tag)
+    {
+        std::vector<int> slot_of(N, -1);
+        const int(*ftab)[3] = dim == 2 ? TRI_FACE : TET_FACE;
+        const int nfe = npe, nfn = dim;
+        for (int e = 0; e < T; ++e)
+            for (int f = 0; f < nfe; ++f) {
+                int tag = p->face_tags[(size_t)e * npe + f];
+                if (!tag) continue;
+                int fn[3];
+                for (int q = 0; q < nfn; ++q) fn[q] = p->conn[(size_t)e * npe + ftab[f][q]];
+                // face area and outward unit normal
+                double A, nx, ny, nzv;
+                const double *a = p->xyz + 3 * fn[0], *b = p->xyz + 3 * fn[1];
+                if (dim == 2) {
+                    double dx = b[0] - a[0], dy = b[1] - a[1];
+                    A = std::sqrt(dx * dx + dy * dy);
+                    nx = dy / A; ny = -dx / A; nzv = 0;
+                } else {
+                    const double *cc = p->xyz + 3 * fn[2];
+                    double ux = b[0] - a[0], uy = b[1] - a[1], uz = b[2] - a[2];
+                    double vx = cc[0] - a[0], vy = cc[1] - a[1], vz = cc[2] - a[2];
+                    double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
+                    double l = std::sqrt(cx * cx + cy * cy + cz * cz);
+                    A = 0.5 * l; nx = cx / l; ny = cy / l; nzv = cz / l;
+                }
+                // orient outwards: away from the element centroid
+                double ce[3] = {0, 0, 0}, cf[3] = {0, 0, 0};
+                for (int k = 0; k < npe; ++k)
+                    for (int d = 0; d < 3; ++d) ce[d] += p->xyz[3 * p->conn[(size_t)e * npe + k] + d] / npe;
+                for (int k = 0; k < nfn; ++k)
+                    for (int d = 0; d < 3; ++d) cf[d] += p->xyz[3 * fn[k] + d] / nfn;
+                if (nx * (cf[0] - ce[0]) + ny * (cf[1] - ce[1]) + nzv * (cf[2] - ce[2]) < 0) {
+                    nx = -nx; ny = -ny; nzv = -nzv;
+                }
+                // each face node owns 1/dim of the face: a half segment (2D) or a third (3D)
+                double Ai = A / nfn;
+                for (int k = 0; k < nfn; ++k) {
+                    int i = fn[k];
+                    if (slot_of[i] < 0) {
+                        slot_of[i] = c->nb++;
+                        c->bnode_h.push_back(i);
+                        c->barea_h.resize((size_t)c->nb * c->ntags, 0.0);
+                        c->bnrm_h.resize((size_t)c->nb * c->ntags * 3, 0.0);
+                    }
+                    size_t s = (size_t)slot_of[i] * c->ntags + tag;
+                    c->barea_h[s] += Ai;
+                    c->bnrm_h[3 * s] += Ai * Ai * nx;
+                    c->bnrm_h[3 * s + 1] += Ai * Ai * ny;
+                    c->bnrm_h[3 * s + 2] += Ai * Ai * nzv;
+                }
+            }
+        if (c->bnode.upload(c->bnode_h.data(), c->nb)) return fail();
+        if (upload_bc_values(c)) return fail();
+    }
+
+    // ---- Dirichlet rows (fvmpor.h:404-451)
+    {
+        std::vector<int> dn;
+        std::vector<double> dv;
+        if (p->node_tag_ptr && p->node_tags) {
+            for (int i = 0; i < N; ++i) {
+                int dtag = 0;
+                for (int k = p->node_tag_ptr[i]; k < p->node_tag_ptr[i + 1]; ++k) {
+                    int tag = p->node_tags[k];
+                    if (tag > 0 && tag < c->ntags && (c->bc_type[tag] == 1 || c->bc_type[tag] == 4)) dtag = tag;
+                }
+                if (!dtag) continue;
+                const double *b = &c->bc[dtag * FVP_BC_FIELDS];
+                double el = dim == 2 ? p->xyz[3 * i + 1] : p->xyz[3 * i + 2];
+                dn.push_back(i);
+                dv.push_back(c->bc_type[dtag] == 1 ? b[0] : (1. + b[2]) * (b[1] - el));
+            }
+        }
+        c->ndir = (int)dn.size();
+        if (c->dnode.upload(dn.data(), dn.size()) || c->dval.upload(dv.data(), dv.size())) return fail();
+    }
+    if (c->tmp1.alloc(N) || c->tmp2.alloc(N) || c->tmp3.alloc(N) || c->red.alloc(4096)) return fail();
+    c->red_h.resize(4096);
+    *out = c;
+    return 0;
+}
+
+extern "C" int fvp_destroy(fvp_ctx *c) {
+    if (!c) return 0;
+    (void)hipSetDevice(c->device);
+    if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
+    c->conn.release(); c->ezone.release(); c->emask.release(); c->eorient.release(); c->kmul.release();
+    c->xyz.release(); c->invV.release(); c->zmask.release(); c->zw.release(); c->kr.release();
+    c->cap.release(); c->acc.release(); c->bnode.release(); c->bval.release(); c->dnode.release();
+    c->dval.release(); c->eedge.release(); c->slot_ptr.release(); c->slot.release(); c->qface.release();
+    c->aface.release(); c->eflux.release(); c->wf.release(); c->tmp1.release(); c->tmp2.release();
+    c->tmp3.release(); c->red.release();
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return 0;
+}
+
+extern "C" int fvp_sizes(const fvp_ctx *c, int *s) {
+    if (!c || !s) { set_error("null argument"); return -1; }
+    s[0] = c->N; s[1] = c->T; s[2] = c->NE; s[3] = c->ndir; s[4] = c->dim; s[5] = c->nz;
+    return 0;
+}
+
+extern "C" int fvp_set_stream(fvp_ctx *c, void *stream) {
+    if (!c) { set_error("null context"); return -1; }
+    c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    return 0;
+}
+
+extern "C" int fvp_synchronize(fvp_ctx *c) {
+    if (!c) { set_error("null context"); return -1; }
+    FVP_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int fvp_set_bc(fvp_ctx *c, const double *bc) {
+    if (!c || !bc) { set_error("null argument"); return -1; }
+    FVP_HIP(hipSetDevice(c->device));
+    FVP_HIP(hipStreamSynchronize(c->stream));
+    c->bc.assign(bc, bc + (size_t)c->ntags * FVP_BC_FIELDS);
+    return upload_bc_values(c);
+}

@@ -1,0 +1,168 @@
+"""ctypes binding of libfvp.so (the C ABI declared in include/fvp.h).
+
+Device vectors are passed as raw HIP device pointers; callers holding torch
+tensors pass ``tensor.data_ptr()``.  The library must be built (``make -C
+fvmpor_amd/csrc``); importing this module never falls back to another
+implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfvp.so")
+_LIB = None
+
+ZONE_FIELDS = 8
+BC_FIELDS = 6
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_dbl_p = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes) for every symbol in include/fvp.h
+SIGNATURES = {
+    "fvp_last_error": (ctypes.c_char_p, []),
+    "fvp_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "fvp_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "fvp_sizes": (ctypes.c_int, [ctypes.c_void_p, _c_int_p]),
+    "fvp_set_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "fvp_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "fvp_set_bc": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p]),
+    "fvp_residual": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_void_p]),
+    "fvp_begin_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
+    "fvp_get_upwind": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fvp_set_upwind": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p]),
+    "fvp_capacity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "fvp_jv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                              ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
+}
+
+
+class FvpError(RuntimeError):
+    pass
+
+
+class _Problem(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int), ("n_nodes", ctypes.c_int), ("n_elements", ctypes.c_int),
+        ("xyz", _c_dbl_p), ("conn", _c_int_p), ("face_tags", _c_int_p),
+        ("node_tag_ptr", _c_int_p), ("node_tags", _c_int_p), ("elem_zone", _c_int_p),
+        ("elem_kmul", _c_dbl_p), ("n_zones", ctypes.c_int), ("zones", _c_dbl_p),
+        ("n_tags", ctypes.c_int), ("bc_type", _c_int_p), ("bc", _c_dbl_p), ("rho0", ctypes.c_double),
+    ]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise FvpError(f"{LIB_PATH} is not built (run make -C fvmpor_amd/csrc)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc: int, what: str) -> int:
+    if rc < 0:
+        raise FvpError(f"{what} failed: {lib().fvp_last_error().decode()}")
+    return rc
+
+
+def _ptr(x):
+    """Device pointer of a torch tensor (or an int address)."""
+    if isinstance(x, int):
+        return ctypes.c_void_p(x)
+    return ctypes.c_void_p(x.data_ptr())
+
+
+class Context:
+    """One device-resident CVFE problem (fvp_create / fvp_destroy)."""
+
+    def __init__(self, problem, device: int = 0):
+        m = problem.mesh
+        self._keep = dict(
+            xyz=np.ascontiguousarray(m.xyz, dtype=np.float64),
+            conn=np.ascontiguousarray(m.conn, dtype=np.int32),
+            ft=np.ascontiguousarray(m.face_tags, dtype=np.int32),
+            ntp=np.ascontiguousarray(m.node_tag_ptr, dtype=np.int32),
+            nt=np.ascontiguousarray(m.node_tags if len(m.node_tags) else np.zeros(1), dtype=np.int32),
+            ez=np.ascontiguousarray(problem.ezone, dtype=np.int32),
+            km=np.ascontiguousarray(problem.ekmul, dtype=np.float64),
+            zones=np.ascontiguousarray(problem.zones, dtype=np.float64),
+            bct=np.ascontiguousarray(problem.bc_type, dtype=np.int32),
+            bc=np.ascontiguousarray(problem.bc, dtype=np.float64),
+        )
+        k = self._keep
+        P = _Problem()
+        P.dim = m.dim
+        P.n_nodes = k["xyz"].shape[0]
+        P.n_elements = k["conn"].shape[0]
+        P.xyz = k["xyz"].ctypes.data_as(_c_dbl_p)
+        P.conn = k["conn"].ctypes.data_as(_c_int_p)
+        P.face_tags = k["ft"].ctypes.data_as(_c_int_p)
+        P.node_tag_ptr = k["ntp"].ctypes.data_as(_c_int_p)
+        P.node_tags = k["nt"].ctypes.data_as(_c_int_p)
+        P.elem_zone = k["ez"].ctypes.data_as(_c_int_p)
+        P.elem_kmul = k["km"].ctypes.data_as(_c_dbl_p)
+        P.n_zones = k["zones"].shape[0]
+        P.zones = k["zones"].ctypes.data_as(_c_dbl_p)
+        P.n_tags = k["bct"].shape[0]
+        P.bc_type = k["bct"].ctypes.data_as(_c_int_p)
+        P.bc = k["bc"].ctypes.data_as(_c_dbl_p)
+        P.rho0 = float(problem.rho0)
+        h = ctypes.c_void_p()
+        _check(lib().fvp_create(ctypes.byref(P), int(device), ctypes.byref(h)), "fvp_create")
+        self.h = h
+        s = (ctypes.c_int * 6)()
+        _check(lib().fvp_sizes(self.h, s), "fvp_sizes")
+        self.N, self.T, self.NE, self.ndir, self.dim, self.nz = list(s)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fvp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_stream(self, stream_ptr: int):
+        _check(lib().fvp_set_stream(self.h, ctypes.c_void_p(stream_ptr)), "fvp_set_stream")
+
+    def synchronize(self):
+        _check(lib().fvp_synchronize(self.h), "fvp_synchronize")
+
+    def residual(self, t, psi, psip, F):
+        _check(lib().fvp_residual(self.h, float(t), _ptr(psi), _ptr(psip), _ptr(F)), "fvp_residual")
+
+    def begin_step(self, t, psi, psip):
+        _check(lib().fvp_begin_step(self.h, float(t), _ptr(psi), _ptr(psip)), "fvp_begin_step")
+
+    def get_upwind(self):
+        wf = np.zeros(self.NE)
+        fl = np.zeros(self.NE)
+        _check(lib().fvp_get_upwind(self.h, wf.ctypes.data_as(ctypes.c_void_p),
+                                    fl.ctypes.data_as(ctypes.c_void_p)), "fvp_get_upwind")
+        return wf, fl
+
+    def set_upwind(self, wf):
+        wf = np.ascontiguousarray(wf, dtype=np.float64)
+        _check(lib().fvp_set_upwind(self.h, wf.ctypes.data_as(_c_dbl_p)), "fvp_set_upwind")
+
+    def set_bc(self, bc):
+        bc = np.ascontiguousarray(bc, dtype=np.float64)
+        _check(lib().fvp_set_bc(self.h, bc.ctypes.data_as(_c_dbl_p)), "fvp_set_bc")
+
+    def capacity(self, out):
+        _check(lib().fvp_capacity(self.h, _ptr(out)), "fvp_capacity")
+
+    def jv(self, t, psi, psip, F0, cj, v, Jv):
+        _check(lib().fvp_jv(self.h, float(t), _ptr(psi), _ptr(psip), _ptr(F0), float(cj), _ptr(v), _ptr(Jv)),
+               "fvp_jv")

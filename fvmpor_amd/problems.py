@@ -1,0 +1,113 @@
+"""Problem definitions: soil zones, boundary conditions and initial heads.
+
+These mirror the reference's problem headers (``cassion.h:9-117``,
+``vs.h:8-87``) and the workloads of the 2D slice and 3D box runs.
+
+Zone parameter rows (``ZONE_FIELDS``): K_xx, K_yy, K_zz, phi, alpha_vG, n_vG,
+S_r, S_s.  ``m_vG = 1 - 1/n_vG`` is derived where it is used (cassion.h:50).
+Boundary rows (``BC_FIELDS``): value, b, eta, qx, qy, qz, indexed by the
+boundary tag; types follow definitions.h:114-146 (1 Dirichlet, 3 prescribed
+flux, 4 hydrostatic Dirichlet, 6 directional flux, 7 seepage, 8 shore).
+Fluxes are outward-positive: infiltration is negative (cassion.h:115).
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from .meshgen import Mesh, box_mesh_2d, box_mesh_3d, gaussian_field
+
+ZONE_FIELDS = ("Kxx", "Kyy", "Kzz", "phi", "alpha", "n", "Sr", "Ss")
+BC_FIELDS = ("value", "b", "eta", "qx", "qy", "qz")
+
+# van Genuchten classes (Carsel & Parrish 1988): alpha [1/m], n, theta_r, theta_s
+SOIL_CLASSES = {
+    "clay": (0.8, 1.09, 0.068, 0.38),
+    "loam": (3.6, 1.56, 0.078, 0.43),
+    "sandy_loam": (7.5, 1.89, 0.065, 0.41),
+    "sand": (14.5, 2.68, 0.045, 0.43),
+}
+
+
+@dataclasses.dataclass
+class Problem:
+    name: str
+    mesh: Mesh
+    zones: np.ndarray      # (nz, 8)
+    ezone: np.ndarray      # (T,) int32
+    ekmul: np.ndarray      # (T,) float64
+    bc_type: np.ndarray    # (ntags,) int32
+    bc: np.ndarray         # (ntags, 6)
+    rho0: float
+    psi0: np.ndarray       # (N,)
+    t_end: float
+    time_unit: str
+
+
+def _zone_row(K, phi, alpha, n, Sr, Ss=0.0):
+    return [K, K, K, phi, alpha, n, Sr, Ss]
+
+
+def cassion_problem(mesh: Mesh) -> Problem:
+    """The reference's cassion problem (cassion.h): four zones, SI units."""
+    rho0, g, mu = 1000.0, 9.80665, 1e-3
+    f = rho0 * g / mu
+    zones = np.array([
+        _zone_row(f * 9.33e-12, 0.368, 3.34, 1.982, 0.2771),
+        _zone_row(f * 5.55e-12, 0.3510, 3.63, 1.632, 0.2806),
+        _zone_row(f * 4.898e-12, 0.3250, 3.45, 1.573, 0.2643),
+        _zone_row(f * 4.898e-11, 0.3250, 3.45, 1.573, 0.2643),
+    ])
+    bc_type = np.array([0, 3, 3], dtype=np.int32)
+    bc = np.zeros((3, 6))
+    bc[2, 0] = -2.3148e-7
+    return Problem("cassion", mesh, zones, mesh.zone.astype(np.int32),
+                   np.ones(mesh.n_elements), bc_type, bc, rho0,
+                   np.full(mesh.n_nodes, -7.34), 2.5e6, "s")
+
+
+def slice2d_problem(nx=101, ny=41, seed=1) -> Problem:
+    """2D vertical slice 10 m x 5 m, homogeneous loam, hydrostatic psi = 1 - y,
+    0.05 m/day infiltration on the top strip x in [4, 6] m, to t = 1 day."""
+    mesh = box_mesh_2d(nx, ny, 10.0, 5.0, seed=seed)
+    alpha, n, tr, ts = SOIL_CLASSES["loam"]
+    zones = np.array([_zone_row(0.25, ts, alpha, n, tr / ts, 1e-5)])
+    bc_type = np.array([0, 3, 3], dtype=np.int32)
+    bc = np.zeros((3, 6))
+    bc[2, 0] = -0.05
+    psi0 = 1.0 - mesh.xyz[:, 1]
+    return Problem("slice2d", mesh, zones, np.zeros(mesh.n_elements, dtype=np.int32),
+                   np.ones(mesh.n_elements), bc_type, bc, 1000.0, psi0, 1.0, "day")
+
+
+def box3d_problem(nx=160, ny=160, nz=40, seed_mesh=2, seed_field=3, seed_layers=4,
+                  sigma_lnK=1.0, mean_lnK=float(np.log(0.5)), rain=0.02, mesh=None) -> Problem:
+    """3D box 100 x 100 x 20 m, log-normal Ks (exponential covariance, 10 m / 10 m /
+    2 m correlation lengths), four van Genuchten classes in random horizontal
+    layers, water table at z = 5 m, rain on the whole top face, to t = 1 day."""
+    lx, ly, lz = 100.0, 100.0, 20.0
+    if mesh is None:
+        mesh = box_mesh_3d(nx, ny, nz, lx, ly, lz, seed=seed_mesh)
+    p = mesh.xyz[mesh.conn]
+    cen = p.mean(axis=1)
+    gshape = (nx, ny, nz)
+    spacing = (lx / (nx - 1), ly / (ny - 1), lz / (nz - 1))
+    lnK = gaussian_field(gshape, spacing, (10.0, 10.0, 2.0), sigma_lnK, seed_field) + mean_lnK
+    idx = [np.clip(np.rint(cen[:, a] / spacing[a]).astype(np.int64), 0, gshape[a] - 1) for a in range(3)]
+    ekmul = np.exp(lnK[idx[0], idx[1], idx[2]])
+    rng = np.random.default_rng(seed_layers)
+    names = list(SOIL_CLASSES)
+    n_layers = 6
+    cuts = np.sort(rng.uniform(0.0, lz, n_layers - 1))
+    layer_class = rng.integers(0, len(names), n_layers)
+    layer_class[:4] = rng.permutation(4)  # every class appears
+    layer = np.searchsorted(cuts, cen[:, 2])
+    ezone = layer_class[layer].astype(np.int32)
+    zones = np.array([_zone_row(1.0, SOIL_CLASSES[k][3], SOIL_CLASSES[k][0], SOIL_CLASSES[k][1],
+                                SOIL_CLASSES[k][2] / SOIL_CLASSES[k][3], 1e-5) for k in names])
+    bc_type = np.array([0, 3, 3], dtype=np.int32)
+    bc = np.zeros((3, 6))
+    bc[2, 0] = -rain
+    psi0 = 5.0 - mesh.xyz[:, 2]
+    return Problem("box3d", mesh, zones, ezone, ekmul, bc_type, bc, 1000.0, psi0, 1.0, "day")

@@ -1,0 +1,101 @@
+/*
+ * libfvp: MI355X (gfx950) CVFE Richards residual and Newton-Krylov inner loop.
+ *
+ * C ABI.  All array arguments are plain pointers with sizes; "device" marks
+ * pointers into HIP device memory of the context's device, everything else is
+ * host memory.  Every entry point returns an int status: 0 = success,
+ * > 0 = recoverable failure (the IDA convention of ida_integrator.h:18-36),
+ * < 0 = fatal / invalid argument.  fvp_last_error() gives the message.
+ * A context is bound to one device and one HIP stream and is not thread safe.
+ *
+ * Node numbering: the state vectors at this boundary use the node order the
+ * caller passed to fvp_create (the reference renumbers at mesh load with RCM,
+ * mesh.cpp:118-209; a caller that wants that ordering passes renumbered
+ * arrays).  Edges are numbered as the reference numbers them: sorted by
+ * (min, max) node id, front/back fixed by the first element that lists the
+ * edge (mesh.cpp:385-395, 157-166).
+ */
+#ifndef FVP_H
+#define FVP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fvp_ctx fvp_ctx;
+
+/* Zone parameter row (per physical zone): K_xx, K_yy, K_zz, phi, alpha_vG, n_vG,
+ * S_r, S_s (PhysicalZone, definitions.h:203-230; S_s is the specific-storage
+ * extension, 0 reproduces the reference).  Boundary row (per boundary tag):
+ * value, b, eta, qx, qy, qz (BoundaryCondition, definitions.h:105-200). */
+#define FVP_ZONE_FIELDS 8
+#define FVP_BC_FIELDS 6
+
+typedef struct {
+    int dim;                  /* 2 (triangles) or 3 (tetrahedra) */
+    int n_nodes;
+    int n_elements;
+    const double *xyz;        /* n_nodes x 3 */
+    const int *conn;          /* n_elements x (dim+1) */
+    const int *face_tags;     /* n_elements x (dim+1), 0 = interior face */
+    const int *node_tag_ptr;  /* n_nodes + 1 (CSR of boundary tags per node) */
+    const int *node_tags;
+    const int *elem_zone;     /* n_elements, index into zones */
+    const double *elem_kmul;  /* n_elements conductivity multiplier, NULL = 1 */
+    int n_zones;
+    const double *zones;      /* n_zones x FVP_ZONE_FIELDS */
+    int n_tags;               /* boundary tags 0 .. n_tags-1 */
+    const int *bc_type;       /* n_tags: 0 none, 1, 3, 4, 6, 7, 8 */
+    const double *bc;         /* n_tags x FVP_BC_FIELDS */
+    double rho0;              /* reference density (Constants, definitions.h:296-326) */
+} fvp_problem;
+
+const char *fvp_last_error(void);
+
+/* Build the device-side problem: replaces mesh::Mesh construction
+ * (mesh.cpp:70-80) plus VarSatPhysicsImpl::initialise_vectors and
+ * initialise_shape_functions (fvmpor.h:346-644, 947-1124). */
+int fvp_create(const fvp_problem *problem, int device, fvp_ctx **out);
+int fvp_destroy(fvp_ctx *ctx);
+/* sizes[0..5] = n_nodes, n_elements, n_edges, n_dirichlet, dim, n_zones */
+int fvp_sizes(const fvp_ctx *ctx, int *sizes);
+/* Run all work of this context on a caller HIP stream (hipStream_t); NULL = the context's own. */
+int fvp_set_stream(fvp_ctx *ctx, void *hip_stream);
+int fvp_synchronize(fvp_ctx *ctx);
+/* Update boundary values per tag (n_tags x FVP_BC_FIELDS), e.g. a time series
+ * evaluated by the caller (BoundaryCondition::value(t), definitions.h:148-155). */
+int fvp_set_bc(fvp_ctx *ctx, const double *bc);
+
+/* Residual F(t, psi, psi') (device vectors, n_nodes each).
+ * Replaces Callback::operator() -> SolverBase::compute_residual ->
+ * FVMAssembler::compute_residual -> preprocess_evaluation + residual_evaluation
+ * (fvm.h:348-359, solver.h:123-133, fvm_assembler.h:62-83,
+ * fvmpor_ODE_impl.cpp:62-115). */
+int fvp_residual(fvp_ctx *ctx, double t, const double *psi, const double *psip, double *F);
+
+/* Freeze upwind weights for a time step: one averaged-weight flux evaluation
+ * and the edge-flux sign test (Physics::preprocess_timestep,
+ * fvmpor_ODE_impl.cpp:118-142; process_spatial_weights, fvmpor.h:761-797). */
+int fvp_begin_step(fvp_ctx *ctx, double t, const double *psi, const double *psip);
+/* Host copies of the per-edge front weight (1 = front upwind, 0 = back,
+ * 0.5 = averaging) and of the averaged edge flux of the last begin_step.
+ * Either pointer may be NULL.  n_edges entries each. */
+int fvp_get_upwind(fvp_ctx *ctx, double *w_front, double *edge_flux);
+/* Set per-edge front weights from the host (values 0, 0.5 or 1). */
+int fvp_set_upwind(fvp_ctx *ctx, const double *w_front);
+
+/* Per-node capacity coefficient ahh (device, n_nodes) of the last residual
+ * evaluation (process_derivative_coefficients, fvmpor.h:930-945). */
+int fvp_capacity(fvp_ctx *ctx, double *ahh);
+
+/* Finite-difference Jacobian-vector product of the DAE iteration matrix
+ *   Jv = (F(t, psi + e v, psi' + c_j e v) - F0) / e,  e = sqrt(eps_mach) ||psi||_2 / ||v||_2
+ * (IDA's DQ Jv, ida_integrator.h:233-265; increment of eem_integrator.h:323-337).
+ * F0 = F(t, psi, psi').  All vectors device, n_nodes. */
+int fvp_jv(fvp_ctx *ctx, double t, const double *psi, const double *psip, const double *F0,
+           double c_j, const double *v, double *Jv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,175 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes front end of the CPU oracle (liboracle.so).
+
+The oracle restates the reference's algorithms on the CPU (see orc.h and the
+header of each .c file for the reference file:line each function follows).
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_vp = ctypes.c_void_p
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.orc_mesh_create.restype = _vp
+        L.orc_mesh_create.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, _ip,
+                                      ctypes.POINTER(ctypes.c_int)]
+        L.orc_mesh_free.argtypes = [_vp]
+        L.orc_mesh_counts.argtypes = [_vp, _ip]
+        L.orc_mesh_volumes.argtypes = [_vp, _dp, ctypes.c_void_p]
+        L.orc_mesh_edges.argtypes = [_vp, _ip, _ip]
+        L.orc_mesh_cvfaces.argtypes = [_vp, _ip, _dp]
+        L.orc_mesh_gradients.argtypes = [_vp, _dp]
+        L.orc_problem_create.restype = _vp
+        L.orc_problem_create.argtypes = [_vp, ctypes.c_int, _dp, _ip, _dp, ctypes.c_int, _ip, _dp,
+                                         _ip, _ip, ctypes.c_double]
+        L.orc_problem_free.argtypes = [_vp]
+        L.orc_residual.argtypes = [_vp, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_begin_step.argtypes = [_vp, ctypes.c_double, _dp, _dp]
+        L.orc_set_weights.argtypes = [_vp, _dp]
+        L.orc_get_weights.argtypes = [_vp, _dp, _dp]
+        L.orc_get_node_fields.argtypes = [_vp, _dp, _dp, _dp]
+        L.orc_get_face_fields.argtypes = [_vp, _dp, _dp]
+        L.orc_flux_scale.restype = ctypes.c_double
+        L.orc_flux_scale.argtypes = [_vp]
+        L.orc_num_dirichlet.restype = ctypes.c_int
+        L.orc_num_dirichlet.argtypes = [_vp]
+        L.orc_vg.argtypes = [ctypes.c_double, _dp, _dp]
+        _LIB = L
+    return _LIB
+
+
+def vg(h: float, zone_row) -> tuple:
+    out = np.zeros(3)
+    lib().orc_vg(float(h), np.ascontiguousarray(zone_row, dtype=np.float64), out)
+    return tuple(out)
+
+
+class OracleMesh:
+    """CVFE dual of a simplex mesh built as the reference's mesh.cpp builds it."""
+
+    def __init__(self, mesh):
+        L = lib()
+        self.dim = mesh.dim
+        self._xyz = np.ascontiguousarray(mesh.xyz, dtype=np.float64)
+        self._conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+        self._ft = np.ascontiguousarray(mesh.face_tags, dtype=np.int32)
+        st = ctypes.c_int(0)
+        self.h = L.orc_mesh_create(mesh.dim, self._xyz.shape[0], self._xyz, self._conn.shape[0],
+                                   self._conn, self._ft, ctypes.byref(st))
+        if not self.h:
+            raise ValueError(f"oracle mesh construction failed ({st.value})")
+        c = np.zeros(6, dtype=np.int32)
+        L.orc_mesh_counts(self.h, c)
+        self.N, self.T, self.NE, self.NFI, self.NFB, self.nnz_cv = (int(v) for v in c)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mesh_free(self.h)
+            self.h = None
+
+    def volumes(self):
+        V = np.zeros(self.N)
+        scv = np.zeros(self.T * (self.dim + 1))
+        lib().orc_mesh_volumes(self.h, V, scv.ctypes.data)
+        return V, scv.reshape(self.T, self.dim + 1)
+
+    def edges(self):
+        f = np.zeros(self.NE, dtype=np.int32)
+        b = np.zeros(self.NE, dtype=np.int32)
+        lib().orc_mesh_edges(self.h, f, b)
+        return f, b
+
+    def cvfaces(self):
+        nf = self.NFI + self.NFB
+        ints = np.zeros(5 * nf, dtype=np.int32)
+        dbl = np.zeros(4 * nf)
+        lib().orc_mesh_cvfaces(self.h, ints, dbl)
+        return ints.reshape(nf, 5), dbl.reshape(nf, 4)
+
+    def gradients(self):
+        g = np.zeros(self.T * (self.dim + 1) * 3)
+        lib().orc_mesh_gradients(self.h, g)
+        return g.reshape(self.T, self.dim + 1, 3)
+
+
+class OracleProblem:
+    """The reference's VarSatPhysics residual on an OracleMesh."""
+
+    def __init__(self, problem, omesh: OracleMesh | None = None):
+        L = lib()
+        self.omesh = omesh if omesh is not None else OracleMesh(problem.mesh)
+        m = problem.mesh
+        self._keep = [
+            np.ascontiguousarray(problem.zones, dtype=np.float64),
+            np.ascontiguousarray(problem.ezone, dtype=np.int32),
+            np.ascontiguousarray(problem.ekmul, dtype=np.float64),
+            np.ascontiguousarray(problem.bc_type, dtype=np.int32),
+            np.ascontiguousarray(problem.bc, dtype=np.float64),
+            np.ascontiguousarray(m.node_tag_ptr, dtype=np.int32),
+            np.ascontiguousarray(m.node_tags if len(m.node_tags) else np.zeros(1), dtype=np.int32),
+        ]
+        z, ez, ek, bt, bc, ntp, nt = self._keep
+        self.h = L.orc_problem_create(self.omesh.h, z.shape[0], z, ez, ek, bt.shape[0], bt, bc,
+                                      ntp, nt, float(problem.rho0))
+        self.N = self.omesh.N
+        self.NE = self.omesh.NE
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_problem_free(self.h)
+            self.h = None
+
+    def residual(self, t, psi, psip):
+        F = np.zeros(self.N)
+        lib().orc_residual(self.h, float(t), np.ascontiguousarray(psi, dtype=np.float64),
+                           np.ascontiguousarray(psip, dtype=np.float64), F)
+        return F
+
+    def begin_step(self, t, psi, psip):
+        lib().orc_begin_step(self.h, float(t), np.ascontiguousarray(psi, dtype=np.float64),
+                             np.ascontiguousarray(psip, dtype=np.float64))
+
+    def weights(self):
+        wf = np.zeros(self.NE)
+        fl = np.zeros(self.NE)
+        lib().orc_get_weights(self.h, wf, fl)
+        return wf, fl
+
+    def set_weights(self, wf):
+        lib().orc_set_weights(self.h, np.ascontiguousarray(wf, dtype=np.float64))
+
+    def node_fields(self):
+        a, s, d = np.zeros(self.N), np.zeros(self.N), np.zeros(self.N)
+        lib().orc_get_node_fields(self.h, a, s, d)
+        return a, s, d
+
+    def face_fields(self):
+        nf = self.omesh.NFI + self.omesh.NFB
+        q, M = np.zeros(nf), np.zeros(nf)
+        lib().orc_get_face_fields(self.h, q, M)
+        return q, M
+
+    def flux_scale(self):
+        return lib().orc_flux_scale(self.h)
