@@ -34,6 +34,8 @@ struct DevBuf {
 
 }  // namespace fvp
 
+struct fvp_jacdata;
+
 struct fvp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -81,11 +83,13 @@ struct fvp_ctx {
     fvp::DevBuf<double> tmp1, tmp2, tmp3;
     fvp::DevBuf<double> red;       // reduction partials
     std::vector<double> red_h;
+    fvp_jacdata *jd = nullptr;  // coloured FD Jacobian and its factorisation
 };
 
 namespace fvp {
 int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, double *F, int averaging);
 int upload_bc_values(fvp_ctx *c);
+void jac_free(fvp_ctx *c);
 }  // namespace fvp
 
 #define FVP_HIP(call)                                                                         \

@@ -1,0 +1,821 @@
+// Sparse linear algebra and the Newton-Krylov inner loop on gfx950.
+//
+//  * CSR SpMV: 8 lanes per row, shuffle reduction (rows hold ~7 (2D) to ~15 (3D) entries).
+//  * ILU(0): level-scheduled factorisation and triangular solves.  Levels come
+//    from the dependency DAG of L (forward) and U (backward); rows of a level
+//    are independent and run as one launch.
+//  * Jacobi: inverse diagonal.
+//  * GMRES: right-preconditioned, restarted (gmres.c:26-201) with classical
+//    Gram-Schmidt fused into one multi-dot and one multi-axpy pass per sweep
+//    (applied twice when re-orthogonalising), so each Arnoldi step reads the
+//    basis twice instead of 2(j+1) times as modified Gram-Schmidt would.  The
+//    (m+1) x m Hessenberg system is reduced with Givens rotations on the host.
+//  * Coloured finite-difference Jacobian (preconditioner_dss.cpp:284-372): one
+//    residual per colour of a distance-2 colouring.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <set>
+#include <vector>
+
+#include "fvp_blas.h"
+#include "fvp_internal.h"
+
+using fvp::DevBuf;
+using fvp::set_error;
+
+struct fvp_csr {
+    int device = 0;
+    hipStream_t stream = nullptr, own_stream = nullptr;
+    int n = 0, nnz = 0;
+    std::vector<int> ia_h, ja_h;
+    DevBuf<int> ia, ja;
+    DevBuf<double> a;
+    // preconditioner
+    int ptype = -1;
+    DevBuf<double> lu, jac;
+    DevBuf<int> diag;
+    bool levels = false;
+    std::vector<int> lptr_h, uptr_h;
+    DevBuf<int> lrows, urows;
+    DevBuf<int> flag;  // device error flag (zero pivot)
+    // workspace
+    int m_alloc = -1;
+    DevBuf<double> Q, w, r, z, part, scal, hdev;
+};
+
+namespace fvp {
+
+constexpr int kBlocks = 512;
+constexpr int kThreads = 256;
+
+// ---- kernels ------------------------------------------------------------------
+template <int LANES>
+__global__ void __launch_bounds__(256) csr_gemv_kernel(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                                       const double *__restrict__ a, double alpha,
+                                                       const double *__restrict__ x, double beta,
+                                                       double *__restrict__ y) {
+    int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    int row = tid / LANES, lane = tid % LANES;
+    double s = 0.;
+    if (row < n)
+        for (int k = ia[row] + lane; k < ia[row + 1]; k += LANES) s += a[k] * x[ja[k]];
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LANES);
+    if (row < n && lane == 0) y[row] = beta == 0. ? alpha * s : alpha * s + beta * y[row];
+}
+
+__global__ void ilu0_factor_level(int cnt, const int *__restrict__ rows, const int *__restrict__ ia,
+                                  const int *__restrict__ ja, const int *__restrict__ diag, double *__restrict__ lu,
+                                  int *__restrict__ flag) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    int i = rows[t];
+    int end = ia[i + 1];
+    for (int kk = ia[i]; kk < diag[i]; ++kk) {
+        int k = ja[kk];
+        int kpos = ia[k], kend = ia[k + 1];
+        double lik = lu[kk] * lu[diag[k]];
+        lu[kk] = lik;
+        for (int jj = kk + 1; jj < end; ++jj) {
+            int j = ja[jj];
+            while (kpos < kend && ja[kpos] < j) kpos++;
+            if (kpos == kend) break;
+            if (ja[kpos] == j) lu[jj] -= lik * lu[kpos];
+        }
+    }
+    double d = lu[diag[i]];
+    if (d == 0.) { atomicOr(flag, 1); lu[diag[i]] = 0.; }
+    else lu[diag[i]] = 1. / d;
+}
+
+template <int LANES>
+__global__ void __launch_bounds__(256) ilu0_lower_level(int cnt, const int *__restrict__ rows,
+                                                        const int *__restrict__ ia, const int *__restrict__ ja,
+                                                        const int *__restrict__ diag, const double *__restrict__ lu,
+                                                        const double *__restrict__ x, double *__restrict__ y) {
+    int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = tid / LANES, lane = tid % LANES;
+    int i = t < cnt ? rows[t] : 0;
+    double s = 0.;
+    if (t < cnt)
+        for (int k = ia[i] + lane; k < diag[i]; k += LANES) s += lu[k] * y[ja[k]];
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LANES);
+    if (t < cnt && lane == 0) y[i] = x[i] - s;
+}
+
+template <int LANES>
+__global__ void __launch_bounds__(256) ilu0_upper_level(int cnt, const int *__restrict__ rows,
+                                                        const int *__restrict__ ia, const int *__restrict__ ja,
+                                                        const int *__restrict__ diag, const double *__restrict__ lu,
+                                                        double *__restrict__ y) {
+    int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = tid / LANES, lane = tid % LANES;
+    int i = t < cnt ? rows[t] : 0;
+    double s = 0.;
+    if (t < cnt)
+        for (int k = diag[i] + 1 + lane; k < ia[i + 1]; k += LANES) s += lu[k] * y[ja[k]];
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LANES);
+    if (t < cnt && lane == 0) y[i] = (y[i] - s) * lu[diag[i]];
+}
+
+__global__ void jacobi_build(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                             const double *__restrict__ a, double *__restrict__ J, int *__restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int pos = ia[i];
+    while (pos < ia[i + 1] && ja[pos] < i) pos++;
+    if (pos == ia[i + 1] || ja[pos] != i || a[pos] == 0.) { atomicOr(flag, 1); J[i] = 0.; return; }
+    J[i] = 1. / a[pos];
+}
+
+__global__ void vmul(long n, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ c) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) c[i] = a[i] * b[i];
+}
+
+__global__ void vcopy(long n, const double *__restrict__ a, double *__restrict__ b) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+// y = b - y
+__global__ void vrsub(long n, const double *__restrict__ b, double *__restrict__ y) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) y[i] = b[i] - y[i];
+}
+
+// y += x
+__global__ void vadd(long n, const double *__restrict__ x, double *__restrict__ y) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) y[i] += x[i];
+}
+
+// dst = src / sqrt(nsq[0])
+__global__ void vscale_inv_norm(long n, const double *__restrict__ src, const double *__restrict__ nsq,
+                                double *__restrict__ dst) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] * (1. / sqrt(nsq[0]));
+}
+
+// part[c * gridDim.x + block] = sum over the block's slice of w * Q[:, c0 + c], c < nc <= 8
+__global__ void __launch_bounds__(kThreads) multidot8(long n, const double *__restrict__ w,
+                                                      const double *__restrict__ Q, long ldq, int nc,
+                                                      double *__restrict__ part) {
+    __shared__ double lds[8][kThreads / 64];
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double wi = w[i];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (c < nc) s[c] += wi * Q[c * ldq + i];
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        double v = s[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) lds[c][wid] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < nc) {
+        double v = 0.;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) v += lds[threadIdx.x][k];
+        part[threadIdx.x * gridDim.x + blockIdx.x] = v;
+    }
+}
+
+// out[c] (+)= sum_b part[c * nb + b]; one block per column
+__global__ void __launch_bounds__(kThreads) reduce_cols(int nb, const double *__restrict__ part,
+                                                        double *__restrict__ out, int accumulate) {
+    __shared__ double lds[kThreads / 64];
+    int c = blockIdx.x;
+    double s = 0.;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[c * nb + b];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = 0.;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) v += lds[k];
+        out[c] = accumulate ? out[c] + v : v;
+    }
+}
+
+// w += sign * sum_c coef[c] Q[:, c]
+__global__ void multiaxpy(long n, double *__restrict__ w, const double *__restrict__ Q, long ldq, int nc,
+                          const double *__restrict__ coef, double sign) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.;
+    for (int c = 0; c < nc; ++c) s += coef[c] * Q[c * ldq + i];
+    w[i] += sign * s;
+}
+
+struct Coef32 {
+    double v[64];
+};
+// out = sum_c y[c] Q[:, c]
+__global__ void lincomb(long n, double *__restrict__ out, const double *__restrict__ Q, long ldq, int nc, Coef32 y) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.;
+    for (int c = 0; c < nc; ++c) s += Q[c * ldq + i] * y.v[c];
+    out[i] = s;
+}
+
+static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// ---- host helpers ---------------------------------------------------------------
+static int ensure_work(fvp_csr *A, int m) {
+    if (A->m_alloc >= m) return 0;
+    long n = A->n;
+    if (A->Q.alloc((size_t)n * (m + 1)) || A->w.alloc(n) || A->r.alloc(n) || A->z.alloc(n) ||
+        A->part.alloc((size_t)kBlocks * 8) || A->scal.alloc(8) || A->hdev.alloc(2 * (m + 2)))
+        return -1;
+    A->m_alloc = m;
+    return 0;
+}
+
+// out[0..nc) = Q[:, 0..nc)^T w (device), chunks of 8 columns
+static void multidot(hipStream_t s, long n, const double *w, const double *Q, long ldq, int nc, double *part,
+                     double *out, int accumulate) {
+    for (int c0 = 0; c0 < nc; c0 += 8) {
+        int k = std::min(8, nc - c0);
+        multidot8<<<kBlocks, kThreads, 0, s>>>(n, w, Q + c0 * ldq, ldq, k, part);
+        reduce_cols<<<k, kThreads, 0, s>>>(kBlocks, part, out + c0, accumulate);
+    }
+}
+
+static double host_norm(hipStream_t s, long n, const double *x, double *part, double *dev_out) {
+    dot_async(s, n, x, x, part, dev_out);
+    double v = 0.;
+    (void)hipMemcpyAsync(&v, dev_out, sizeof(double), hipMemcpyDeviceToHost, s);
+    (void)hipStreamSynchronize(s);
+    return std::sqrt(v);
+}
+
+// netlib drotg / drot
+static void drotg(double *da, double *db, double *c, double *s) {
+    double roe = std::fabs(*da) > std::fabs(*db) ? *da : *db;
+    double scale = std::fabs(*da) + std::fabs(*db), r, z;
+    if (scale == 0.) { *c = 1.; *s = 0.; r = 0.; z = 0.; }
+    else {
+        double a = *da / scale, b = *db / scale;
+        r = scale * std::sqrt(a * a + b * b);
+        r = (roe >= 0 ? 1. : -1.) * r;
+        *c = *da / r; *s = *db / r;
+        z = 1.;
+        if (std::fabs(*da) > std::fabs(*db)) z = *s;
+        if (std::fabs(*db) >= std::fabs(*da) && *c != 0.) z = 1. / *c;
+    }
+    *da = r; *db = z;
+}
+
+using Op = std::function<int(const double *, double *)>;
+
+// Restarted right-preconditioned GMRES (gmres.c:26-201).  Each Arnoldi step
+// stops the restart cycle early once the Givens residual estimate meets tol.
+static int gmres_device(fvp_csr *W, hipStream_t s, long n, const Op &A, const Op &M, const double *b, double *x,
+                        const fvp_gmres_params *p, fvp_gmres_stats *st) {
+    const int m = p->dim_krylov;
+    if (m < 1 || m > 63) { set_error("dim_krylov must be 1..63"); return -1; }
+    if (ensure_work(W, m)) return -1;
+    double *Q = W->Q.p, *w = W->w.p, *r = W->r.p, *z = W->z.p, *part = W->part.p, *sc = W->scal.p,
+           *hd = W->hdev.p;
+    const long ldq = n;
+    fvp_gmres_stats stats{0, 0, 0, 0., 0.};
+    // r = b - A x
+    if (A(x, r)) return -1;
+    vrsub<<<nblk(n, 256), 256, 0, s>>>(n, b, r);
+    double beta = host_norm(s, n, r, part, sc);
+    double normb = host_norm(s, n, b, part, sc);
+    double error = normb > 0 ? beta / normb : 0.;
+    if (beta == 0. || error < p->tol) {
+        stats.converged = 1; stats.residual = beta; stats.error = error;
+        if (st) *st = stats;
+        return 0;
+    }
+    std::vector<double> H((size_t)(m + 1) * m), g(m + 1), hcol(m + 2), hcol2(m + 2);
+    std::vector<double> cs(m), sn(m);
+    int restarts = 0, converged = 0, total = 0;
+    // q0 = r / beta
+    dot_async(s, n, r, r, part, sc);
+    vscale_inv_norm<<<nblk(n, 256), 256, 0, s>>>(n, r, sc, Q);
+    while (restarts <= p->max_restarts && !converged) {
+        std::fill(H.begin(), H.end(), 0.);
+        std::fill(g.begin(), g.end(), 0.);
+        g[0] = beta;
+        int j = 0;
+        while (j < m) {
+            const double *qj = Q + (size_t)j * ldq;
+            if (M) { if (M(qj, z)) return -1; if (A(z, w)) return -1; }
+            else if (A(qj, w)) return -1;
+            multidot(s, n, w, Q, ldq, j + 1, part, hd, 0);
+            multiaxpy<<<nblk(n, 256), 256, 0, s>>>(n, w, Q, ldq, j + 1, hd, -1.0);
+            if (p->reorthogonalise) {  // CGS2: second sweep, coefficients added on the host
+                multidot(s, n, w, Q, ldq, j + 1, part, hd + (m + 2), 0);
+                multiaxpy<<<nblk(n, 256), 256, 0, s>>>(n, w, Q, ldq, j + 1, hd + (m + 2), -1.0);
+            }
+            FVP_HIP(hipGetLastError());
+            dot_async(s, n, w, w, part, sc);
+            FVP_HIP(hipMemcpyAsync(hcol.data(), hd, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s));
+            if (p->reorthogonalise)
+                FVP_HIP(hipMemcpyAsync(hcol2.data(), hd + (m + 2), sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s));
+            double hn2 = 0.;
+            FVP_HIP(hipMemcpyAsync(&hn2, sc, sizeof(double), hipMemcpyDeviceToHost, s));
+            FVP_HIP(hipStreamSynchronize(s));
+            if (p->reorthogonalise)
+                for (int i = 0; i <= j; ++i) hcol[i] += hcol2[i];
+            double hn = std::sqrt(hn2);
+            for (int i = 0; i <= j; ++i) H[(size_t)j * (m + 1) + i] = hcol[i];
+            H[(size_t)j * (m + 1) + j + 1] = hn;
+            ++total;
+            if (hn < 1e-16) { ++j; break; }
+            vscale_inv_norm<<<nblk(n, 256), 256, 0, s>>>(n, w, sc, Q + (size_t)(j + 1) * ldq);
+            // Givens on the new column for the residual estimate
+            for (int i = 0; i < j; ++i) {
+                double *hc = &H[(size_t)j * (m + 1)];
+                double t1 = cs[i] * hc[i] + sn[i] * hc[i + 1];
+                hc[i + 1] = cs[i] * hc[i + 1] - sn[i] * hc[i];
+                hc[i] = t1;
+            }
+            {
+                double *hc = &H[(size_t)j * (m + 1)];
+                double a = hc[j], bb = hc[j + 1], c, sv;
+                drotg(&a, &bb, &c, &sv);
+                cs[j] = c; sn[j] = sv;
+                hc[j] = c * hc[j] + sv * hc[j + 1];
+                hc[j + 1] = 0.;
+                double gj = g[j];
+                g[j] = c * gj;
+                g[j + 1] = -sv * gj;
+            }
+            ++j;
+            if (std::fabs(g[j]) < p->tol * normb) break;
+        }
+        // back substitution on the rotated (upper triangular) H
+        std::vector<double> y(j, 0.);
+        for (int i = j - 1; i >= 0; --i) {
+            double sum = g[i];
+            for (int k = i + 1; k < j; ++k) sum -= H[(size_t)k * (m + 1) + i] * y[k];
+            y[i] = sum / H[(size_t)i * (m + 1) + i];
+        }
+        Coef32 yc{};
+        for (int i = 0; i < j; ++i) yc.v[i] = y[i];
+        lincomb<<<nblk(n, 256), 256, 0, s>>>(n, w, Q, ldq, j, yc);
+        if (M) { if (M(w, z)) return -1; }
+        else vcopy<<<nblk(n, 256), 256, 0, s>>>(n, w, z);
+        vadd<<<nblk(n, 256), 256, 0, s>>>(n, z, x);
+        if (A(x, r)) return -1;
+        vrsub<<<nblk(n, 256), 256, 0, s>>>(n, b, r);
+        beta = host_norm(s, n, r, part, sc);
+        error = beta / normb;
+        if (error < p->tol) converged = 1;
+        else ++restarts;
+        if (!converged && restarts <= p->max_restarts) {
+            dot_async(s, n, r, r, part, sc);
+            vscale_inv_norm<<<nblk(n, 256), 256, 0, s>>>(n, r, sc, Q);
+        }
+    }
+    stats.restarts = converged ? restarts : restarts - 1;
+    stats.iterations = total;
+    stats.converged = converged;
+    stats.residual = beta;
+    stats.error = error;
+    if (st) *st = stats;
+    return converged ? 0 : FVP_ERR_NOCONV;
+}
+
+}  // namespace fvp
+
+// ================================================================================
+// decomp-style CSR entry points
+// ================================================================================
+using namespace fvp;
+
+namespace fvp {
+
+static int csr_gemv_impl(fvp_csr *A, double alpha, const double *x, double beta, double *y) {
+    const int L = 8;
+    csr_gemv_kernel<L><<<nblk((long)A->n * L, 256), 256, 0, A->stream>>>(A->n, A->ia.p, A->ja.p, A->a.p, alpha, x,
+                                                                         beta, y);
+    FVP_HIP(hipGetLastError());
+    return 0;
+}
+
+// level sets of the L (forward) and U (backward) dependency DAGs
+static int build_levels(fvp_csr *A, const std::vector<int> &diag) {
+    const int n = A->n;
+    std::vector<int> lev(n, 0), ulev(n, 0);
+    int nl = 0, nu = 0;
+    for (int i = 0; i < n; ++i) {
+        int l = 0;
+        for (int k = A->ia_h[i]; k < diag[i]; ++k) l = std::max(l, lev[A->ja_h[k]] + 1);
+        lev[i] = l;
+        nl = std::max(nl, l + 1);
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        int l = 0;
+        for (int k = diag[i] + 1; k < A->ia_h[i + 1]; ++k) l = std::max(l, ulev[A->ja_h[k]] + 1);
+        ulev[i] = l;
+        nu = std::max(nu, l + 1);
+    }
+    auto group = [&](const std::vector<int> &lv, int nlev, std::vector<int> &ptr, std::vector<int> &rows) {
+        ptr.assign(nlev + 1, 0);
+        for (int i = 0; i < n; ++i) ptr[lv[i] + 1]++;
+        for (int l = 0; l < nlev; ++l) ptr[l + 1] += ptr[l];
+        rows.resize(n);
+        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+        for (int i = 0; i < n; ++i) rows[fill[lv[i]]++] = i;
+    };
+    std::vector<int> lrows, urows;
+    group(lev, nl, A->lptr_h, lrows);
+    group(ulev, nu, A->uptr_h, urows);
+    if (A->lrows.upload(lrows.data(), n) || A->urows.upload(urows.data(), n)) return -1;
+    A->levels = true;
+    return 0;
+}
+
+static int precon_build_impl(fvp_csr *A, int type) {
+    hipStream_t s = A->stream;
+    const int n = A->n;
+    if (type == FVP_PRECON_NONE) { A->ptype = type; return 0; }
+    if (type != FVP_PRECON_JACOBI && type != FVP_PRECON_ILU0) { set_error("invalid preconditioner type"); return -1; }
+    if (!A->flag.p && A->flag.alloc(1)) return -1;
+    FVP_HIP(hipMemsetAsync(A->flag.p, 0, sizeof(int), s));
+    if (type == FVP_PRECON_JACOBI) {
+        if (A->jac.n != (size_t)n && A->jac.alloc(n)) return -1;
+        jacobi_build<<<nblk(n, 256), 256, 0, s>>>(n, A->ia.p, A->ja.p, A->a.p, A->jac.p, A->flag.p);
+    } else {
+        if (!A->diag.p) {
+            std::vector<int> diag(n);
+            for (int i = 0; i < n; ++i) {
+                int k = A->ia_h[i];
+                while (k < A->ia_h[i + 1] && A->ja_h[k] != i) ++k;
+                if (k == A->ia_h[i + 1]) { set_error("zero diagonal on row " + std::to_string(i)); A->ptype = -1; return FVP_ERR_PIVOT; }
+                diag[i] = k;
+            }
+            if (A->diag.upload(diag.data(), n)) return -1;
+            if (build_levels(A, diag)) return -1;
+        }
+        if (A->lu.n != (size_t)A->nnz && A->lu.alloc(A->nnz)) return -1;
+        FVP_HIP(hipMemcpyAsync(A->lu.p, A->a.p, sizeof(double) * A->nnz, hipMemcpyDeviceToDevice, s));
+        for (size_t l = 0; l + 1 < A->lptr_h.size(); ++l) {
+            int cnt = A->lptr_h[l + 1] - A->lptr_h[l];
+            ilu0_factor_level<<<nblk(cnt, 64), 64, 0, s>>>(cnt, A->lrows.p + A->lptr_h[l], A->ia.p, A->ja.p,
+                                                           A->diag.p, A->lu.p, A->flag.p);
+        }
+    }
+    FVP_HIP(hipGetLastError());
+    int flag = 0;
+    FVP_HIP(hipMemcpyAsync(&flag, A->flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    FVP_HIP(hipStreamSynchronize(s));
+    if (flag) { set_error("zero pivot in preconditioner factorisation"); A->ptype = -1; return FVP_ERR_PIVOT; }
+    A->ptype = type;
+    return 0;
+}
+
+static int precon_apply_impl(fvp_csr *A, const double *x, double *y) {
+    hipStream_t s = A->stream;
+    const int n = A->n;
+    switch (A->ptype) {
+    case FVP_PRECON_NONE:
+        vcopy<<<nblk(n, 256), 256, 0, s>>>(n, x, y);
+        break;
+    case FVP_PRECON_JACOBI:
+        vmul<<<nblk(n, 256), 256, 0, s>>>(n, x, A->jac.p, y);
+        break;
+    case FVP_PRECON_ILU0: {
+        const int L = 8;
+        for (size_t l = 0; l + 1 < A->lptr_h.size(); ++l) {
+            int cnt = A->lptr_h[l + 1] - A->lptr_h[l];
+            ilu0_lower_level<L><<<nblk((long)cnt * L, 256), 256, 0, s>>>(cnt, A->lrows.p + A->lptr_h[l], A->ia.p,
+                                                                        A->ja.p, A->diag.p, A->lu.p, x, y);
+        }
+        for (size_t l = 0; l + 1 < A->uptr_h.size(); ++l) {
+            int cnt = A->uptr_h[l + 1] - A->uptr_h[l];
+            ilu0_upper_level<L><<<nblk((long)cnt * L, 256), 256, 0, s>>>(cnt, A->urows.p + A->uptr_h[l], A->ia.p,
+                                                                        A->ja.p, A->diag.p, A->lu.p, y);
+        }
+        break;
+    }
+    default:
+        set_error("preconditioner not built");
+        return -1;
+    }
+    FVP_HIP(hipGetLastError());
+    return 0;
+}
+
+int csr_create_impl(int device, int n, const int *ia, const int *ja, fvp_csr **out) {
+    if (!out || !ia || !ja || n <= 0) { set_error("invalid CSR arguments"); return -1; }
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (ia[i + 1] < ia[i]) { set_error("row pointers must be non-decreasing"); return -1; }
+        for (int k = ia[i]; k < ia[i + 1]; ++k) {
+            if (ja[k] < 0 || ja[k] >= n) { set_error("column index out of range"); return -1; }
+            if (k > ia[i] && ja[k] <= ja[k - 1]) { set_error("column indices must be sorted and unique"); return -1; }
+        }
+    }
+    FVP_HIP(hipSetDevice(device));
+    fvp_csr *A = new fvp_csr();
+    A->device = device;
+    A->n = n;
+    A->nnz = ia[n];
+    A->ia_h.assign(ia, ia + n + 1);
+    A->ja_h.assign(ja, ja + A->nnz);
+    if (hipStreamCreateWithFlags(&A->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete A; set_error("hipStreamCreate failed"); return -1;
+    }
+    A->stream = A->own_stream;
+    if (A->ia.upload(ia, n + 1) || A->ja.upload(ja, A->nnz) || A->a.alloc(A->nnz)) { fvp_csr_destroy(A); return -1; }
+    *out = A;
+    return 0;
+}
+
+}  // namespace fvp
+
+extern "C" int fvp_csr_create(int device, int n, const int *ia, const int *ja, fvp_csr **out) {
+    return csr_create_impl(device, n, ia, ja, out);
+}
+
+extern "C" int fvp_csr_destroy(fvp_csr *A) {
+    if (!A) return 0;
+    (void)hipSetDevice(A->device);
+    if (A->own_stream) (void)hipStreamSynchronize(A->own_stream);
+    A->ia.release(); A->ja.release(); A->a.release(); A->lu.release(); A->jac.release(); A->diag.release();
+    A->lrows.release(); A->urows.release(); A->flag.release(); A->Q.release(); A->w.release(); A->r.release();
+    A->z.release(); A->part.release(); A->scal.release(); A->hdev.release();
+    if (A->own_stream) (void)hipStreamDestroy(A->own_stream);
+    delete A;
+    return 0;
+}
+
+extern "C" int fvp_csr_set_stream(fvp_csr *A, void *stream) {
+    if (!A) { set_error("null matrix"); return -1; }
+    A->stream = stream ? (hipStream_t)stream : A->own_stream;
+    return 0;
+}
+
+extern "C" int fvp_csr_set_values(fvp_csr *A, const double *values) {
+    if (!A || !values) { set_error("null argument"); return -1; }
+    FVP_HIP(hipSetDevice(A->device));
+    FVP_HIP(hipMemcpyAsync(A->a.p, values, sizeof(double) * A->nnz, hipMemcpyDeviceToDevice, A->stream));
+    A->ptype = -1;
+    return 0;
+}
+
+extern "C" int fvp_csr_get_values(fvp_csr *A, double *values_host) {
+    if (!A || !values_host) { set_error("null argument"); return -1; }
+    FVP_HIP(hipSetDevice(A->device));
+    FVP_HIP(hipStreamSynchronize(A->stream));
+    FVP_HIP(hipMemcpy(values_host, A->a.p, sizeof(double) * A->nnz, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int fvp_csr_gemv(fvp_csr *A, double alpha, const double *x, double beta, double *y) {
+    if (!A || !x || !y) { set_error("null argument"); return -1; }
+    FVP_HIP(hipSetDevice(A->device));
+    return csr_gemv_impl(A, alpha, x, beta, y);
+}
+
+extern "C" int fvp_precon_build(fvp_csr *A, int type) {
+    if (!A) { set_error("null matrix"); return -1; }
+    FVP_HIP(hipSetDevice(A->device));
+    return precon_build_impl(A, type);
+}
+
+extern "C" int fvp_precon_apply(fvp_csr *A, const double *x, double *y) {
+    if (!A || !x || !y) { set_error("null argument"); return -1; }
+    if (x == y && A->ptype == FVP_PRECON_ILU0) { set_error("ILU(0) apply cannot run in place"); return -1; }
+    FVP_HIP(hipSetDevice(A->device));
+    return precon_apply_impl(A, x, y);
+}
+
+extern "C" int fvp_precon_values(fvp_csr *A, double *values_host) {
+    if (!A || !values_host) { set_error("null argument"); return -1; }
+    FVP_HIP(hipSetDevice(A->device));
+    FVP_HIP(hipStreamSynchronize(A->stream));
+    if (A->ptype == FVP_PRECON_ILU0)
+        FVP_HIP(hipMemcpy(values_host, A->lu.p, sizeof(double) * A->nnz, hipMemcpyDeviceToHost));
+    else if (A->ptype == FVP_PRECON_JACOBI)
+        FVP_HIP(hipMemcpy(values_host, A->jac.p, sizeof(double) * A->n, hipMemcpyDeviceToHost));
+    else { set_error("no factored values"); return -1; }
+    return 0;
+}
+
+extern "C" int fvp_gmres_csr(fvp_csr *A, const double *b, double *x, const fvp_gmres_params *p,
+                             fvp_gmres_stats *st) {
+    if (!A || !b || !x || !p) { set_error("null argument"); return -1; }
+    if (A->ptype < 0) { set_error("preconditioner not built"); return -1; }
+    FVP_HIP(hipSetDevice(A->device));
+    Op op = [A](const double *in, double *out) { return csr_gemv_impl(A, 1.0, in, 0.0, out); };
+    Op pc;
+    if (A->ptype != FVP_PRECON_NONE) pc = [A](const double *in, double *out) { return precon_apply_impl(A, in, out); };
+    return gmres_device(A, A->stream, A->n, op, pc, b, x, p, st);
+}
+
+// ================================================================================
+// Coloured FD Jacobian and the Newton linear solve on a problem context
+// ================================================================================
+struct fvp_jacdata {
+    fvp_csr *J = nullptr;
+    int ncol = 0;
+    std::vector<int> colour;           // per node
+    std::vector<int> cptr;             // colour -> [cptr[c], cptr[c+1]) in cols
+    std::vector<int> eptr;             // colour -> entries
+    DevBuf<int> cols, erow, epos, ecol;
+    DevBuf<double> shift, ys, yps, Fs;
+};
+
+namespace fvp {
+
+__global__ void shift_kernel(int n, double eps, double h, const double *__restrict__ y, const double *__restrict__ yp,
+                             const double *__restrict__ ewt, double *__restrict__ shift) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    shift[j] = eps * fmax(fabs(y[j]), fmax(fabs(h * yp[j]), 1.0 / ewt[j]));
+}
+
+__global__ void perturb_kernel(int cnt, const int *__restrict__ cols, const double *__restrict__ shift, double c,
+                               const double *__restrict__ y, const double *__restrict__ yp, double *__restrict__ ys,
+                               double *__restrict__ yps, int restore) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    int j = cols[k];
+    if (restore) { ys[j] = y[j]; yps[j] = yp[j]; }
+    else { ys[j] = y[j] + shift[j]; yps[j] = yp[j] + c * shift[j]; }
+}
+
+// J[pos] = (F0[row] - Fs[row]) / shift[col]   (-(dF/dy + c dF/dy') column entries)
+__global__ void fd_scatter(int cnt, const int *__restrict__ erow, const int *__restrict__ epos,
+                           const int *__restrict__ ecol, const double *__restrict__ F0,
+                           const double *__restrict__ Fs, const double *__restrict__ shift, double *__restrict__ J) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    int r = erow[k];
+    J[epos[k]] = (F0[r] - Fs[r]) / shift[ecol[k]];
+}
+
+// Node adjacency over element pairs (preconditioner_dss.cpp:107-142) and the
+// greedy distance-2 colouring of preconditioner_dss.cpp:72-94.
+static int jac_init(fvp_ctx *c) {
+    if (c->jd) return 0;
+    const int N = c->N, T = c->T, npe = c->npe;
+    std::vector<int4> conn(T);
+    FVP_HIP(hipMemcpy(conn.data(), c->conn.p, sizeof(int4) * T, hipMemcpyDeviceToHost));
+    // adjacency lists
+    std::vector<std::vector<int>> adj(N);
+    for (int e = 0; e < T; ++e) {
+        int n[4] = {conn[e].x, conn[e].y, conn[e].z, conn[e].w};
+        for (int a = 0; a < npe; ++a)
+            for (int b = 0; b < npe; ++b) adj[n[a]].push_back(n[b]);
+    }
+    std::vector<int> ia(N + 1, 0), ja;
+    for (int i = 0; i < N; ++i) {
+        auto &v = adj[i];
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        ia[i + 1] = ia[i] + (int)v.size();
+    }
+    ja.reserve(ia[N]);
+    for (int i = 0; i < N; ++i) ja.insert(ja.end(), adj[i].begin(), adj[i].end());
+    fvp_jacdata *jd = new fvp_jacdata();
+    if (csr_create_impl(c->device, N, ia.data(), ja.data(), &jd->J)) { delete jd; return -1; }
+    fvp_csr_set_stream(jd->J, c->stream);
+    // greedy distance-2 colouring in node order
+    std::vector<int> colour(N, N - 1), mark(N, -1);
+    int maxc = 0;
+    for (int i = 0; i < N; ++i) {
+        for (int p = ia[i]; p < ia[i + 1]; ++p) {
+            int nb = ja[p];
+            for (int q = ia[nb]; q < ia[nb + 1]; ++q) mark[colour[ja[q]]] = i;
+        }
+        int sc = 0;
+        while (sc < maxc && mark[sc] == i) ++sc;
+        if (sc == maxc) ++maxc;
+        colour[i] = sc;
+    }
+    jd->ncol = maxc;
+    jd->colour = colour;
+    std::vector<int> cols(N), cptr(maxc + 1, 0);
+    for (int i = 0; i < N; ++i) cptr[colour[i] + 1]++;
+    for (int k = 0; k < maxc; ++k) cptr[k + 1] += cptr[k];
+    {
+        std::vector<int> fill(cptr.begin(), cptr.end() - 1);
+        for (int i = 0; i < N; ++i) cols[fill[colour[i]]++] = i;
+    }
+    // entries per colour: for each column j of the colour, rows = pattern of column j
+    std::vector<int> erow, epos, ecol, eptr(maxc + 1, 0);
+    erow.reserve(ia[N]); epos.reserve(ia[N]); ecol.reserve(ia[N]);
+    for (int k = 0; k < maxc; ++k) {
+        for (int q = cptr[k]; q < cptr[k + 1]; ++q) {
+            int j = cols[q];
+            for (int p = ia[j]; p < ia[j + 1]; ++p) {
+                int row = ja[p];
+                // position of (row, j) in row's sorted column list
+                int lo = ia[row], hi = ia[row + 1];
+                int pos = (int)(std::lower_bound(ja.begin() + lo, ja.begin() + hi, j) - ja.begin());
+                erow.push_back(row); epos.push_back(pos); ecol.push_back(j);
+            }
+        }
+        eptr[k + 1] = (int)erow.size();
+    }
+    jd->cptr = cptr;
+    jd->eptr = eptr;
+    if (jd->cols.upload(cols.data(), N) || jd->erow.upload(erow.data(), erow.size()) ||
+        jd->epos.upload(epos.data(), epos.size()) || jd->ecol.upload(ecol.data(), ecol.size()) ||
+        jd->shift.alloc(N) || jd->ys.alloc(N) || jd->yps.alloc(N) || jd->Fs.alloc(N)) {
+        fvp_csr_destroy(jd->J);
+        delete jd;
+        return -1;
+    }
+    c->jd = jd;
+    return 0;
+}
+
+void jac_free(fvp_ctx *c) {
+    if (!c->jd) return;
+    fvp_jacdata *jd = c->jd;
+    fvp_csr_destroy(jd->J);
+    jd->cols.release(); jd->erow.release(); jd->epos.release(); jd->ecol.release();
+    jd->shift.release(); jd->ys.release(); jd->yps.release(); jd->Fs.release();
+    delete jd;
+    c->jd = nullptr;
+}
+
+}  // namespace fvp
+
+extern "C" int fvp_precon_setup(fvp_ctx *c, double t, double c_j, double h, const double *psi, const double *psip,
+                                const double *F0, const double *ewt, int type) {
+    if (!c || !psi || !psip || !F0 || !ewt) { set_error("null argument"); return -1; }
+    FVP_HIP(hipSetDevice(c->device));
+    if (jac_init(c)) return -1;
+    fvp_jacdata *jd = c->jd;
+    hipStream_t s = c->stream;
+    fvp_csr_set_stream(jd->J, s);
+    const int N = c->N;
+    const double eps = 1.4901161193847656e-08;  // sqrt(2^-52)
+    shift_kernel<<<nblk(N, 256), 256, 0, s>>>(N, eps, h, psi, psip, ewt, jd->shift.p);
+    vcopy<<<nblk(N, 256), 256, 0, s>>>(N, psi, jd->ys.p);
+    vcopy<<<nblk(N, 256), 256, 0, s>>>(N, psip, jd->yps.p);
+    for (int k = 0; k < jd->ncol; ++k) {
+        int nc = jd->cptr[k + 1] - jd->cptr[k];
+        int ne = jd->eptr[k + 1] - jd->eptr[k];
+        perturb_kernel<<<nblk(nc, 256), 256, 0, s>>>(nc, jd->cols.p + jd->cptr[k], jd->shift.p, c_j, psi, psip,
+                                                     jd->ys.p, jd->yps.p, 0);
+        if (residual_impl(c, t, jd->ys.p, jd->yps.p, jd->Fs.p, c->averaging)) return -1;
+        fd_scatter<<<nblk(ne, 256), 256, 0, s>>>(ne, jd->erow.p + jd->eptr[k], jd->epos.p + jd->eptr[k],
+                                                 jd->ecol.p + jd->eptr[k], F0, jd->Fs.p, jd->shift.p, jd->J->a.p);
+        perturb_kernel<<<nblk(nc, 256), 256, 0, s>>>(nc, jd->cols.p + jd->cptr[k], jd->shift.p, c_j, psi, psip,
+                                                     jd->ys.p, jd->yps.p, 1);
+    }
+    FVP_HIP(hipGetLastError());
+    return precon_build_impl(jd->J, type);
+}
+
+extern "C" int fvp_precon_solve(fvp_ctx *c, const double *r, double *z) {
+    if (!c || !r || !z) { set_error("null argument"); return -1; }
+    if (!c->jd || c->jd->J->ptype < 0) { set_error("fvp_precon_setup has not run"); return -1; }
+    FVP_HIP(hipSetDevice(c->device));
+    fvp_csr_set_stream(c->jd->J, c->stream);
+    return precon_apply_impl(c->jd->J, r, z);
+}
+
+extern "C" int fvp_jacobian(fvp_ctx *c, fvp_csr **A) {
+    if (!c || !A) { set_error("null argument"); return -1; }
+    FVP_HIP(hipSetDevice(c->device));
+    if (jac_init(c)) return -1;
+    *A = c->jd->J;
+    return 0;
+}
+
+extern "C" int fvp_colours(fvp_ctx *c, int *n_colours, int *colour_host) {
+    if (!c || !n_colours) { set_error("null argument"); return -1; }
+    FVP_HIP(hipSetDevice(c->device));
+    if (jac_init(c)) return -1;
+    *n_colours = c->jd->ncol;
+    if (colour_host) std::copy(c->jd->colour.begin(), c->jd->colour.end(), colour_host);
+    return 0;
+}
+
+extern "C" int fvp_newton_gmres(fvp_ctx *c, double t, const double *psi, const double *psip, const double *F0,
+                                double c_j, const double *b, double *x, const fvp_gmres_params *p,
+                                fvp_gmres_stats *st) {
+    if (!c || !psi || !psip || !F0 || !b || !x || !p) { set_error("null argument"); return -1; }
+    FVP_HIP(hipSetDevice(c->device));
+    if (jac_init(c)) return -1;
+    fvp_csr *W = c->jd->J;
+    fvp_csr_set_stream(W, c->stream);
+    Op op = [=](const double *in, double *out) { return fvp_jv(c, t, psi, psip, F0, c_j, in, out); };
+    Op pc;
+    if (W->ptype > 0) pc = [W](const double *in, double *out) { return precon_apply_impl(W, in, out); };
+    return gmres_device(W, c->stream, c->N, op, pc, b, x, p, st);
+}

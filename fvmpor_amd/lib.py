@@ -39,7 +39,46 @@ SIGNATURES = {
     "fvp_capacity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "fvp_jv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
                               ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
+    "fvp_csr_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p,
+                                      ctypes.POINTER(ctypes.c_void_p)]),
+    "fvp_csr_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "fvp_csr_set_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "fvp_csr_set_values": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "fvp_csr_get_values": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p]),
+    "fvp_csr_gemv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_double,
+                                    ctypes.c_void_p]),
+    "fvp_precon_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "fvp_precon_apply": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fvp_precon_values": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p]),
+    "fvp_gmres_csr": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p]),
+    "fvp_precon_setup": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_int]),
+    "fvp_precon_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fvp_jacobian": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "fvp_colours": (ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p]),
+    "fvp_newton_gmres": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p]),
 }
+
+PRECON_NONE, PRECON_JACOBI, PRECON_ILU0 = 0, 2, 5
+ERR_PIVOT, ERR_NOCONV = 1, 2
+
+
+class GmresParams(ctypes.Structure):
+    _fields_ = [("dim_krylov", ctypes.c_int), ("max_restarts", ctypes.c_int), ("reorthogonalise", ctypes.c_int),
+                ("tol", ctypes.c_double)]
+
+
+class GmresStats(ctypes.Structure):
+    _fields_ = [("restarts", ctypes.c_int), ("iterations", ctypes.c_int), ("converged", ctypes.c_int),
+                ("residual", ctypes.c_double), ("error", ctypes.c_double)]
+
+    def as_dict(self):
+        return dict(restarts=self.restarts, iterations=self.iterations, converged=bool(self.converged),
+                    residual=self.residual, error=self.error)
 
 
 class FvpError(RuntimeError):
@@ -166,3 +205,99 @@ class Context:
     def jv(self, t, psi, psip, F0, cj, v, Jv):
         _check(lib().fvp_jv(self.h, float(t), _ptr(psi), _ptr(psip), _ptr(F0), float(cj), _ptr(v), _ptr(Jv)),
                "fvp_jv")
+
+
+class CsrMatrix:
+    """Device CSR matrix with its preconditioner (the decomp Tmtx_CRS + Tprecon pair)."""
+
+    def __init__(self, ia, ja, device: int = 0, handle=None):
+        self.owned = handle is None
+        if handle is None:
+            self._ia = np.ascontiguousarray(ia, dtype=np.int32)
+            self._ja = np.ascontiguousarray(ja, dtype=np.int32)
+            h = ctypes.c_void_p()
+            _check(lib().fvp_csr_create(int(device), len(self._ia) - 1, self._ia.ctypes.data_as(_c_int_p),
+                                        self._ja.ctypes.data_as(_c_int_p), ctypes.byref(h)), "fvp_csr_create")
+            self.h = h
+            self.n = len(self._ia) - 1
+            self.nnz = int(self._ia[-1])
+        else:
+            self.h = handle
+            self.n = self.nnz = None
+
+    def close(self):
+        if self.owned and getattr(self, "h", None):
+            lib().fvp_csr_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_stream(self, stream_ptr: int):
+        _check(lib().fvp_csr_set_stream(self.h, ctypes.c_void_p(stream_ptr)), "fvp_csr_set_stream")
+
+    def set_values(self, vals):
+        _check(lib().fvp_csr_set_values(self.h, _ptr(vals)), "fvp_csr_set_values")
+
+    def get_values(self, count):
+        out = np.zeros(count)
+        _check(lib().fvp_csr_get_values(self.h, out.ctypes.data_as(_c_dbl_p)), "fvp_csr_get_values")
+        return out
+
+    def gemv(self, alpha, x, beta, y):
+        _check(lib().fvp_csr_gemv(self.h, float(alpha), _ptr(x), float(beta), _ptr(y)), "fvp_csr_gemv")
+
+    def precon_build(self, ptype) -> int:
+        return _check(lib().fvp_precon_build(self.h, int(ptype)), "fvp_precon_build")
+
+    def precon_apply(self, x, y):
+        _check(lib().fvp_precon_apply(self.h, _ptr(x), _ptr(y)), "fvp_precon_apply")
+
+    def precon_values(self, count):
+        out = np.zeros(count)
+        _check(lib().fvp_precon_values(self.h, out.ctypes.data_as(_c_dbl_p)), "fvp_precon_values")
+        return out
+
+    def gmres(self, b, x, m=30, max_restarts=20, tol=1e-10, reorth=1):
+        p = GmresParams(m, max_restarts, reorth, tol)
+        st = GmresStats()
+        rc = _check(lib().fvp_gmres_csr(self.h, _ptr(b), _ptr(x), ctypes.byref(p), ctypes.byref(st)),
+                    "fvp_gmres_csr")
+        return rc, st.as_dict()
+
+
+def _ctx_precon_setup(self, t, cj, h, psi, psip, F0, ewt, ptype=PRECON_ILU0) -> int:
+    return _check(lib().fvp_precon_setup(self.h, float(t), float(cj), float(h), _ptr(psi), _ptr(psip), _ptr(F0),
+                                         _ptr(ewt), int(ptype)), "fvp_precon_setup")
+
+
+def _ctx_precon_solve(self, r, z):
+    _check(lib().fvp_precon_solve(self.h, _ptr(r), _ptr(z)), "fvp_precon_solve")
+
+
+def _ctx_jacobian(self) -> CsrMatrix:
+    h = ctypes.c_void_p()
+    _check(lib().fvp_jacobian(self.h, ctypes.byref(h)), "fvp_jacobian")
+    return CsrMatrix(None, None, handle=h)
+
+
+def _ctx_colours(self):
+    nc = ctypes.c_int()
+    col = np.zeros(self.N, dtype=np.int32)
+    _check(lib().fvp_colours(self.h, ctypes.byref(nc), col.ctypes.data_as(_c_int_p)), "fvp_colours")
+    return nc.value, col
+
+
+def _ctx_newton_gmres(self, t, psi, psip, F0, cj, b, x, m=30, max_restarts=10, tol=1e-8, reorth=1):
+    p = GmresParams(m, max_restarts, reorth, tol)
+    st = GmresStats()
+    rc = _check(lib().fvp_newton_gmres(self.h, float(t), _ptr(psi), _ptr(psip), _ptr(F0), float(cj), _ptr(b),
+                                       _ptr(x), ctypes.byref(p), ctypes.byref(st)), "fvp_newton_gmres")
+    return rc, st.as_dict()
+
+
+Context.precon_setup = _ctx_precon_setup
+Context.precon_solve = _ctx_precon_solve
+Context.jacobian = _ctx_jacobian
+Context.colours = _ctx_colours
+Context.newton_gmres = _ctx_newton_gmres

@@ -95,6 +95,78 @@ int fvp_capacity(fvp_ctx *ctx, double *ahh);
 int fvp_jv(fvp_ctx *ctx, double t, const double *psi, const double *psip, const double *F0,
            double c_j, const double *v, double *Jv);
 
+/* ---------------------------------------------------------------------------
+ * Sparse linear algebra (the decomp C API on the device).
+ * A CSR matrix is 0-based with sorted column indices per row (Tmtx_CRS with
+ * block = 0, linalg_sparse.h:21-50).  Preconditioner type tags follow
+ * precon.h:8-13.  decomp's build calls return 1 on success (precon.c:151-195);
+ * here 0 = success and a missing / zero pivot returns FVP_ERR_PIVOT (> 0). */
+#define FVP_PRECON_NONE 0
+#define FVP_PRECON_JACOBI 2
+#define FVP_PRECON_ILU0 5
+#define FVP_ERR_PIVOT 1
+#define FVP_ERR_NOCONV 2
+
+typedef struct fvp_csr fvp_csr;
+
+/* Pattern from host arrays (ia: n+1, ja: ia[n]); values are set separately. */
+int fvp_csr_create(int device, int n, const int *ia, const int *ja, fvp_csr **out);
+int fvp_csr_destroy(fvp_csr *A);
+int fvp_csr_set_stream(fvp_csr *A, void *hip_stream);
+/* values: nnz doubles, device pointer (copied) */
+int fvp_csr_set_values(fvp_csr *A, const double *values);
+/* host copy of the matrix values (nnz doubles) */
+int fvp_csr_get_values(fvp_csr *A, double *values_host);
+/* y = alpha A x + beta y (mtx_CRS_gemv, linalg_sparse.c:1321-1348), device vectors */
+int fvp_csr_gemv(fvp_csr *A, double alpha, const double *x, double beta, double *y);
+/* build the preconditioner of A (precon(), precon.c:151-195; ILU0_factor,
+ * ILU.c:12-115 with the diagonal of row i inverted; precon_jacobi, MR.c:143-200) */
+int fvp_precon_build(fvp_csr *A, int type);
+/* y = M^{-1} x (precon_apply, precon.c:112-149) */
+int fvp_precon_apply(fvp_csr *A, const double *x, double *y);
+/* host copy of the factored values (ILU(0): L unit-lower, U with inverted diagonal) */
+int fvp_precon_values(fvp_csr *A, double *values_host);
+
+/* GMRES parameters and statistics (Tgmres, gmres.h:15-32) */
+typedef struct {
+    int dim_krylov;       /* restart length m */
+    int max_restarts;
+    int reorthogonalise;  /* 1: classical Gram-Schmidt applied twice (CGS2) */
+    double tol;           /* ||b - A x|| / ||b|| */
+} fvp_gmres_params;
+typedef struct {
+    int restarts;
+    int iterations;       /* Arnoldi steps over all restarts */
+    int converged;
+    double residual;      /* ||b - A x|| */
+    double error;         /* residual / ||b|| */
+} fvp_gmres_stats;
+
+/* Right-preconditioned restarted GMRES on A x = b with A's built
+ * preconditioner (gmres(), gmres.c:26-201).  x holds the initial guess. */
+int fvp_gmres_csr(fvp_csr *A, const double *b, double *x, const fvp_gmres_params *p, fvp_gmres_stats *st);
+
+/* ---------------------------------------------------------------------------
+ * Newton-Krylov pieces on a problem context (IDA psetup / psolve and SPGMR,
+ * ida_integrator.h:478-587, preconditioner_dss.cpp:284-405). */
+
+/* Coloured finite-difference Jacobian of the iteration matrix, stored as
+ * -(dF/dpsi + c_j dF/dpsi') like the reference (preconditioner_dss.cpp:317-357),
+ * then factored with the given preconditioner type.  ewt: IDA error weights
+ * (device).  Returns FVP_ERR_PIVOT if the factorisation meets a zero pivot. */
+int fvp_precon_setup(fvp_ctx *ctx, double t, double c_j, double h, const double *psi, const double *psip,
+                     const double *F0, const double *ewt, int type);
+/* z = M^{-1} r with the last setup (device vectors) */
+int fvp_precon_solve(fvp_ctx *ctx, const double *r, double *z);
+/* the context's Jacobian as a CSR handle (pattern = node adjacency plus diagonal) */
+int fvp_jacobian(fvp_ctx *ctx, fvp_csr **A);
+/* number of colours of the distance-2 colouring (preconditioner_dss.cpp:72-94) */
+int fvp_colours(fvp_ctx *ctx, int *n_colours, int *colour_host);
+/* Solve J x = b, J = dF/dpsi + c_j dF/dpsi' applied matrix-free (fvp_jv) and
+ * right-preconditioned by the last fvp_precon_setup. */
+int fvp_newton_gmres(fvp_ctx *ctx, double t, const double *psi, const double *psip, const double *F0, double c_j,
+                     const double *b, double *x, const fvp_gmres_params *p, fvp_gmres_stats *st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,26 @@ def lib():
         L.orc_num_dirichlet.restype = ctypes.c_int
         L.orc_num_dirichlet.argtypes = [_vp]
         L.orc_vg.argtypes = [ctypes.c_double, _dp, _dp]
+        L.orc_csr_gemv.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _dp]
+        L.orc_ilu0_factor.restype = ctypes.c_int
+        L.orc_ilu0_factor.argtypes = [ctypes.c_int, _ip, _ip, _dp, _ip]
+        L.orc_ilu0_apply.argtypes = [ctypes.c_int, _ip, _ip, _dp, _ip, _dp, _dp]
+        L.orc_jacobi_factor.restype = ctypes.c_int
+        L.orc_jacobi_factor.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp]
+        L.orc_gmres_csr.restype = ctypes.c_int
+        L.orc_gmres_csr.argtypes = [ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, _dp, _dp, _ip, ctypes.c_double, _dp]
+        L.orc_arnoldi_csr.restype = ctypes.c_int
+        L.orc_arnoldi_csr.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_jac_pattern.restype = ctypes.c_int
+        L.orc_jac_pattern.argtypes = [_vp, _ip, ctypes.c_void_p]
+        L.orc_colouring.restype = ctypes.c_int
+        L.orc_colouring.argtypes = [ctypes.c_int, _ip, _ip, _ip]
+        L.orc_fd_jacobian.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp,
+                                      _dp, _dp, _ip, _ip, _ip, ctypes.c_int, _dp]
+        L.orc_jv.argtypes = [_vp, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, ctypes.c_double, _dp, _dp]
+        L.orc_newton_gmres.restype = ctypes.c_int
+        L.orc_newton_gmres.argtypes = [_vp, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, ctypes.c_double, _ip, _ip,
+                                       _dp, ctypes.c_int, _dp, _dp, _ip, ctypes.c_double, _dp]
         _LIB = L
     return _LIB
 
@@ -173,3 +193,98 @@ class OracleProblem:
 
     def flux_scale(self):
         return lib().orc_flux_scale(self.h)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def csr_gemv(ia, ja, a, x):
+    n = len(ia) - 1
+    y = np.zeros(n)
+    lib().orc_csr_gemv(n, _i32(ia), _i32(ja), _f64(a), _f64(x), y)
+    return y
+
+
+def ilu0(ia, ja, a):
+    """ILU(0) values (L unit lower, U with inverted diagonal) and diagonal positions."""
+    n = len(ia) - 1
+    lu = _f64(a).copy()
+    dp = np.zeros(n, dtype=np.int32)
+    ok = lib().orc_ilu0_factor(n, _i32(ia), _i32(ja), lu, dp)
+    return (lu, dp) if ok else (None, None)
+
+
+def ilu0_apply(ia, ja, lu, dp, x):
+    n = len(ia) - 1
+    y = np.zeros(n)
+    lib().orc_ilu0_apply(n, _i32(ia), _i32(ja), _f64(lu), _i32(dp), _f64(x), y)
+    return y
+
+
+def jacobi(ia, ja, a):
+    n = len(ia) - 1
+    J = np.zeros(n)
+    return J if lib().orc_jacobi_factor(n, _i32(ia), _i32(ja), _f64(a), J) else None
+
+
+def gmres_csr(ia, ja, a, b, x0=None, precon=0, m=30, max_restarts=20, tol=1e-10, reorth=0):
+    n = len(ia) - 1
+    x = np.zeros(n) if x0 is None else _f64(x0).copy()
+    st = np.zeros(5)
+    ok = lib().orc_gmres_csr(n, _i32(ia), _i32(ja), _f64(a), int(precon), _f64(b), x,
+                             np.array([m, max_restarts, reorth], dtype=np.int32), float(tol), st)
+    if not ok:
+        raise ValueError("preconditioner build failed")
+    return x, dict(restarts=int(st[0]), iterations=int(st[1]), converged=bool(st[2]), residual=st[3], error=st[4])
+
+
+def arnoldi_csr(ia, ja, a, b, m, reorth=0):
+    n = len(ia) - 1
+    Q = np.zeros(n * (m + 1))
+    H = np.zeros((m + 1) * m)
+    j = lib().orc_arnoldi_csr(n, _i32(ia), _i32(ja), _f64(a), _f64(b), m, reorth, Q, H)
+    return j, Q.reshape(m + 1, n).T, H.reshape(m, m + 1).T
+
+
+def jac_pattern(omesh):
+    ia = np.zeros(omesh.N + 1, dtype=np.int32)
+    nnz = lib().orc_jac_pattern(omesh.h, ia, None)
+    ja = np.zeros(nnz, dtype=np.int32)
+    lib().orc_jac_pattern(omesh.h, ia, ja.ctypes.data)
+    return ia, ja
+
+
+def colouring(ia, ja):
+    n = len(ia) - 1
+    col = np.zeros(n, dtype=np.int32)
+    nc = lib().orc_colouring(n, _i32(ia), _i32(ja), col)
+    return nc, col
+
+
+def fd_jacobian(oprob, t, c, h, y, yp, F0, ewt, ia, ja, colour, ncol):
+    vals = np.zeros(len(ja))
+    lib().orc_fd_jacobian(oprob.h, oprob.N, float(t), float(c), float(h), _f64(y), _f64(yp), _f64(F0), _f64(ewt),
+                          _i32(ia), _i32(ja), _i32(colour), int(ncol), vals)
+    return vals
+
+
+def jv(oprob, t, y, yp, F0, c, v):
+    out = np.zeros(oprob.N)
+    lib().orc_jv(oprob.h, oprob.N, float(t), _f64(y), _f64(yp), _f64(F0), float(c), _f64(v), out)
+    return out
+
+
+def newton_gmres(oprob, t, y, yp, F0, c, ia, ja, jvals, b, precon=5, m=30, max_restarts=10, tol=1e-8, reorth=1):
+    x = np.zeros(oprob.N)
+    st = np.zeros(5)
+    ok = lib().orc_newton_gmres(oprob.h, oprob.N, float(t), _f64(y), _f64(yp), _f64(F0), float(c), _i32(ia),
+                                _i32(ja), _f64(jvals), int(precon), _f64(b), x,
+                                np.array([m, max_restarts, reorth], dtype=np.int32), float(tol), st)
+    if not ok:
+        raise ValueError("preconditioner build failed")
+    return x, dict(restarts=int(st[0]), iterations=int(st[1]), converged=bool(st[2]), residual=st[3], error=st[4])
