@@ -282,7 +282,7 @@ using Op = std::function<int(const double *, double *)>;
 
 // Restarted right-preconditioned GMRES (gmres.c:26-201).  Each Arnoldi step
 // stops the restart cycle early once the Givens residual estimate meets tol.
-static int gmres_device(fvp_csr *W, hipStream_t s, long n, const Op &A, const Op &M, const double *b, double *x,
+int gmres_device(fvp_csr *W, hipStream_t s, long n, const Op &A, const Op &M, const double *b, double *x,
                         const fvp_gmres_params *p, fvp_gmres_stats *st) {
     const int m = p->dim_krylov;
     if (m < 1 || m > 63) { set_error("dim_krylov must be 1..63"); return -1; }
@@ -739,6 +739,12 @@ static int jac_init(fvp_ctx *c) {
     }
     c->jd = jd;
     return 0;
+}
+
+fvp_csr *jacobian_handle(fvp_ctx *c) {
+    if (jac_init(c)) return nullptr;
+    fvp_csr_set_stream(c->jd->J, c->stream);
+    return c->jd->J;
 }
 
 void jac_free(fvp_ctx *c) {

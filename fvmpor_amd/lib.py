@@ -61,6 +61,14 @@ SIGNATURES = {
     "fvp_newton_gmres": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_void_p]),
+    "fvp_integrator_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+                                             ctypes.POINTER(ctypes.c_void_p)]),
+    "fvp_integrator_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "fvp_integrator_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "fvp_integrator_advance": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p]),
+    "fvp_integrator_state": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p, ctypes.POINTER(ctypes.c_void_p),
+                                            ctypes.POINTER(ctypes.c_void_p)]),
+    "fvp_integrator_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 PRECON_NONE, PRECON_JACOBI, PRECON_ILU0 = 0, 2, 5
@@ -301,3 +309,63 @@ Context.precon_solve = _ctx_precon_solve
 Context.jacobian = _ctx_jacobian
 Context.colours = _ctx_colours
 Context.newton_gmres = _ctx_newton_gmres
+
+
+class IntegratorParams(ctypes.Structure):
+    _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("max_order", ctypes.c_int),
+                ("h0", ctypes.c_double), ("h_max", ctypes.c_double), ("precon_type", ctypes.c_int),
+                ("dim_krylov", ctypes.c_int), ("max_restarts", ctypes.c_int), ("lin_factor", ctypes.c_double),
+                ("newton_tol", ctypes.c_double), ("max_newton", ctypes.c_int), ("max_steps", ctypes.c_int)]
+
+
+class IntegratorStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_long), ("error_fails", ctypes.c_long), ("newton_fails", ctypes.c_long),
+                ("residuals", ctypes.c_long), ("newton_iters", ctypes.c_long), ("gmres_iters", ctypes.c_long),
+                ("precon_setups", ctypes.c_long), ("t", ctypes.c_double), ("h_last", ctypes.c_double),
+                ("order_last", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Integrator:
+    """BDF / Newton-GMRES time integrator on a Context (fvp_integrator_*)."""
+
+    def __init__(self, ctx: Context, psi0, t0=0.0, rtol=1e-8, atol=1e-8, max_order=3, h0=0.0, h_max=0.0,
+                 precon=PRECON_ILU0, dim_krylov=20, max_restarts=5, lin_factor=0.05, newton_tol=0.33,
+                 max_newton=4, max_steps=1000000):
+        self.ctx = ctx
+        self.params = IntegratorParams(rtol, atol, max_order, h0, h_max, precon, dim_krylov, max_restarts,
+                                       lin_factor, newton_tol, max_newton, max_steps)
+        h = ctypes.c_void_p()
+        _check(lib().fvp_integrator_create(ctx.h, ctypes.byref(self.params), float(t0), _ptr(psi0),
+                                           ctypes.byref(h)), "fvp_integrator_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fvp_integrator_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, t_stop) -> int:
+        return _check(lib().fvp_integrator_step(self.h, float(t_stop)), "fvp_integrator_step")
+
+    def advance(self, t_out, psi_out=None) -> int:
+        return _check(lib().fvp_integrator_advance(self.h, float(t_out), _ptr(psi_out) if psi_out is not None
+                                                   else ctypes.c_void_p(0)), "fvp_integrator_advance")
+
+    def state(self):
+        t = ctypes.c_double()
+        y = ctypes.c_void_p()
+        yp = ctypes.c_void_p()
+        _check(lib().fvp_integrator_state(self.h, ctypes.byref(t), ctypes.byref(y), ctypes.byref(yp)),
+               "fvp_integrator_state")
+        return t.value, y.value, yp.value
+
+    def stats(self):
+        st = IntegratorStats()
+        _check(lib().fvp_integrator_get_stats(self.h, ctypes.byref(st)), "fvp_integrator_get_stats")
+        return st.as_dict()

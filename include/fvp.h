@@ -167,6 +167,48 @@ int fvp_colours(fvp_ctx *ctx, int *n_colours, int *colour_host);
 int fvp_newton_gmres(fvp_ctx *ctx, double t, const double *psi, const double *psip, const double *F0, double c_j,
                      const double *b, double *x, const fvp_gmres_params *p, fvp_gmres_stats *st);
 
+/* ---------------------------------------------------------------------------
+ * Time integration: variable-step, variable-order (1..max_order) BDF with
+ * modified Newton and right-preconditioned, error-weight-scaled GMRES on
+ * F(t, psi, psi') = 0 -- the role SUNDIALS IDAS plays for the reference
+ * (IDAIntegrator, ida_integrator.h:169-340; rtol/atol and max order 3 at
+ * fvmpor_ODE.cpp:79-80,176-180).  Upwind weights are frozen at the start of
+ * every step (preprocess_timestep, fvmpor_ODE_impl.cpp:118-142). */
+typedef struct fvp_integrator fvp_integrator;
+
+typedef struct {
+    double rtol, atol;    /* error weights ewt = 1 / (rtol |psi| + atol) */
+    int max_order;        /* 1..5 */
+    double h0;            /* first step; 0 = from the initial residual */
+    double h_max;         /* 0 = unlimited */
+    int precon_type;      /* FVP_PRECON_ILU0 / FVP_PRECON_JACOBI / FVP_PRECON_NONE */
+    int dim_krylov;       /* GMRES restart length (IDA maxl) */
+    int max_restarts;
+    double lin_factor;    /* linear tolerance = lin_factor * newton_tol (IDA eplifac, 0.05) */
+    double newton_tol;    /* IDA epsNewt, 0.33 */
+    int max_newton;       /* IDA maxcor, 4 */
+    int max_steps;        /* per advance call */
+} fvp_integrator_params;
+
+typedef struct {
+    long steps, error_fails, newton_fails, residuals, newton_iters, gmres_iters, precon_setups;
+    double t, h_last;
+    int order_last;
+} fvp_integrator_stats;
+
+/* psi0: device, n_nodes.  Consistent psi'(t0) = F/ahh on non-Dirichlet nodes with
+ * ahh != 0, 0 elsewhere (Physics::initialise, fvmpor_ODE_impl.cpp:34-60). */
+int fvp_integrator_create(fvp_ctx *ctx, const fvp_integrator_params *p, double t0, const double *psi0,
+                          fvp_integrator **out);
+int fvp_integrator_destroy(fvp_integrator *I);
+/* One accepted step, never past t_stop (IDASolve IDA_ONE_STEP with a stop time). */
+int fvp_integrator_step(fvp_integrator *I, double t_stop);
+/* Step until t_out (Solver::advance(t), solver.h:176-180); psi(t_out) into psi_out (device). */
+int fvp_integrator_advance(fvp_integrator *I, double t_out, double *psi_out);
+/* current state (device pointers owned by the integrator) */
+int fvp_integrator_state(fvp_integrator *I, double *t, const double **psi, const double **psip);
+int fvp_integrator_get_stats(const fvp_integrator *I, fvp_integrator_stats *st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,10 @@ def lib():
         L.orc_newton_gmres.restype = ctypes.c_int
         L.orc_newton_gmres.argtypes = [_vp, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, ctypes.c_double, _ip, _ip,
                                        _dp, ctypes.c_int, _dp, _dp, _ip, ctypes.c_double, _dp]
+        L.orc_integrate_replay.restype = ctypes.c_int
+        L.orc_integrate_replay.argtypes = [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                           _dp, ctypes.c_int, _dp, _ip, _dp, ctypes.c_int, _dp]
+        L.orc_initial_derivative.argtypes = [_vp, ctypes.c_int, ctypes.c_double, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -288,3 +292,20 @@ def newton_gmres(oprob, t, y, yp, F0, c, ia, ja, jvals, b, precon=5, m=30, max_r
     if not ok:
         raise ValueError("preconditioner build failed")
     return x, dict(restarts=int(st[0]), iterations=int(st[1]), converged=bool(st[2]), residual=st[3], error=st[4])
+
+
+def integrate_replay(oprob, psi0, hs, ks, wfs, t0=0.0, rtol=1e-8, atol=1e-8, newton_tol=1e-6):
+    """Replay a BDF step sequence (sizes hs, orders ks, per-step upwind front weights wfs)."""
+    out = np.zeros(oprob.N)
+    wfs = _f64(np.asarray(wfs).reshape(-1))
+    n = lib().orc_integrate_replay(oprob.h, float(rtol), float(atol), float(newton_tol), float(t0), _f64(psi0),
+                                   len(hs), _f64(hs), _i32(ks), wfs, oprob.NE, out)
+    if n < 0:
+        raise RuntimeError("oracle replay: Newton failed")
+    return out, n
+
+
+def initial_derivative(oprob, t0, psi0):
+    yp = np.zeros(oprob.N)
+    lib().orc_initial_derivative(oprob.h, oprob.N, float(t0), _f64(psi0), yp)
+    return yp

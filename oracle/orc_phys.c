@@ -367,3 +367,5 @@ void orc_dirichlet(const OrcProblem *p, int *nodes, double *vals) {
     memcpy(nodes, p->dir_node, sizeof(int) * p->ndir);
     memcpy(vals, p->dir_val, sizeof(double) * p->ndir);
 }
+
+OrcMesh *orc_problem_mesh(OrcProblem *p) { return p->m; }

