@@ -252,7 +252,9 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
         } else {
             double rate = std::pow(delnrm / oldnrm, 1.0 / m);
             if (rate > 0.9) break;
-            I->ss = rate / (1. - rate);
+            // IDA carries ss into the next step's first iteration; floored here so one
+            // fast-contracting step cannot accept a large first update on the next
+            I->ss = std::max(rate / (1. - rate), 0.05);
             if (I->ss * delnrm <= I->p.newton_tol) { conv = true; break; }
         }
     }

@@ -197,7 +197,15 @@ def box_mesh_3d(nx: int = 160, ny: int = 160, nz: int = 40, lx: float = 100.0, l
 
     rng = np.random.default_rng(seed)
     pts = _jittered_lattice((lx, ly, lz), (nx, ny, nz), rng, jitter)
-    tet = Delaunay(pts).simplices.astype(np.int64)
+    # Exactly coplanar boundary points make Qhull crawl; triangulate a copy whose
+    # boundary points are moved by 1e-7 cells off their plane, keep exact
+    # coordinates, and drop the tetrahedra that become flat again.
+    h = np.array([lx / (nx - 1), ly / (ny - 1), lz / (nz - 1)])
+    tmp = pts.copy()
+    for ax, L in enumerate((lx, ly, lz)):
+        on = np.isclose(pts[:, ax], 0.0) | np.isclose(pts[:, ax], L)
+        tmp[on, ax] += rng.uniform(-1e-7, 1e-7, on.sum()) * h[ax]
+    tet = Delaunay(tmp).simplices.astype(np.int64)
     p = pts[tet]
     vol6 = np.einsum("ij,ij->i", p[:, 1] - p[:, 0], np.cross(p[:, 2] - p[:, 0], p[:, 3] - p[:, 0]))
     cell = (lx / (nx - 1)) * (ly / (ny - 1)) * (lz / (nz - 1))

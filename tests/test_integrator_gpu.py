@@ -29,7 +29,7 @@ def _gpu_run(prob, T, **kw):
     from fvmpor_amd.lib import Context, Integrator
     ctx = Context(prob)
     psi0 = torch.tensor(prob.psi0, device=device())
-    integ = Integrator(ctx, psi0, rtol=1e-6, atol=1e-6, newton_tol=1e-5, **kw)
+    integ = Integrator(ctx, psi0, rtol=1e-6, atol=1e-6, newton_tol=1e-6, **kw)
     hs, ks, wfs = [], [], []
     t = 0.0
     while t < T:
