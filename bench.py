@@ -1,0 +1,193 @@
+"""Benchmark: simulated days per wall-second of the plain FP64 path on the 1M-node
+3D heterogeneous Richards box (100 x 100 x 20 m, jittered 160 x 160 x 40 lattice,
+Delaunay tetrahedra, log-normal Ks, four van Genuchten soil classes, water table
+at 5 m, 0.02 m/day rain), BDF / Newton-GMRES / ILU(0) on one MI355X.
+
+A "step" is one accepted time step of the integrator (begin-step upwinding,
+predictor, Newton iterations each with a residual, matrix-free GMRES with
+ILU(0) of the coloured FD Jacobian).  W untimed warm-up steps from t = 0,
+then exactly K timed steps; value = simulated days advanced in the timed
+steps / wall seconds.  Inputs live in HBM before the timed region starts.
+
+Prints one JSON line (see the fields below).  Roofline: the residual's
+element-flux kernel, algorithmic bytes B_F = 48 N + 28 T + N_edges per launch
+(state, coordinates, connectivity, conductivity, class, one upwind byte per
+edge), divided by its average duration measured with HIP events on the
+context stream over the timed steps; peak 8 TB/s.  CPU baseline: the CPU
+oracle (a C restatement of the reference's residual) timed on a bounded
+sample, scaled to sim-days/wall-s (see "sample" in the output).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "sim-days/wall-s, 1M-node 3D Richards: FP32-precon+learned vs plain FP64 @1e-6"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--nx", type=int, default=160)
+    ap.add_argument("--ny", type=int, default=160)
+    ap.add_argument("--nz", type=int, default=40)
+    ap.add_argument("--rtol", type=float, default=1e-8)
+    ap.add_argument("--atol", type=float, default=1e-8)
+    ap.add_argument("--precon", default="ilu0", choices=["ilu0", "jacobi"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
+    ap.add_argument("--cpu-nx", type=int, default=80, help="lattice size of the CPU sample box")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(args, gpu_window_days, gpu_window_residuals, n_nodes_gpu):
+    """Oracle residual evaluations on a smaller box of the same generator and
+    statistics, time per node-evaluation scaled to the full mesh, times the
+    residual evaluations the GPU needed for the timed window."""
+    from fvmpor_amd.problems import box3d_problem
+    from oracle.oracle import OracleProblem
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    nx = args.cpu_nx
+    prob = box3d_problem(nx=nx, ny=nx, nz=max(4, nx // 4))
+    op = OracleProblem(prob)
+    psi = prob.psi0 - 0.5
+    psip = np.zeros_like(psi)
+    op.begin_step(0.0, psi, psip)
+    op.residual(0.0, psi, psip)
+    n_eval, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 10.0 or n_eval < 3:
+        op.residual(0.0, psi, psip)
+        n_eval += 1
+    per_eval = (time.perf_counter() - t0) / n_eval
+    per_eval_full = per_eval * n_nodes_gpu / prob.mesh.n_nodes
+    cpu_seconds = per_eval_full * gpu_window_residuals
+    return {
+        "value": gpu_window_days / cpu_seconds,
+        "unit": "sim-days/wall-s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{n_eval} CPU-oracle residual evaluations on the {prob.mesh.n_nodes}-node box of the same "
+                   f"generator ({per_eval*1e3:.1f} ms each), scaled by node count to {per_eval_full:.2f} s per "
+                   f"1M-node residual and by the {gpu_window_residuals} residual evaluations the GPU run "
+                   f"needed for the timed steps; preconditioner and Krylov vector work not counted"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is design-only.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from fvmpor_amd.lib import Context, Integrator, PRECON_ILU0, PRECON_JACOBI
+    from fvmpor_amd.problems import box3d_problem
+
+    t0 = time.perf_counter()
+    prob = box3d_problem(nx=args.nx, ny=args.ny, nz=args.nz)
+    log(f"mesh N={prob.mesh.n_nodes} T={prob.mesh.n_elements} built in {time.perf_counter() - t0:.1f}s")
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    ctx = Context(prob)
+    stream = torch.cuda.current_stream()
+    ctx.set_stream(stream.cuda_stream)
+    psi0 = torch.tensor(prob.psi0, device=dev)
+    pc = PRECON_ILU0 if args.precon == "ilu0" else PRECON_JACOBI
+    integ = Integrator(ctx, psi0, rtol=args.rtol, atol=args.atol, max_order=3, precon=pc)
+    ncol, _ = ctx.colours()
+    torch.cuda.synchronize()
+    log(f"setup {time.perf_counter() - t0:.1f}s, {ncol} colours, N_edges={ctx.NE}")
+    T_END = 1.0
+    for i in range(args.warmup):
+        integ.step(T_END)
+    torch.cuda.synchronize()
+    st0 = integ.stats()
+    ctx.profile(True)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    for i in range(args.steps):
+        integ.step(T_END)
+        if (i + 1) % 10 == 0:
+            s = integ.stats()
+            log(f"step {i + 1}/{args.steps}: t={s['t']:.4e} d, h={s['h_last']:.2e}, order {s['order_last']}")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    st1 = integ.stats()
+    ek_ms, ek_n = ctx.profile_get(0)
+    rs_ms, rs_n = ctx.profile_get(1)
+    pc_ms, pc_n = ctx.profile_get(2)
+    ctx.profile(False)
+    days = st1["t"] - st0["t"]
+    N, T, NE = ctx.N, ctx.T, ctx.NE
+    bf = 48 * N + 28 * T + NE
+    ek_avg_s = ek_ms / max(ek_n, 1) * 1e-3
+    achieved = bf / ek_avg_s / 1e9
+    window_res = st1["residuals"] - st0["residuals"]
+    log(f"timed {args.steps} steps: {days:.4e} days in {wall:.2f}s; element kernel {ek_avg_s*1e6:.1f} us x {ek_n}, "
+        f"residual {rs_ms/max(rs_n,1)*1e3:.1f} us x {rs_n}, precon solve {pc_ms/max(pc_n,1)*1e3:.1f} us x {pc_n}")
+    log(f"stats {st1}")
+    out = {
+        "metric": METRIC,
+        "value": days / wall,
+        "unit": "sim-days/wall-s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded jittered-lattice Delaunay mesh, FFT log-normal Ks, random soil layers",
+        "config": {
+            "workload": f"plain FP64 3D box {args.nx}x{args.ny}x{args.nz} lattice ({N} nodes, {T} tets), "
+                        f"BDF(<=3) Newton-GMRES {args.precon}, rtol=atol={args.rtol:g}, timed steps "
+                        f"{args.warmup + 1}..{args.warmup + args.steps} from t=0",
+            "nodes": N, "elements": T, "edges": NE,
+            "sim_days_timed": days, "t_start": st0["t"], "t_end": st1["t"],
+            "newton_iters": st1["newton_iters"] - st0["newton_iters"],
+            "gmres_iters": st1["gmres_iters"] - st0["gmres_iters"],
+            "precon_setups": st1["precon_setups"] - st0["precon_setups"],
+            "residual_evals": window_res,
+            "residual_us": rs_ms / max(rs_n, 1) * 1e3,
+            "precon_solve_us": pc_ms / max(pc_n, 1) * 1e3,
+            "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "kernel": "elem_kernel<3,0> (residual element fluxes)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": bf,
+            "avg_launch_us": ek_avg_s * 1e6,
+            "launches": ek_n,
+        },
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(args, days, window_res, N)
+        except Exception as exc:  # the GPU measurement stands on its own
+            log(f"cpu baseline failed: {exc!r}")
+            out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

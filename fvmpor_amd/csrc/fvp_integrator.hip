@@ -44,6 +44,7 @@ struct fvp_integrator {
     double h = 0., ss = 20., cj_setup = -1.;
     int k = 1, steps_at_k = 0, steps_since_setup = 0;
     bool need_setup = true;
+    long nres0 = 0;
     fvp_integrator_stats st{};
 };
 
@@ -185,7 +186,6 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
     };
     deriv();
     if (fvp::residual_impl(c, t, I->y.p, I->ypn.p, I->G.p, c->averaging)) return -1;
-    ++I->st.residuals;
     // preconditioner setup when stale (IDA: first step, after a failure, c_j moved by > 25%)
     double cjratio = I->cj_setup > 0 ? cj / I->cj_setup : 0.;
     if (I->p.precon_type != FVP_PRECON_NONE &&
@@ -209,7 +209,6 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
         if (m > 0) {
             deriv();
             if (fvp::residual_impl(c, t, I->y.p, I->ypn.p, I->G.p, c->averaging)) return -1;
-            ++I->st.residuals;
         }
         // b = -ewt * G ; solve (S J) delta = S (-G), right precond M S^-1
         mul_kernel<<<nb(N), 256, 0, s>>>(N, I->G.p, I->ewt.p, I->b.p);
@@ -238,7 +237,6 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
             };
         int rc = fvp::gmres_device(W, s, N, A, M, I->b.p, I->delta.p, &gp, &gs);
         I->st.gmres_iters += gs.iterations;
-        I->st.residuals += gs.iterations + gs.restarts + 1;
         if (rc < 0) return -1;
         if (rc > 0 && gs.error > 1.0) return 2;  // linear solve failed outright
         // IDA damps the update by 2/(1+cjratio) with a stale iteration matrix
@@ -388,6 +386,7 @@ extern "C" int fvp_integrator_create(fvp_ctx *c, const fvp_integrator_params *p,
     FVP_HIP(hipGetLastError());
     I->st.t = t0;
     I->h = p->h0;
+    I->nres0 = c->nres;
     *out = I;
     return 0;
 }
@@ -446,5 +445,6 @@ extern "C" int fvp_integrator_state(fvp_integrator *I, double *t, const double *
 extern "C" int fvp_integrator_get_stats(const fvp_integrator *I, fvp_integrator_stats *st) {
     if (!I || !st) { set_error("null argument"); return -1; }
     *st = I->st;
+    st->residuals = I->c->nres - I->nres0;
     return 0;
 }

@@ -84,12 +84,22 @@ struct fvp_ctx {
     fvp::DevBuf<double> red;       // reduction partials
     std::vector<double> red_h;
     fvp_jacdata *jd = nullptr;  // coloured FD Jacobian and its factorisation
+    // live timing: event pairs per kind (0 element kernel, 1 residual, 2 precon solve)
+    long nres = 0;  // residual evaluations issued on this context
+    int prof = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[3];
+    size_t prof_used[3] = {0, 0, 0};
+    double prof_ms[3] = {0, 0, 0};
+    long prof_n[3] = {0, 0, 0};
 };
 
 namespace fvp {
 int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, double *F, int averaging);
 int upload_bc_values(fvp_ctx *c);
 void jac_free(fvp_ctx *c);
+// record a start / stop event of kind k on the context stream when profiling
+void prof_mark(fvp_ctx *c, int k, int stop);
+void prof_release(fvp_ctx *c);
 }  // namespace fvp
 
 #define FVP_HIP(call)                                                                         \

@@ -792,7 +792,10 @@ extern "C" int fvp_precon_solve(fvp_ctx *c, const double *r, double *z) {
     if (!c->jd || c->jd->J->ptype < 0) { set_error("fvp_precon_setup has not run"); return -1; }
     FVP_HIP(hipSetDevice(c->device));
     fvp_csr_set_stream(c->jd->J, c->stream);
-    return precon_apply_impl(c->jd->J, r, z);
+    prof_mark(c, 2, 0);
+    int rc = precon_apply_impl(c->jd->J, r, z);
+    prof_mark(c, 2, 1);
+    return rc;
 }
 
 extern "C" int fvp_jacobian(fvp_ctx *c, fvp_csr **A) {

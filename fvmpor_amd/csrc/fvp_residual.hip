@@ -238,8 +238,11 @@ int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, d
     (void)t;
     hipStream_t s = c->stream;
     const int N = c->N, T = c->T;
+    ++c->nres;
+    prof_mark(c, 1, 0);
     FVP_HIP(hipMemsetAsync(c->acc.p, 0, sizeof(double) * N, s));
     vg_kernel<<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p, c->cap.p);
+    prof_mark(c, 0, 0);
     if (c->dim == 3) {
         if (averaging)
             elem_kernel<3, 1><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
@@ -259,11 +262,52 @@ int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, d
                                                            c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p,
                                                            c->acc.p, c->qface.p, c->aface.p);
     }
+    prof_mark(c, 0, 1);
     if (c->nb) boundary_kernel<<<blocks(c->nb, 256), 256, 0, s>>>(c->nb, c->bnode.p, c->bval.p, c->acc.p);
     finalize_kernel<<<blocks(N, 256), 256, 0, s>>>(N, c->acc.p, c->invV.p, psip, c->cap.p, F);
     if (c->ndir) dirichlet_kernel<<<blocks(c->ndir, 256), 256, 0, s>>>(c->ndir, c->dnode.p, c->dval.p, psi, F);
+    prof_mark(c, 1, 1);
     FVP_HIP(hipGetLastError());
     return 0;
+}
+
+// Event pairs are pooled per kind and folded into running totals every 4096 pairs.
+static void prof_fold(fvp_ctx *c, int k) {
+    if (!c->prof_used[k]) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (size_t i = 0; i < c->prof_used[k]; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->prof_ev[k][i].first, c->prof_ev[k][i].second) == hipSuccess) {
+            c->prof_ms[k] += ms;
+            c->prof_n[k] += 1;
+        }
+    }
+    c->prof_used[k] = 0;
+}
+
+void prof_mark(fvp_ctx *c, int k, int stop) {
+    if (!c->prof) return;
+    if (!stop) {
+        if (c->prof_used[k] == 4096) prof_fold(c, k);
+        if (c->prof_used[k] == c->prof_ev[k].size()) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess) return;
+            if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }
+            c->prof_ev[k].push_back({a, b});
+        }
+        (void)hipEventRecord(c->prof_ev[k][c->prof_used[k]].first, c->stream);
+    } else {
+        (void)hipEventRecord(c->prof_ev[k][c->prof_used[k]].second, c->stream);
+        ++c->prof_used[k];
+    }
+}
+
+void prof_release(fvp_ctx *c) {
+    for (int k = 0; k < 3; ++k) {
+        for (auto &e : c->prof_ev[k]) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+        c->prof_ev[k].clear();
+        c->prof_used[k] = 0;
+    }
 }
 
 }  // namespace fvp
@@ -328,5 +372,23 @@ extern "C" int fvp_capacity(fvp_ctx *c, double *ahh) {
     if (!c || !ahh) { set_error("null argument"); return -1; }
     FVP_HIP(hipSetDevice(c->device));
     FVP_HIP(hipMemcpyAsync(ahh, c->cap.p, sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}
+
+extern "C" int fvp_profile(fvp_ctx *c, int enable) {
+    if (!c) { set_error("null context"); return -1; }
+    FVP_HIP(hipSetDevice(c->device));
+    FVP_HIP(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 3; ++k) { c->prof_used[k] = 0; c->prof_ms[k] = 0.; c->prof_n[k] = 0; }
+    c->prof = enable ? 1 : 0;
+    return 0;
+}
+
+extern "C" int fvp_profile_get(fvp_ctx *c, int kind, double *total_ms, long *count) {
+    if (!c || kind < 0 || kind > 2) { set_error("invalid argument"); return -1; }
+    FVP_HIP(hipSetDevice(c->device));
+    prof_fold(c, kind);
+    if (total_ms) *total_ms = c->prof_ms[kind];
+    if (count) *count = c->prof_n[kind];
     return 0;
 }

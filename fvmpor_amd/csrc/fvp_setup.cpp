@@ -341,6 +341,7 @@ extern "C" int fvp_destroy(fvp_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
     jac_free(c);
+    prof_release(c);
     c->conn.release(); c->ezone.release(); c->emask.release(); c->eorient.release(); c->kmul.release();
     c->xyz.release(); c->invV.release(); c->zmask.release(); c->zw.release(); c->kr.release();
     c->cap.release(); c->acc.release(); c->bnode.release(); c->bval.release(); c->dnode.release();

@@ -69,6 +69,8 @@ SIGNATURES = {
     "fvp_integrator_state": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p, ctypes.POINTER(ctypes.c_void_p),
                                             ctypes.POINTER(ctypes.c_void_p)]),
     "fvp_integrator_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "fvp_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "fvp_profile_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_dbl_p, ctypes.POINTER(ctypes.c_long)]),
 }
 
 PRECON_NONE, PRECON_JACOBI, PRECON_ILU0 = 0, 2, 5
@@ -304,6 +306,19 @@ def _ctx_newton_gmres(self, t, psi, psip, F0, cj, b, x, m=30, max_restarts=10, t
     return rc, st.as_dict()
 
 
+def _ctx_profile(self, enable=True):
+    _check(lib().fvp_profile(self.h, int(bool(enable))), "fvp_profile")
+
+
+def _ctx_profile_get(self, kind):
+    ms = ctypes.c_double()
+    n = ctypes.c_long()
+    _check(lib().fvp_profile_get(self.h, int(kind), ctypes.byref(ms), ctypes.byref(n)), "fvp_profile_get")
+    return ms.value, n.value
+
+
+Context.profile = _ctx_profile
+Context.profile_get = _ctx_profile_get
 Context.precon_setup = _ctx_precon_setup
 Context.precon_solve = _ctx_precon_solve
 Context.jacobian = _ctx_jacobian

@@ -191,7 +191,9 @@ typedef struct {
 } fvp_integrator_params;
 
 typedef struct {
-    long steps, error_fails, newton_fails, residuals, newton_iters, gmres_iters, precon_setups;
+    long steps, error_fails, newton_fails;
+    long residuals;       /* residual evaluations on the context since create (incl. J v, FD Jacobian) */
+    long newton_iters, gmres_iters, precon_setups;
     double t, h_last;
     int order_last;
 } fvp_integrator_stats;
@@ -208,6 +210,14 @@ int fvp_integrator_advance(fvp_integrator *I, double t_out, double *psi_out);
 /* current state (device pointers owned by the integrator) */
 int fvp_integrator_state(fvp_integrator *I, double *t, const double **psi, const double **psip);
 int fvp_integrator_get_stats(const fvp_integrator *I, fvp_integrator_stats *st);
+
+/* ---------------------------------------------------------------------------
+ * Live kernel timing with HIP events on the context's stream (the reference's
+ * util::TimerCuda, timer_cuda.h:11-55).  While enabled every residual
+ * evaluation records its element-flux kernel and the whole residual; kind 0 =
+ * element kernel, 1 = whole residual, 2 = preconditioner solve. */
+int fvp_profile(fvp_ctx *ctx, int enable);
+int fvp_profile_get(fvp_ctx *ctx, int kind, double *total_ms, long *count);
 
 #ifdef __cplusplus
 }
