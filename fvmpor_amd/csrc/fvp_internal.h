@@ -15,6 +15,7 @@ void set_error(const std::string &msg);
 
 constexpr int kMaxZones = 16;
 constexpr int kMaxTags = 64;
+constexpr int kTileNodes = 1024;  // nodes owned by one workgroup of the tiled residual kernel
 
 // Zone parameters passed by value to kernels: K_xx, K_yy, K_zz, phi, alpha, n, m, S_r, S_s.
 struct ZoneTable {
@@ -70,6 +71,10 @@ struct fvp_ctx {
     int ndir = 0;
     fvp::DevBuf<int> dnode;
     fvp::DevBuf<double> dval;
+    // node tiles of the residual kernel (see fvp_setup.cpp)
+    int ntiles = 0;
+    long tile_elems = 0;            // element visits per residual (primary + halo)
+    fvp::DevBuf<int> t_eptr, t_hptr, t_hlist, t_bptr, t_dptr;
     // time-step upwinding
     int averaging = 1;             // 1: all edge weights 0.5
     fvp::DevBuf<int> eedge;        // T x ned global edge of each local edge

@@ -79,23 +79,23 @@ __constant__ int kTetB[6] = {1, 2, 0, 3, 3, 3};
 __constant__ int kTriA[3] = {0, 1, 2};
 __constant__ int kTriB[3] = {1, 2, 0};
 
-// Per-element face fluxes.  MODE 0: upwind k_r from emask; MODE 1: averaged
-// k_r (0.5 front + 0.5 back) and the averaged face fluxes are also written per
+// Face fluxes of one element, summed per element node (f[k] = net inflow into
+// node k's control volume, rho0 k_r q.S over the element's CV faces).
+// MODE 0: upwind k_r from the element's mask bits; MODE 1: averaged k_r
+// (0.5 front + 0.5 back) and the averaged face fluxes are also written per
 // (element, local edge) in back->front orientation for the edge-flux sign test.
 template <int DIM, int MODE>
-__global__ void __launch_bounds__(256) elem_kernel(int T, int N, ZoneTable zt, double rho0,
-                                                   const int4 *__restrict__ conn, const uint8_t *__restrict__ ezone,
-                                                   const double *__restrict__ kmul, const uint8_t *__restrict__ emask,
-                                                   const uint8_t *__restrict__ eorient,
-                                                   const double *__restrict__ xyz, const double *__restrict__ psi,
-                                                   const double *__restrict__ kr, double *__restrict__ acc,
-                                                   double *__restrict__ qface, double *__restrict__ aface) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= T) return;
+__device__ __forceinline__ void element_fluxes(int e, int N, const ZoneTable &zt, double rho0,
+                                               const int4 *__restrict__ conn, const uint8_t *__restrict__ ezone,
+                                               const double *__restrict__ kmul, const uint8_t *__restrict__ emask,
+                                               const uint8_t *__restrict__ eorient,
+                                               const double *__restrict__ xyz, const double *__restrict__ psi,
+                                               const double *__restrict__ kr, double *__restrict__ qface,
+                                               double *__restrict__ aface, int n[4], double f[4]) {
     constexpr int NPE = DIM + 1;
     constexpr int NED = DIM == 2 ? 3 : 6;
     int4 c4 = conn[e];
-    int n[4] = {c4.x, c4.y, c4.z, c4.w};
+    n[0] = c4.x; n[1] = c4.y; n[2] = c4.z; n[3] = c4.w;
     double px[NPE], py[NPE], pz[NPE], h[NPE];
 #pragma unroll
     for (int k = 0; k < NPE; ++k) {
@@ -143,9 +143,8 @@ __global__ void __launch_bounds__(256) elem_kernel(int T, int N, ZoneTable zt, d
     for (int k = 0; k < NPE; ++k) g[k] = G[k][0] * qx + G[k][1] * qy + G[k][2] * qz;
     // |e|/(d+1) * (grad N_b - grad N_a) . q  =  sign(det) / (d! (d+1)) * (g_b - g_a)
     const double sc = (det > 0 ? 1.0 : -1.0) / (DIM == 3 ? 24.0 : 6.0);
-    double f[NPE];
 #pragma unroll
-    for (int k = 0; k < NPE; ++k) f[k] = 0.;
+    for (int k = 0; k < 4; ++k) f[k] = 0.;
     const unsigned mask = MODE == 0 ? emask[e] : 0u;
     const unsigned orient = MODE == 1 ? eorient[e] : 0u;
 #pragma unroll
@@ -172,8 +171,69 @@ __global__ void __launch_bounds__(256) elem_kernel(int T, int N, ZoneTable zt, d
         f[a] -= M;
         f[b] += M;
     }
+}
+
+// Atomic scatter variant (averaged weights in begin_step).
+template <int DIM, int MODE>
+__global__ void __launch_bounds__(256) elem_kernel(int T, int N, ZoneTable zt, double rho0,
+                                                   const int4 *__restrict__ conn, const uint8_t *__restrict__ ezone,
+                                                   const double *__restrict__ kmul, const uint8_t *__restrict__ emask,
+                                                   const uint8_t *__restrict__ eorient,
+                                                   const double *__restrict__ xyz, const double *__restrict__ psi,
+                                                   const double *__restrict__ kr, double *__restrict__ acc,
+                                                   double *__restrict__ qface, double *__restrict__ aface) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= T) return;
+    int n[4];
+    double f[4];
+    element_fluxes<DIM, MODE>(e, N, zt, rho0, conn, ezone, kmul, emask, eorient, xyz, psi, kr, qface, aface, n, f);
 #pragma unroll
-    for (int k = 0; k < NPE; ++k) atomicAdd(&acc[n[k]], f[k]);
+    for (int k = 0; k < DIM + 1; ++k) atomicAdd(&acc[n[k]], f[k]);
+}
+
+// Owner-computes residual: one workgroup per node tile of kTileNodes nodes.  The
+// tile's flux sums live in LDS; every element touching the tile is evaluated
+// (elements shared by several tiles are evaluated once per tile) and only the
+// contributions to owned nodes are kept, so no global atomics are issued.  The
+// finalisation F = acc / V - psi' ahh, boundary fluxes and Dirichlet rows are
+// fused.  Tiles of one XCD are contiguous so their halo nodes share that XCD's L2.
+template <int DIM>
+__global__ void __launch_bounds__(256) tile_kernel(
+    int N, int ntiles, int per_xcd, ZoneTable zt, double rho0, const int4 *__restrict__ conn,
+    const uint8_t *__restrict__ ezone, const double *__restrict__ kmul, const uint8_t *__restrict__ emask,
+    const double *__restrict__ xyz, const double *__restrict__ psi, const double *__restrict__ kr,
+    const int *__restrict__ eptr, const int *__restrict__ hptr, const int *__restrict__ hlist,
+    const double *__restrict__ invV, const double *__restrict__ psip, const double *__restrict__ cap,
+    const int *__restrict__ bptr, const int *__restrict__ bnode, const double *__restrict__ bval,
+    const int *__restrict__ dptr, const int *__restrict__ dnode, const double *__restrict__ dval,
+    double *__restrict__ F) {
+    __shared__ double acc[kTileNodes];
+    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (tile >= ntiles) return;  // uniform for the whole workgroup
+    const int lo = tile * kTileNodes, hi = min(N, lo + kTileNodes);
+    for (int i = threadIdx.x; i < kTileNodes; i += blockDim.x) acc[i] = 0.;
+    __syncthreads();
+    const int e0 = eptr[tile], np = eptr[tile + 1] - e0;
+    const int h0 = hptr[tile], ntot = np + hptr[tile + 1] - h0;
+    for (int k = threadIdx.x; k < ntot; k += blockDim.x) {
+        const int e = k < np ? e0 + k : hlist[h0 + k - np];
+        int n[4];
+        double f[4];
+        element_fluxes<DIM, 0>(e, N, zt, rho0, conn, ezone, kmul, emask, nullptr, xyz, psi, kr, nullptr, nullptr, n,
+                               f);
+#pragma unroll
+        for (int q = 0; q < DIM + 1; ++q)
+            if (n[q] >= lo && n[q] < hi) atomicAdd(&acc[n[q] - lo], f[q]);
+    }
+    __syncthreads();
+    for (int k = bptr[tile] + threadIdx.x; k < bptr[tile + 1]; k += blockDim.x) acc[bnode[k] - lo] -= bval[k];
+    __syncthreads();
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) F[i] = acc[i - lo] * invV[i] - psip[i] * cap[i];
+    __syncthreads();
+    for (int k = dptr[tile] + threadIdx.x; k < dptr[tile + 1]; k += blockDim.x) {
+        const int i = dnode[k];
+        F[i] = dval[k] - psi[i];
+    }
 }
 
 __global__ void boundary_kernel(int nb, const int *__restrict__ bnode, const double *__restrict__ bval,
@@ -240,32 +300,36 @@ int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, d
     const int N = c->N, T = c->T;
     ++c->nres;
     prof_mark(c, 1, 0);
-    FVP_HIP(hipMemsetAsync(c->acc.p, 0, sizeof(double) * N, s));
     vg_kernel<<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p, c->cap.p);
-    prof_mark(c, 0, 0);
-    if (c->dim == 3) {
-        if (averaging)
+    if (!averaging) {
+        // tiled owner-computes path: element fluxes, boundary, finalize and Dirichlet in one launch
+        const int per = (c->ntiles + 7) / 8;
+        prof_mark(c, 0, 0);
+        if (c->dim == 3)
+            tile_kernel<3><<<per * 8, 256, 0, s>>>(N, c->ntiles, per, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
+                                                   c->emask.p, c->xyz.p, psi, c->kr.p, c->t_eptr.p, c->t_hptr.p,
+                                                   c->t_hlist.p, c->invV.p, psip, c->cap.p, c->t_bptr.p, c->bnode.p,
+                                                   c->bval.p, c->t_dptr.p, c->dnode.p, c->dval.p, F);
+        else
+            tile_kernel<2><<<per * 8, 256, 0, s>>>(N, c->ntiles, per, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
+                                                   c->emask.p, c->xyz.p, psi, c->kr.p, c->t_eptr.p, c->t_hptr.p,
+                                                   c->t_hlist.p, c->invV.p, psip, c->cap.p, c->t_bptr.p, c->bnode.p,
+                                                   c->bval.p, c->t_dptr.p, c->dnode.p, c->dval.p, F);
+        prof_mark(c, 0, 1);
+    } else {
+        FVP_HIP(hipMemsetAsync(c->acc.p, 0, sizeof(double) * N, s));
+        if (c->dim == 3)
             elem_kernel<3, 1><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
                                                            c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p,
                                                            c->acc.p, c->qface.p, c->aface.p);
         else
-            elem_kernel<3, 0><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
-                                                           c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p,
-                                                           c->acc.p, c->qface.p, c->aface.p);
-    } else {
-        if (averaging)
             elem_kernel<2, 1><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
                                                            c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p,
                                                            c->acc.p, c->qface.p, c->aface.p);
-        else
-            elem_kernel<2, 0><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
-                                                           c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p,
-                                                           c->acc.p, c->qface.p, c->aface.p);
+        if (c->nb) boundary_kernel<<<blocks(c->nb, 256), 256, 0, s>>>(c->nb, c->bnode.p, c->bval.p, c->acc.p);
+        finalize_kernel<<<blocks(N, 256), 256, 0, s>>>(N, c->acc.p, c->invV.p, psip, c->cap.p, F);
+        if (c->ndir) dirichlet_kernel<<<blocks(c->ndir, 256), 256, 0, s>>>(c->ndir, c->dnode.p, c->dval.p, psi, F);
     }
-    prof_mark(c, 0, 1);
-    if (c->nb) boundary_kernel<<<blocks(c->nb, 256), 256, 0, s>>>(c->nb, c->bnode.p, c->bval.p, c->acc.p);
-    finalize_kernel<<<blocks(N, 256), 256, 0, s>>>(N, c->acc.p, c->invV.p, psip, c->cap.p, F);
-    if (c->ndir) dirichlet_kernel<<<blocks(c->ndir, 256), 256, 0, s>>>(c->ndir, c->dnode.p, c->dval.p, psi, F);
     prof_mark(c, 1, 1);
     FVP_HIP(hipGetLastError());
     return 0;

@@ -144,15 +144,30 @@ extern "C" int fvp_create(const fvp_problem *p, int device, fvp_ctx **out) {
     c->bc_type.assign(p->bc_type, p->bc_type + c->ntags);
     c->bc.assign(p->bc, p->bc + (size_t)c->ntags * FVP_BC_FIELDS);
 
-    // ---- element arrays
+    // ---- element order: sorted by lowest node (stable), so the elements a node
+    // tile owns are contiguous; every per-element table below uses this order
+    std::vector<int> eperm(T), einv(T);  // new -> old, old -> new
+    {
+        std::vector<int> emin(T);
+        for (int e = 0; e < T; ++e) {
+            const int *q = p->conn + (size_t)e * npe;
+            int m = q[0];
+            for (int k = 1; k < npe; ++k) m = std::min(m, q[k]);
+            emin[e] = m;
+        }
+        std::iota(eperm.begin(), eperm.end(), 0);
+        std::stable_sort(eperm.begin(), eperm.end(), [&](int a, int b) { return emin[a] < emin[b]; });
+        for (int e = 0; e < T; ++e) einv[eperm[e]] = e;
+    }
     std::vector<int4> conn4(T);
     std::vector<uint8_t> ez(T);
     std::vector<double> km(T, 1.0);
     for (int e = 0; e < T; ++e) {
-        const int *q = p->conn + (size_t)e * npe;
+        const int o = eperm[e];
+        const int *q = p->conn + (size_t)o * npe;
         conn4[e] = make_int4(q[0], q[1], q[2], dim == 3 ? q[3] : -1);
-        ez[e] = (uint8_t)p->elem_zone[e];
-        if (p->elem_kmul) km[e] = p->elem_kmul[e];
+        ez[e] = (uint8_t)p->elem_zone[o];
+        if (p->elem_kmul) km[e] = p->elem_kmul[o];
     }
     if (c->conn.upload(conn4.data(), T) || c->ezone.upload(ez.data(), T) || c->kmul.upload(km.data(), T))
         return fail();
@@ -238,8 +253,21 @@ extern "C" int fvp_create(const fvp_problem *p, int device, fvp_ctx **out) {
         }
         slot_ptr.push_back((int)ns);
         c->NE = ne + 1;
-        if (c->eedge.upload(eedge.data(), ns) || c->slot.upload(slot.data(), ns) ||
-            c->slot_ptr.upload(slot_ptr.data(), slot_ptr.size()) || c->eorient.upload(orient.data(), T))
+        // move the per-element tables to the sorted element order; slots keep their
+        // (edge, original element) order, the order the reference sums CV faces in
+        std::vector<int> eedge_n(ns), slot_n(ns);
+        std::vector<uint8_t> orient_n(T);
+        for (int e = 0; e < T; ++e) {
+            const int o = eperm[e];
+            orient_n[e] = orient[o];
+            for (int j = 0; j < ned; ++j) eedge_n[(size_t)e * ned + j] = eedge[(size_t)o * ned + j];
+        }
+        for (size_t r = 0; r < ns; ++r) {
+            int o = slot[r] / ned, j = slot[r] % ned;
+            slot_n[r] = einv[o] * ned + j;
+        }
+        if (c->eedge.upload(eedge_n.data(), ns) || c->slot.upload(slot_n.data(), ns) ||
+            c->slot_ptr.upload(slot_ptr.data(), slot_ptr.size()) || c->eorient.upload(orient_n.data(), T))
             return fail();
     }
     if (c->emask.alloc(T) || c->qface.alloc((size_t)T * ned) || c->aface.alloc((size_t)T * ned) ||
@@ -305,6 +333,23 @@ extern "C" int fvp_create(const fvp_problem *p, int device, fvp_ctx **out) {
                     c->bnrm_h[3 * s + 2] += Ai * Ai * nzv;
                 }
             }
+        // boundary slots in ascending node order (per-tile ranges below)
+        std::vector<int> ord(c->nb);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::sort(ord.begin(), ord.end(), [&](int a, int b) { return c->bnode_h[a] < c->bnode_h[b]; });
+        std::vector<int> bn(c->nb);
+        std::vector<double> ba((size_t)c->nb * c->ntags), bv((size_t)c->nb * c->ntags * 3);
+        for (int k = 0; k < c->nb; ++k) {
+            bn[k] = c->bnode_h[ord[k]];
+            for (int tg = 0; tg < c->ntags; ++tg) {
+                ba[(size_t)k * c->ntags + tg] = c->barea_h[(size_t)ord[k] * c->ntags + tg];
+                for (int d = 0; d < 3; ++d)
+                    bv[((size_t)k * c->ntags + tg) * 3 + d] = c->bnrm_h[((size_t)ord[k] * c->ntags + tg) * 3 + d];
+            }
+        }
+        c->bnode_h.swap(bn);
+        c->barea_h.swap(ba);
+        c->bnrm_h.swap(bv);
         if (c->bnode.upload(c->bnode_h.data(), c->nb)) return fail();
         if (upload_bc_values(c)) return fail();
     }
@@ -330,6 +375,54 @@ extern "C" int fvp_create(const fvp_problem *p, int device, fvp_ctx **out) {
         c->ndir = (int)dn.size();
         if (c->dnode.upload(dn.data(), dn.size()) || c->dval.upload(dv.data(), dv.size())) return fail();
     }
+    // ---- node tiles: tile t owns nodes [t*TILE, (t+1)*TILE); it processes its
+    // primary elements (lowest node inside, a contiguous range) and the halo
+    // elements that reach into it from earlier tiles
+    {
+        const int TILE = kTileNodes;
+        const int nt = (N + TILE - 1) / TILE;
+        c->ntiles = nt;
+        std::vector<int> eptr(nt + 1, 0), hcnt(nt + 1, 0);
+        std::vector<int> first_tile(T);
+        for (int e = 0; e < T; ++e) {
+            const int4 q = conn4[e];
+            int n[4] = {q.x, q.y, q.z, q.w};
+            int m = n[0];
+            for (int k = 1; k < npe; ++k) m = std::min(m, n[k]);
+            first_tile[e] = m / TILE;
+            eptr[m / TILE + 1]++;
+        }
+        for (int t = 0; t < nt; ++t) eptr[t + 1] += eptr[t];
+        std::vector<std::vector<int>> halo(nt);
+        for (int e = 0; e < T; ++e) {
+            const int4 q = conn4[e];
+            int n[4] = {q.x, q.y, q.z, q.w};
+            int tl[4], cnt = 0;
+            for (int k = 0; k < npe; ++k) {
+                int t = n[k] / TILE;
+                if (t == first_tile[e]) continue;
+                bool seen = false;
+                for (int r = 0; r < cnt; ++r) seen |= tl[r] == t;
+                if (!seen) tl[cnt++] = t;
+            }
+            for (int r = 0; r < cnt; ++r) halo[tl[r]].push_back(e);
+        }
+        std::vector<int> hptr(nt + 1, 0), hlist;
+        for (int t = 0; t < nt; ++t) hptr[t + 1] = hptr[t] + (int)halo[t].size();
+        hlist.reserve(hptr[nt]);
+        for (int t = 0; t < nt; ++t) hlist.insert(hlist.end(), halo[t].begin(), halo[t].end());
+        std::vector<int> bptr(nt + 1, 0), dptr(nt + 1, 0);
+        for (int k = 0; k < c->nb; ++k) bptr[c->bnode_h[k] / TILE + 1]++;
+        std::vector<int> dn_h(c->ndir);
+        if (c->ndir) FVP_HIP(hipMemcpy(dn_h.data(), c->dnode.p, sizeof(int) * c->ndir, hipMemcpyDeviceToHost));
+        for (int k = 0; k < c->ndir; ++k) dptr[dn_h[k] / TILE + 1]++;
+        for (int t = 0; t < nt; ++t) { bptr[t + 1] += bptr[t]; dptr[t + 1] += dptr[t]; }
+        c->tile_elems = (long)T + hptr[nt];
+        if (c->t_eptr.upload(eptr.data(), nt + 1) || c->t_hptr.upload(hptr.data(), nt + 1) ||
+            c->t_hlist.upload(hlist.data(), hlist.size()) || c->t_bptr.upload(bptr.data(), nt + 1) ||
+            c->t_dptr.upload(dptr.data(), nt + 1))
+            return fail();
+    }
     if (c->tmp1.alloc(N) || c->tmp2.alloc(N) || c->tmp3.alloc(N) || c->red.alloc(4096)) return fail();
     c->red_h.resize(4096);
     *out = c;
@@ -345,7 +438,8 @@ extern "C" int fvp_destroy(fvp_ctx *c) {
     c->conn.release(); c->ezone.release(); c->emask.release(); c->eorient.release(); c->kmul.release();
     c->xyz.release(); c->invV.release(); c->zmask.release(); c->zw.release(); c->kr.release();
     c->cap.release(); c->acc.release(); c->bnode.release(); c->bval.release(); c->dnode.release();
-    c->dval.release(); c->eedge.release(); c->slot_ptr.release(); c->slot.release(); c->qface.release();
+    c->dval.release(); c->t_eptr.release(); c->t_hptr.release(); c->t_hlist.release(); c->t_bptr.release();
+    c->t_dptr.release(); c->eedge.release(); c->slot_ptr.release(); c->slot.release(); c->qface.release();
     c->aface.release(); c->eflux.release(); c->wf.release(); c->tmp1.release(); c->tmp2.release();
     c->tmp3.release(); c->red.release();
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);

@@ -244,3 +244,37 @@ def gaussian_field(shape, spacing, corr, sigma, seed):
     field = np.fft.ifftn(np.sqrt(lam) * noise) * np.sqrt(np.prod(ext))
     sl = tuple(slice(0, s) for s in shape)
     return field.real[sl]
+
+
+def renumber_nodes(mesh: Mesh, perm: np.ndarray) -> Mesh:
+    """Mesh with node i of the result = node perm[i] of the input."""
+    perm = np.asarray(perm, dtype=np.int64)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(perm))
+    counts = np.diff(mesh.node_tag_ptr)[perm]
+    ptr = np.zeros(len(perm) + 1, dtype=np.int32)
+    ptr[1:] = np.cumsum(counts)
+    tags = np.zeros(int(ptr[-1]), dtype=np.int32)
+    for i, o in enumerate(perm.tolist()):
+        if counts[i]:
+            tags[ptr[i]:ptr[i + 1]] = mesh.node_tags[mesh.node_tag_ptr[o]:mesh.node_tag_ptr[o + 1]]
+    conn = inv[mesh.conn].astype(np.int32)
+    return Mesh(mesh.dim, mesh.xyz[perm].copy(), conn, mesh.face_tags.copy(), ptr, tags, mesh.zone.copy())
+
+
+def tile_order(xyz: np.ndarray, dim: int, tile_nodes: int = 1024) -> np.ndarray:
+    """Locality ordering: nodes grouped by cells of a coarse grid holding about
+    ``tile_nodes`` nodes each (cells in lattice order, nodes keep their relative
+    order inside a cell).  Consecutive blocks of ``tile_nodes`` node ids are then
+    spatially compact, which is what the residual's node tiles and the
+    block-Jacobi preconditioner blocks want."""
+    lo = xyz[:, :dim].min(axis=0)
+    ext = np.maximum(xyz[:, :dim].max(axis=0) - lo, 1e-300)
+    h = (np.prod(ext) / len(xyz)) ** (1.0 / dim)
+    side = h * tile_nodes ** (1.0 / dim)
+    cells = np.maximum(1, np.rint(ext / side)).astype(np.int64)
+    idx = np.minimum((xyz[:, :dim] - lo) / ext * cells, cells - 1).astype(np.int64)
+    key = idx[:, 0]
+    for d in range(1, dim):
+        key = key * cells[d] + idx[:, d]
+    return np.argsort(key, kind="stable")

@@ -16,7 +16,7 @@ import dataclasses
 
 import numpy as np
 
-from .meshgen import Mesh, box_mesh_2d, box_mesh_3d, gaussian_field
+from .meshgen import Mesh, box_mesh_2d, box_mesh_3d, gaussian_field, renumber_nodes, tile_order
 
 ZONE_FIELDS = ("Kxx", "Kyy", "Kzz", "phi", "alpha", "n", "Sr", "Ss")
 BC_FIELDS = ("value", "b", "eta", "qx", "qy", "qz")
@@ -82,13 +82,16 @@ def slice2d_problem(nx=101, ny=41, seed=1) -> Problem:
 
 
 def box3d_problem(nx=160, ny=160, nz=40, seed_mesh=2, seed_field=3, seed_layers=4,
-                  sigma_lnK=1.0, mean_lnK=float(np.log(0.5)), rain=0.02, mesh=None) -> Problem:
+                  sigma_lnK=1.0, mean_lnK=float(np.log(0.5)), rain=0.02, mesh=None, tiled=True) -> Problem:
     """3D box 100 x 100 x 20 m, log-normal Ks (exponential covariance, 10 m / 10 m /
     2 m correlation lengths), four van Genuchten classes in random horizontal
-    layers, water table at z = 5 m, rain on the whole top face, to t = 1 day."""
+    layers, water table at z = 5 m, rain on the whole top face, to t = 1 day.
+    ``tiled`` renumbers the generated nodes with ``tile_order``."""
     lx, ly, lz = 100.0, 100.0, 20.0
     if mesh is None:
         mesh = box_mesh_3d(nx, ny, nz, lx, ly, lz, seed=seed_mesh)
+        if tiled:  # node ids grouped into compact ~1024-node tiles (locality of the residual / block precon)
+            mesh = renumber_nodes(mesh, tile_order(mesh.xyz, 3))
     p = mesh.xyz[mesh.conn]
     cen = p.mean(axis=1)
     gshape = (nx, ny, nz)
