@@ -41,6 +41,16 @@ struct fvp_csr {
     std::vector<int> lptr_h, uptr_h;
     DevBuf<int> lrows, urows;
     DevBuf<int> flag;  // device error flag (zero pivot)
+    // block-Jacobi ILU(0): per-block local CSR (columns relative to the block start)
+    bool bj = false;
+    int nblk = 0, b_maxnz = 0;
+    size_t b_lds = 0;
+    DevBuf<int> b_ia, b_src, b_diag;        // row pointers into the block arrays, source entry, diagonal position
+    DevBuf<unsigned short> b_ja;            // local column
+    DevBuf<double> b_lu;
+    DevBuf<int> b_lev_l, b_lev_u;           // per block: level pointer range [b_lev[b], b_lev[b+1]) into *_lptr
+    DevBuf<int> b_lptr_l, b_lptr_u;         // level -> rows range in *_rows
+    DevBuf<unsigned short> b_rows_l, b_rows_u;  // local rows grouped by level
     // workspace
     int m_alloc = -1;
     DevBuf<double> Q, w, r, z, part, scal, hdev;
@@ -228,6 +238,100 @@ __global__ void lincomb(long n, double *__restrict__ out, const double *__restri
     double s = 0.;
     for (int c = 0; c < nc; ++c) s += Q[c * ldq + i] * y.v[c];
     out[i] = s;
+}
+
+// ---- block-Jacobi ILU(0): one 64-lane workgroup per block of FVP_BLOCK_ROWS rows
+// Factorisation: the whole block (values, local columns, row pointers, diagonal
+// offsets) is staged in LDS; one wavefront factors one row with a lane per
+// entry: for each L entry k (in column order) the multiplier l_ik = a_ik / u_kk
+// is broadcast, row k is loaded across lanes and matched against the row's
+// columns with shuffles.  Rows of one level run on different wavefronts.
+__global__ void __launch_bounds__(256) bj_factor(int n, int max_nnz, const int *__restrict__ ia,
+                                                 const unsigned short *__restrict__ ja, const int *__restrict__ diag,
+                                                 const int *__restrict__ src, const double *__restrict__ a,
+                                                 const int *__restrict__ blev, const int *__restrict__ lptr,
+                                                 const unsigned short *__restrict__ rows, double *__restrict__ lu,
+                                                 int *__restrict__ flag) {
+    extern __shared__ double smem[];
+    double *val = smem;
+    unsigned short *col = (unsigned short *)(val + max_nnz);
+    unsigned short *rp = col + max_nnz;
+    unsigned short *dg = rp + (FVP_BLOCK_ROWS + 1);
+    const int b = blockIdx.x;
+    const int base = b * FVP_BLOCK_ROWS;
+    const int cnt = min(FVP_BLOCK_ROWS, n - base);
+    const int z0 = ia[base], nzb = ia[base + cnt] - z0;
+    for (int k = threadIdx.x; k < nzb; k += blockDim.x) {
+        val[k] = a[src[z0 + k]];
+        col[k] = ja[z0 + k];
+    }
+    for (int r = threadIdx.x; r < cnt; r += blockDim.x) {
+        rp[r] = (unsigned short)(ia[base + r] - z0);
+        dg[r] = (unsigned short)(diag[base + r] - z0);
+    }
+    if (threadIdx.x == 0) rp[cnt] = (unsigned short)nzb;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    for (int l = blev[b]; l < blev[b + 1]; ++l) {
+        for (int t = lptr[l] + wave; t < lptr[l + 1]; t += nwaves) {
+            const int r = rows[t];
+            const int s0 = rp[r], len = rp[r + 1] - s0, d = dg[r] - s0;
+            double my = lane < len ? val[s0 + lane] : 0.;
+            const int mycol = lane < len ? (int)col[s0 + lane] : -1;
+            for (int kk = 0; kk < d; ++kk) {
+                const int k = __shfl(mycol, kk, 64);
+                const double lik = __shfl(my, kk, 64) * val[dg[k]];
+                if (lane == kk) my = lik;
+                const int ks = rp[k], klen = rp[k + 1] - ks;
+                const int kc = lane < klen ? (int)col[ks + lane] : -2;
+                const double kv = lane < klen ? val[ks + lane] : 0.;
+                for (int m = 0; m < klen; ++m) {
+                    const int c = __shfl(kc, m, 64);
+                    const double v = __shfl(kv, m, 64);
+                    if (lane > kk && c == mycol) my -= lik * v;
+                }
+            }
+            if (lane == d) {
+                if (my == 0.) { atomicOr(flag, 1); my = 0.; }
+                else my = 1. / my;
+            }
+            if (lane < len) val[s0 + lane] = my;
+        }
+        __syncthreads();
+    }
+    for (int k = threadIdx.x; k < nzb; k += blockDim.x) lu[z0 + k] = val[k];
+}
+
+__global__ void __launch_bounds__(64) bj_solve(int n, const int *__restrict__ ia, const unsigned short *__restrict__ ja,
+                                               const int *__restrict__ diag, const double *__restrict__ lu,
+                                               const int *__restrict__ blev_l, const int *__restrict__ lptr_l,
+                                               const unsigned short *__restrict__ rows_l,
+                                               const int *__restrict__ blev_u, const int *__restrict__ lptr_u,
+                                               const unsigned short *__restrict__ rows_u,
+                                               const double *__restrict__ x, double *__restrict__ y) {
+    __shared__ double v[FVP_BLOCK_ROWS];
+    const int b = blockIdx.x;
+    const int base = b * FVP_BLOCK_ROWS;
+    const int cnt = min(FVP_BLOCK_ROWS, n - base);
+    for (int l = blev_l[b]; l < blev_l[b + 1]; ++l) {
+        for (int t = lptr_l[l] + threadIdx.x; t < lptr_l[l + 1]; t += blockDim.x) {
+            const int r = rows_l[t], i = base + r;
+            double s = x[i];
+            for (int k = ia[i]; k < diag[i]; ++k) s -= lu[k] * v[ja[k]];
+            v[r] = s;
+        }
+        __syncthreads();
+    }
+    for (int l = blev_u[b]; l < blev_u[b + 1]; ++l) {
+        for (int t = lptr_u[l] + threadIdx.x; t < lptr_u[l + 1]; t += blockDim.x) {
+            const int r = rows_u[t], i = base + r;
+            double s = v[r];
+            for (int k = diag[i] + 1; k < ia[i + 1]; ++k) s -= lu[k] * v[ja[k]];
+            v[r] = s * lu[diag[i]];
+        }
+        __syncthreads();
+    }
+    for (int r = threadIdx.x; r < cnt; r += blockDim.x) y[base + r] = v[r];
 }
 
 static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -443,14 +547,109 @@ static int build_levels(fvp_csr *A, const std::vector<int> &diag) {
     return 0;
 }
 
+// block-local CSR and the per-block level sets of L and U
+static int build_bj(fvp_csr *A) {
+    const int n = A->n, BS = FVP_BLOCK_ROWS;
+    const int nb = (n + BS - 1) / BS;
+    std::vector<int> bia(n + 1, 0), bsrc, bdiag(n);
+    std::vector<unsigned short> bja;
+    bsrc.reserve(A->nnz);
+    bja.reserve(A->nnz);
+    for (int i = 0; i < n; ++i) {
+        const int base = (i / BS) * BS, top = std::min(n, base + BS);
+        bdiag[i] = -1;
+        for (int k = A->ia_h[i]; k < A->ia_h[i + 1]; ++k) {
+            int j = A->ja_h[k];
+            if (j < base || j >= top) continue;
+            if (j == i) bdiag[i] = (int)bsrc.size();
+            bsrc.push_back(k);
+            bja.push_back((unsigned short)(j - base));
+        }
+        if (bdiag[i] < 0) { set_error("zero diagonal on row " + std::to_string(i)); return FVP_ERR_PIVOT; }
+        bia[i + 1] = (int)bsrc.size();
+    }
+    std::vector<int> blev_l(nb + 1, 0), blev_u(nb + 1, 0), lptr_l(1, 0), lptr_u(1, 0);
+    std::vector<unsigned short> rows_l, rows_u;
+    std::vector<int> lev(BS);
+    for (int b = 0; b < nb; ++b) {
+        const int base = b * BS, cnt = std::min(BS, n - base);
+        auto group = [&](bool lower, std::vector<int> &lptr, std::vector<unsigned short> &rows) {
+            int nl = 0;
+            if (lower) {
+                for (int r = 0; r < cnt; ++r) {
+                    int i = base + r, l = 0;
+                    for (int k = bia[i]; k < bdiag[i]; ++k) l = std::max(l, lev[bja[k]] + 1);
+                    lev[r] = l;
+                    nl = std::max(nl, l + 1);
+                }
+            } else {
+                for (int r = cnt - 1; r >= 0; --r) {
+                    int i = base + r, l = 0;
+                    for (int k = bdiag[i] + 1; k < bia[i + 1]; ++k) l = std::max(l, lev[bja[k]] + 1);
+                    lev[r] = l;
+                    nl = std::max(nl, l + 1);
+                }
+            }
+            std::vector<int> c(nl + 1, 0);
+            for (int r = 0; r < cnt; ++r) c[lev[r] + 1]++;
+            for (int l = 0; l < nl; ++l) c[l + 1] += c[l];
+            const int off = (int)rows.size();
+            rows.resize(off + cnt);
+            std::vector<int> fill(c.begin(), c.end() - 1);
+            for (int r = 0; r < cnt; ++r) rows[off + fill[lev[r]]++] = (unsigned short)r;
+            for (int l = 0; l < nl; ++l) lptr.push_back(off + c[l + 1]);
+            return nl;
+        };
+        blev_l[b + 1] = blev_l[b] + group(true, lptr_l, rows_l);
+        blev_u[b + 1] = blev_u[b] + group(false, lptr_u, rows_u);
+    }
+    if (A->b_ia.upload(bia.data(), n + 1) || A->b_src.upload(bsrc.data(), bsrc.size()) ||
+        A->b_diag.upload(bdiag.data(), n) || A->b_ja.upload(bja.data(), bja.size()) || A->b_lu.alloc(bsrc.size()) ||
+        A->b_lev_l.upload(blev_l.data(), nb + 1) || A->b_lev_u.upload(blev_u.data(), nb + 1) ||
+        A->b_lptr_l.upload(lptr_l.data(), lptr_l.size()) || A->b_lptr_u.upload(lptr_u.data(), lptr_u.size()) ||
+        A->b_rows_l.upload(rows_l.data(), rows_l.size()) || A->b_rows_u.upload(rows_u.data(), rows_u.size()))
+        return -1;
+    int maxnz = 0, maxrow = 0;
+    for (int b = 0; b < nb; ++b) {
+        const int base = b * BS, cnt = std::min(BS, n - base);
+        maxnz = std::max(maxnz, bia[base + cnt] - bia[base]);
+        for (int r = 0; r < cnt; ++r) maxrow = std::max(maxrow, bia[base + r + 1] - bia[base + r]);
+    }
+    if (maxrow > 64) { set_error("block-Jacobi ILU(0) supports rows of at most 64 block entries"); return -1; }
+    A->b_maxnz = maxnz;
+    A->b_lds = (size_t)maxnz * (sizeof(double) + 2) + (size_t)(2 * BS + 1) * 2 + 8;
+    int lds_max = 0;
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, A->device) != hipSuccess)
+        lds_max = 64 * 1024;
+    if (A->b_lds > (size_t)lds_max) {
+        set_error("block-Jacobi ILU(0): block needs " + std::to_string(A->b_lds) + " B of LDS, device allows " +
+                  std::to_string(lds_max));
+        return -1;
+    }
+    (void)hipFuncSetAttribute((const void *)bj_factor, hipFuncAttributeMaxDynamicSharedMemorySize, (int)A->b_lds);
+    A->nblk = nb;
+    A->bj = true;
+    return 0;
+}
+
 static int precon_build_impl(fvp_csr *A, int type) {
     hipStream_t s = A->stream;
     const int n = A->n;
     if (type == FVP_PRECON_NONE) { A->ptype = type; return 0; }
-    if (type != FVP_PRECON_JACOBI && type != FVP_PRECON_ILU0) { set_error("invalid preconditioner type"); return -1; }
+    if (type != FVP_PRECON_JACOBI && type != FVP_PRECON_ILU0 && type != FVP_PRECON_BJILU0) {
+        set_error("invalid preconditioner type");
+        return -1;
+    }
     if (!A->flag.p && A->flag.alloc(1)) return -1;
     FVP_HIP(hipMemsetAsync(A->flag.p, 0, sizeof(int), s));
-    if (type == FVP_PRECON_JACOBI) {
+    if (type == FVP_PRECON_BJILU0) {
+        if (!A->bj) {
+            int rc = build_bj(A);
+            if (rc) { A->ptype = -1; return rc; }
+        }
+        bj_factor<<<A->nblk, 256, A->b_lds, s>>>(n, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p,
+                                                  A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p, A->b_lu.p, A->flag.p);
+    } else if (type == FVP_PRECON_JACOBI) {
         if (A->jac.n != (size_t)n && A->jac.alloc(n)) return -1;
         jacobi_build<<<nblk(n, 256), 256, 0, s>>>(n, A->ia.p, A->ja.p, A->a.p, A->jac.p, A->flag.p);
     } else {
@@ -491,6 +690,10 @@ static int precon_apply_impl(fvp_csr *A, const double *x, double *y) {
         break;
     case FVP_PRECON_JACOBI:
         vmul<<<nblk(n, 256), 256, 0, s>>>(n, x, A->jac.p, y);
+        break;
+    case FVP_PRECON_BJILU0:
+        bj_solve<<<A->nblk, 64, 0, s>>>(n, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_lu.p, A->b_lev_l.p, A->b_lptr_l.p,
+                                        A->b_rows_l.p, A->b_lev_u.p, A->b_lptr_u.p, A->b_rows_u.p, x, y);
         break;
     case FVP_PRECON_ILU0: {
         const int L = 8;
@@ -553,6 +756,9 @@ extern "C" int fvp_csr_destroy(fvp_csr *A) {
     A->ia.release(); A->ja.release(); A->a.release(); A->lu.release(); A->jac.release(); A->diag.release();
     A->lrows.release(); A->urows.release(); A->flag.release(); A->Q.release(); A->w.release(); A->r.release();
     A->z.release(); A->part.release(); A->scal.release(); A->hdev.release();
+    A->b_ia.release(); A->b_src.release(); A->b_diag.release(); A->b_ja.release(); A->b_lu.release();
+    A->b_lev_l.release(); A->b_lev_u.release(); A->b_lptr_l.release(); A->b_lptr_u.release();
+    A->b_rows_l.release(); A->b_rows_u.release();
     if (A->own_stream) (void)hipStreamDestroy(A->own_stream);
     delete A;
     return 0;
@@ -607,6 +813,14 @@ extern "C" int fvp_precon_values(fvp_csr *A, double *values_host) {
         FVP_HIP(hipMemcpy(values_host, A->lu.p, sizeof(double) * A->nnz, hipMemcpyDeviceToHost));
     else if (A->ptype == FVP_PRECON_JACOBI)
         FVP_HIP(hipMemcpy(values_host, A->jac.p, sizeof(double) * A->n, hipMemcpyDeviceToHost));
+    else if (A->ptype == FVP_PRECON_BJILU0) {  // scattered back to the full pattern, 0 off the blocks
+        std::vector<double> lu(A->b_lu.n);
+        std::vector<int> src(A->b_src.n);
+        FVP_HIP(hipMemcpy(lu.data(), A->b_lu.p, sizeof(double) * lu.size(), hipMemcpyDeviceToHost));
+        FVP_HIP(hipMemcpy(src.data(), A->b_src.p, sizeof(int) * src.size(), hipMemcpyDeviceToHost));
+        std::fill(values_host, values_host + A->nnz, 0.);
+        for (size_t k = 0; k < lu.size(); ++k) values_host[src[k]] = lu[k];
+    }
     else { set_error("no factored values"); return -1; }
     return 0;
 }

@@ -104,6 +104,13 @@ int fvp_jv(fvp_ctx *ctx, double t, const double *psi, const double *psip, const 
 #define FVP_PRECON_NONE 0
 #define FVP_PRECON_JACOBI 2
 #define FVP_PRECON_ILU0 5
+/* Block-Jacobi ILU(0): ILU(0) of the diagonal blocks of consecutive
+ * FVP_BLOCK_ROWS rows (entries coupling blocks are dropped), factored and
+ * applied one workgroup per block.  The 1-GPU form of the block
+ * preconditioners decomp builds per subdomain (precon_schur, schur.c:1247-1323
+ * with ILU(0) on the B_i blocks).  No counterpart tag in precon.h. */
+#define FVP_PRECON_BJILU0 7
+#define FVP_BLOCK_ROWS 512
 #define FVP_ERR_PIVOT 1
 #define FVP_ERR_NOCONV 2
 

@@ -81,3 +81,12 @@ def test_jacobi_and_unpreconditioned_runs_agree():
     a, *_ = _gpu_run(prob, 0.02)
     b, *_ = _gpu_run(prob, 0.02, precon=PRECON_JACOBI)
     assert np.abs(a - b).max() < 1e-5 * np.abs(a).max()
+
+
+def test_block_jacobi_run_agrees():
+    from fvmpor_amd.lib import PRECON_BJILU0
+    from fvmpor_amd.problems import box3d_problem
+    prob = box3d_problem(nx=22, ny=20, nz=10, rain=0.05)
+    a, *_ = _gpu_run(prob, 0.01)
+    b, *_ = _gpu_run(prob, 0.01, precon=PRECON_BJILU0)
+    assert np.abs(a - b).max() < 1e-5 * np.abs(a).max()
