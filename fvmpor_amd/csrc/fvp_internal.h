@@ -15,7 +15,7 @@ void set_error(const std::string &msg);
 
 constexpr int kMaxZones = 16;
 constexpr int kMaxTags = 64;
-constexpr int kTileNodes = 1024;  // nodes owned by one workgroup of the tiled residual kernel
+constexpr int kTileNodes = 512;  // nodes owned by one workgroup of the tiled residual kernel
 
 // Zone parameters passed by value to kernels: K_xx, K_yy, K_zz, phi, alpha, n, m, S_r, S_s.
 struct ZoneTable {
@@ -75,6 +75,10 @@ struct fvp_ctx {
     int ntiles = 0;
     long tile_elems = 0;            // element visits per residual (primary + halo)
     fvp::DevBuf<int> t_eptr, t_hptr, t_hlist, t_bptr, t_dptr;
+    fvp::DevBuf<int> t_wptr, t_whalo, t_vptr, t_tz;  // window halo nodes, visits, zones per tile
+    fvp::DevBuf<unsigned short> t_lconn;             // 4 window-local node indices per visit
+    int win_max = 0, ntz_max = 1;
+    size_t tile_lds = 0;
     // time-step upwinding
     int averaging = 1;             // 1: all edge weights 0.5
     fvp::DevBuf<int> eedge;        // T x ned global edge of each local edge
@@ -100,6 +104,7 @@ struct fvp_ctx {
 
 namespace fvp {
 int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, double *F, int averaging);
+int tile_kernel_attrs(fvp_ctx *c);
 int upload_bc_values(fvp_ctx *c);
 void jac_free(fvp_ctx *c);
 // record a start / stop event of kind k on the context stream when profiling

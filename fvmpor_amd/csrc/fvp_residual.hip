@@ -236,6 +236,129 @@ __global__ void __launch_bounds__(256) tile_kernel(
     }
 }
 
+// LDS-staged owner-computes residual.  A workgroup owns kTileNodes nodes; it
+// stages the coordinates, heads and relative permeabilities of its window (owned
+// nodes plus the halo nodes its elements reach) in LDS with coalesced loads for
+// the owned range and one gather per halo node, then evaluates every element
+// touching the tile from LDS using the window-local connectivity (8 B per
+// element visit), accumulates the flux sums of owned nodes in LDS (ds_add_f64),
+// and finishes F = acc / V - psi' ahh, boundary fluxes and Dirichlet rows.
+template <int DIM>
+__global__ void __launch_bounds__(256) tile_lds_kernel(
+    int N, int ntiles, int per_xcd, int wmax, int ntzmax, ZoneTable zt, double rho0, const uint8_t *__restrict__ ezone,
+    const double *__restrict__ kmul, const uint8_t *__restrict__ emask, const double *__restrict__ xyz,
+    const double *__restrict__ psi, const double *__restrict__ kr, const int *__restrict__ eptr,
+    const int *__restrict__ hptr, const int *__restrict__ hlist, const int *__restrict__ wptr,
+    const int *__restrict__ whalo, const int *__restrict__ vptr, const ushort4 *__restrict__ lconn,
+    const int *__restrict__ tzone, const double *__restrict__ invV, const double *__restrict__ psip,
+    const double *__restrict__ cap, const int *__restrict__ bptr, const int *__restrict__ bnode,
+    const double *__restrict__ bval, const int *__restrict__ dptr, const int *__restrict__ dnode,
+    const double *__restrict__ dval, double *__restrict__ F) {
+    extern __shared__ double sm[];
+    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (tile >= ntiles) return;  // uniform for the whole workgroup
+    const int lo = tile * kTileNodes, hi = min(N, lo + kTileNodes), nown = hi - lo;
+    const int w0 = wptr[tile], win = nown + wptr[tile + 1] - w0;
+    int tz[4], ntz = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        tz[q] = tzone[tile * 4 + q];
+        ntz += tz[q] >= 0;
+    }
+    double *sx = sm, *sy = sx + wmax, *sz = sy + wmax, *sp = sz + wmax, *sk = sp + wmax;
+    double *acc = sk + ntzmax * wmax;
+    for (int i = threadIdx.x; i < win; i += blockDim.x) {
+        const int g = i < nown ? lo + i : whalo[w0 + i - nown];
+        sx[i] = xyz[3 * g];
+        sy[i] = xyz[3 * g + 1];
+        sz[i] = xyz[3 * g + 2];
+        sp[i] = psi[g];
+        for (int q = 0; q < ntz; ++q) sk[q * wmax + i] = kr[(size_t)tz[q] * N + g];
+    }
+    for (int i = threadIdx.x; i < nown; i += blockDim.x) acc[i] = 0.;
+    __syncthreads();
+    constexpr int NPE = DIM + 1;
+    constexpr int NED = DIM == 2 ? 3 : 6;
+    const int e0 = eptr[tile], np = eptr[tile + 1] - e0, h0 = hptr[tile];
+    const int v0 = vptr[tile], nv = vptr[tile + 1] - v0;
+    for (int k = threadIdx.x; k < nv; k += blockDim.x) {
+        const int e = k < np ? e0 + k : hlist[h0 + k - np];
+        const ushort4 lc = lconn[v0 + k];
+        const int l[4] = {lc.x, lc.y, lc.z, lc.w};
+        const int z = ezone[e];
+        const double km = kmul[e];
+        const unsigned mask = emask[e];
+        const int slot = z == tz[0] ? 0 : z == tz[1] ? 1 : z == tz[2] ? 2 : 3;
+        const double *zp = zt.p[z];
+        double px[NPE], py[NPE], pz[NPE], h[NPE];
+#pragma unroll
+        for (int q = 0; q < NPE; ++q) {
+            px[q] = sx[l[q]]; py[q] = sy[l[q]]; pz[q] = sz[l[q]]; h[q] = sp[l[q]];
+        }
+        double G[NPE][3];
+        double det;
+        if (DIM == 3) {
+            double ux = px[1] - px[0], uy = py[1] - py[0], uz = pz[1] - pz[0];
+            double vx = px[2] - px[0], vy = py[2] - py[0], vz = pz[2] - pz[0];
+            double wx = px[3] - px[0], wy = py[3] - py[0], wz = pz[3] - pz[0];
+            G[1][0] = vy * wz - vz * wy; G[1][1] = vz * wx - vx * wz; G[1][2] = vx * wy - vy * wx;
+            G[2][0] = wy * uz - wz * uy; G[2][1] = wz * ux - wx * uz; G[2][2] = wx * uy - wy * ux;
+            G[3][0] = uy * vz - uz * vy; G[3][1] = uz * vx - ux * vz; G[3][2] = ux * vy - uy * vx;
+            det = ux * G[1][0] + uy * G[1][1] + uz * G[1][2];
+        } else {
+            double ux = px[1] - px[0], uy = py[1] - py[0];
+            double vx = px[2] - px[0], vy = py[2] - py[0];
+            G[1][0] = vy; G[1][1] = -vx; G[1][2] = 0.;
+            G[2][0] = -uy; G[2][1] = ux; G[2][2] = 0.;
+            det = ux * vy - uy * vx;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) G[0][d] = -(G[1][d] + G[2][d] + (DIM == 3 ? G[NPE - 1][d] : 0.));
+        const double inv = 1.0 / det;
+        double gx = 0., gy = 0., gz = 0.;
+#pragma unroll
+        for (int q = 1; q < NPE; ++q) {
+            double dh = h[q] - h[0];
+            gx += dh * G[q][0]; gy += dh * G[q][1]; gz += dh * G[q][2];
+        }
+        gx *= inv; gy *= inv; gz *= inv;
+        double qx = gx * -(zp[ZK_X] * km);
+        double qy, qzz;
+        if (DIM == 2) { qy = (gy + 1.) * -(zp[ZK_Y] * km); qzz = 0.; }
+        else { qy = gy * -(zp[ZK_Y] * km); qzz = (gz + 1.) * -(zp[ZK_Z] * km); }
+        double g[NPE];
+#pragma unroll
+        for (int q = 0; q < NPE; ++q) g[q] = G[q][0] * qx + G[q][1] * qy + G[q][2] * qzz;
+        const double sc = (det > 0 ? 1.0 : -1.0) / (DIM == 3 ? 24.0 : 6.0);
+        double f[NPE];
+#pragma unroll
+        for (int q = 0; q < NPE; ++q) f[q] = 0.;
+        const double *krs = sk + slot * wmax;
+#pragma unroll
+        for (int j = 0; j < NED; ++j) {
+            const int a = DIM == 3 ? kTetA[j] : kTriA[j];
+            const int b = DIM == 3 ? kTetB[j] : kTriB[j];
+            const double flux = sc * (g[b] - g[a]);
+            const double krf = krs[(mask >> j) & 1u ? l[a] : l[b]];
+            const double M = rho0 * (krf * flux);
+            f[a] -= M;
+            f[b] += M;
+        }
+#pragma unroll
+        for (int q = 0; q < NPE; ++q)
+            if (l[q] < nown) atomicAdd(&acc[l[q]], f[q]);
+    }
+    __syncthreads();
+    for (int k = bptr[tile] + threadIdx.x; k < bptr[tile + 1]; k += blockDim.x) acc[bnode[k] - lo] -= bval[k];
+    __syncthreads();
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) F[i] = acc[i - lo] * invV[i] - psip[i] * cap[i];
+    __syncthreads();
+    for (int k = dptr[tile] + threadIdx.x; k < dptr[tile + 1]; k += blockDim.x) {
+        const int i = dnode[k];
+        F[i] = dval[k] - psi[i];
+    }
+}
+
 __global__ void boundary_kernel(int nb, const int *__restrict__ bnode, const double *__restrict__ bval,
                                 double *__restrict__ acc) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -294,6 +417,24 @@ __global__ void mask_kernel(int T, const int *__restrict__ eedge, const uint8_t 
 
 static inline int blocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
+// Allow the tile kernel's dynamic LDS beyond the default limit; fails cleanly if
+// the device cannot provide it.
+int tile_kernel_attrs(fvp_ctx *c) {
+    int lds_max = 0;
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess)
+        lds_max = 64 * 1024;
+    if (c->tile_lds > (size_t)lds_max) {
+        set_error("residual tile window needs " + std::to_string(c->tile_lds) + " B of LDS, device allows " +
+                  std::to_string(lds_max));
+        return -1;
+    }
+    FVP_HIP(hipFuncSetAttribute((const void *)tile_lds_kernel<3>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)c->tile_lds));
+    FVP_HIP(hipFuncSetAttribute((const void *)tile_lds_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)c->tile_lds));
+    return 0;
+}
+
 int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, double *F, int averaging) {
     (void)t;
     hipStream_t s = c->stream;
@@ -306,15 +447,17 @@ int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, d
         const int per = (c->ntiles + 7) / 8;
         prof_mark(c, 0, 0);
         if (c->dim == 3)
-            tile_kernel<3><<<per * 8, 256, 0, s>>>(N, c->ntiles, per, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
-                                                   c->emask.p, c->xyz.p, psi, c->kr.p, c->t_eptr.p, c->t_hptr.p,
-                                                   c->t_hlist.p, c->invV.p, psip, c->cap.p, c->t_bptr.p, c->bnode.p,
-                                                   c->bval.p, c->t_dptr.p, c->dnode.p, c->dval.p, F);
+            tile_lds_kernel<3><<<per * 8, 256, c->tile_lds, s>>>(
+                N, c->ntiles, per, c->win_max, c->ntz_max, c->zt, c->rho0, c->ezone.p, c->kmul.p, c->emask.p, c->xyz.p, psi,
+                c->kr.p, c->t_eptr.p, c->t_hptr.p, c->t_hlist.p, c->t_wptr.p, c->t_whalo.p, c->t_vptr.p,
+                (const ushort4 *)c->t_lconn.p, c->t_tz.p, c->invV.p, psip, c->cap.p, c->t_bptr.p, c->bnode.p,
+                c->bval.p, c->t_dptr.p, c->dnode.p, c->dval.p, F);
         else
-            tile_kernel<2><<<per * 8, 256, 0, s>>>(N, c->ntiles, per, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
-                                                   c->emask.p, c->xyz.p, psi, c->kr.p, c->t_eptr.p, c->t_hptr.p,
-                                                   c->t_hlist.p, c->invV.p, psip, c->cap.p, c->t_bptr.p, c->bnode.p,
-                                                   c->bval.p, c->t_dptr.p, c->dnode.p, c->dval.p, F);
+            tile_lds_kernel<2><<<per * 8, 256, c->tile_lds, s>>>(
+                N, c->ntiles, per, c->win_max, c->ntz_max, c->zt, c->rho0, c->ezone.p, c->kmul.p, c->emask.p, c->xyz.p, psi,
+                c->kr.p, c->t_eptr.p, c->t_hptr.p, c->t_hlist.p, c->t_wptr.p, c->t_whalo.p, c->t_vptr.p,
+                (const ushort4 *)c->t_lconn.p, c->t_tz.p, c->invV.p, psip, c->cap.p, c->t_bptr.p, c->bnode.p,
+                c->bval.p, c->t_dptr.p, c->dnode.p, c->dval.p, F);
         prof_mark(c, 0, 1);
     } else {
         FVP_HIP(hipMemsetAsync(c->acc.p, 0, sizeof(double) * N, s));

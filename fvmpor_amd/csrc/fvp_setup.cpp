@@ -413,6 +413,61 @@ extern "C" int fvp_create(const fvp_problem *p, int device, fvp_ctx **out) {
         for (int t = 0; t < nt; ++t) hptr[t + 1] = hptr[t] + (int)halo[t].size();
         hlist.reserve(hptr[nt]);
         for (int t = 0; t < nt; ++t) hlist.insert(hlist.end(), halo[t].begin(), halo[t].end());
+        // window of each tile = owned nodes [lo, hi) then its halo nodes (ascending);
+        // per element visit (primary elements, then halo elements) the window-local
+        // node indices, and the zones the tile's elements use (at most 4 per tile)
+        std::vector<int> wptr(nt + 1, 0), whalo, vptr(nt + 1, 0), tz((size_t)nt * 4, -1);
+        std::vector<unsigned short> lconn;
+        lconn.reserve((size_t)4 * (T + hptr[nt]));
+        std::vector<int> mark(N, -1), loc(N, 0);
+        int wmax = 0, ntzmax = 1;
+        for (int t = 0; t < nt; ++t) {
+            const int lo = t * TILE, hi = std::min(N, lo + TILE), nown = hi - lo;
+            std::vector<int> hal;
+            unsigned zones = 0;
+            auto scan = [&](int e) {
+                const int4 q = conn4[e];
+                int n[4] = {q.x, q.y, q.z, q.w};
+                for (int k = 0; k < npe; ++k)
+                    if ((n[k] < lo || n[k] >= hi) && mark[n[k]] != t) { mark[n[k]] = t; hal.push_back(n[k]); }
+                zones |= 1u << ez[e];
+            };
+            for (int e = eptr[t]; e < eptr[t + 1]; ++e) scan(e);
+            for (int e : halo[t]) scan(e);
+            std::sort(hal.begin(), hal.end());
+            for (size_t j = 0; j < hal.size(); ++j) loc[hal[j]] = nown + (int)j;
+            const int win = nown + (int)hal.size();
+            if (win > 65535) { set_error("tile window exceeds 65535 nodes"); return fail(); }
+            wmax = std::max(wmax, win);
+            int nzt = 0;
+            for (int z = 0; z < c->nz; ++z)
+                if (zones & (1u << z)) {
+                    if (nzt == 4) { set_error("a node tile touches more than 4 zones"); return fail(); }
+                    tz[(size_t)t * 4 + nzt++] = z;
+                }
+            ntzmax = std::max(ntzmax, nzt);
+            auto emit = [&](int e) {
+                const int4 q = conn4[e];
+                int n[4] = {q.x, q.y, q.z, q.w};
+                for (int k = 0; k < 4; ++k) {
+                    int l = 0;
+                    if (k < npe) l = (n[k] >= lo && n[k] < hi) ? n[k] - lo : loc[n[k]];
+                    lconn.push_back((unsigned short)l);
+                }
+            };
+            for (int e = eptr[t]; e < eptr[t + 1]; ++e) emit(e);
+            for (int e : halo[t]) emit(e);
+            whalo.insert(whalo.end(), hal.begin(), hal.end());
+            wptr[t + 1] = (int)whalo.size();
+            vptr[t + 1] = (int)(lconn.size() / 4);
+        }
+        c->win_max = wmax;
+        c->ntz_max = ntzmax;
+        c->tile_lds = (size_t)(4 + ntzmax) * wmax * sizeof(double) + (size_t)TILE * sizeof(double);
+        if (c->t_wptr.upload(wptr.data(), nt + 1) || c->t_whalo.upload(whalo.data(), whalo.size()) ||
+            c->t_vptr.upload(vptr.data(), nt + 1) || c->t_lconn.upload(lconn.data(), lconn.size()) ||
+            c->t_tz.upload(tz.data(), tz.size()))
+            return fail();
         std::vector<int> bptr(nt + 1, 0), dptr(nt + 1, 0);
         for (int k = 0; k < c->nb; ++k) bptr[c->bnode_h[k] / TILE + 1]++;
         std::vector<int> dn_h(c->ndir);
@@ -425,6 +480,7 @@ extern "C" int fvp_create(const fvp_problem *p, int device, fvp_ctx **out) {
             c->t_dptr.upload(dptr.data(), nt + 1))
             return fail();
     }
+    if (tile_kernel_attrs(c)) return fail();
     if (c->tmp1.alloc(N) || c->tmp2.alloc(N) || c->tmp3.alloc(N) || c->red.alloc(4096)) return fail();
     c->red_h.resize(4096);
     *out = c;
@@ -440,7 +496,8 @@ extern "C" int fvp_destroy(fvp_ctx *c) {
     c->conn.release(); c->ezone.release(); c->emask.release(); c->eorient.release(); c->kmul.release();
     c->xyz.release(); c->invV.release(); c->zmask.release(); c->zw.release(); c->kr.release();
     c->cap.release(); c->acc.release(); c->bnode.release(); c->bval.release(); c->dnode.release();
-    c->dval.release(); c->t_eptr.release(); c->t_hptr.release(); c->t_hlist.release(); c->t_bptr.release();
+    c->dval.release(); c->t_wptr.release(); c->t_whalo.release(); c->t_vptr.release(); c->t_lconn.release();
+    c->t_tz.release(); c->t_eptr.release(); c->t_hptr.release(); c->t_hlist.release(); c->t_bptr.release();
     c->t_dptr.release(); c->eedge.release(); c->slot_ptr.release(); c->slot.release(); c->qface.release();
     c->aface.release(); c->eflux.release(); c->wf.release(); c->tmp1.release(); c->tmp2.release();
     c->tmp3.release(); c->red.release();
