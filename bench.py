@@ -42,9 +42,9 @@ def parse():
     ap.add_argument("--nx", type=int, default=160)
     ap.add_argument("--ny", type=int, default=160)
     ap.add_argument("--nz", type=int, default=40)
-    ap.add_argument("--rtol", type=float, default=1e-8)
-    ap.add_argument("--atol", type=float, default=1e-8)
-    ap.add_argument("--precon", default="bjilu0", choices=["bjilu0", "ilu0", "jacobi"])
+    ap.add_argument("--rtol", type=float, default=1e-6)
+    ap.add_argument("--atol", type=float, default=1e-6)
+    ap.add_argument("--precon", default="bjilu0", choices=["bjilu0", "bjilu0_f32", "ilu0", "jacobi"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--cpu-nx", type=int, default=80, help="lattice size of the CPU sample box")
     return ap.parse_args()
@@ -94,7 +94,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import torch
-    from fvmpor_amd.lib import Context, Integrator, PRECON_BJILU0, PRECON_ILU0, PRECON_JACOBI
+    from fvmpor_amd.lib import Context, Integrator, PRECON_BJILU0, PRECON_BJILU0_F32, PRECON_ILU0, PRECON_JACOBI
     from fvmpor_amd.problems import box3d_problem
 
     t0 = time.perf_counter()
@@ -106,7 +106,8 @@ def main():
     stream = torch.cuda.current_stream()
     ctx.set_stream(stream.cuda_stream)
     psi0 = torch.tensor(prob.psi0, device=dev)
-    pc = {"bjilu0": PRECON_BJILU0, "ilu0": PRECON_ILU0, "jacobi": PRECON_JACOBI}[args.precon]
+    pc = {"bjilu0": PRECON_BJILU0, "bjilu0_f32": PRECON_BJILU0_F32, "ilu0": PRECON_ILU0,
+          "jacobi": PRECON_JACOBI}[args.precon]
     integ = Integrator(ctx, psi0, rtol=args.rtol, atol=args.atol, max_order=3, precon=pc)
     ncol, _ = ctx.colours()
     torch.cuda.synchronize()

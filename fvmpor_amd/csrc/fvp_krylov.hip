@@ -48,6 +48,7 @@ struct fvp_csr {
     DevBuf<int> b_ia, b_src, b_diag;        // row pointers into the block arrays, source entry, diagonal position
     DevBuf<unsigned short> b_ja;            // local column
     DevBuf<double> b_lu;
+    DevBuf<float> b_lu32;                   // FP32 factor (FVP_PRECON_BJILU0_F32)
     DevBuf<int> b_lev_l, b_lev_u;           // per block: level pointer range [b_lev[b], b_lev[b+1]) into *_lptr
     DevBuf<int> b_lptr_l, b_lptr_u;         // level -> rows range in *_rows
     DevBuf<unsigned short> b_rows_l, b_rows_u;  // local rows grouped by level
@@ -246,14 +247,15 @@ __global__ void lincomb(long n, double *__restrict__ out, const double *__restri
 // entry: for each L entry k (in column order) the multiplier l_ik = a_ik / u_kk
 // is broadcast, row k is loaded across lanes and matched against the row's
 // columns with shuffles.  Rows of one level run on different wavefronts.
-__global__ void __launch_bounds__(256) bj_factor(int n, int max_nnz, const int *__restrict__ ia,
+template <typename T>
+__global__ void __launch_bounds__(1024) bj_factor(int n, int max_nnz, const int *__restrict__ ia,
                                                  const unsigned short *__restrict__ ja, const int *__restrict__ diag,
                                                  const int *__restrict__ src, const double *__restrict__ a,
                                                  const int *__restrict__ blev, const int *__restrict__ lptr,
-                                                 const unsigned short *__restrict__ rows, double *__restrict__ lu,
+                                                 const unsigned short *__restrict__ rows, T *__restrict__ lu,
                                                  int *__restrict__ flag) {
     extern __shared__ double smem[];
-    double *val = smem;
+    T *val = (T *)smem;
     unsigned short *col = (unsigned short *)(val + max_nnz);
     unsigned short *rp = col + max_nnz;
     unsigned short *dg = rp + (FVP_BLOCK_ROWS + 1);
@@ -262,7 +264,7 @@ __global__ void __launch_bounds__(256) bj_factor(int n, int max_nnz, const int *
     const int cnt = min(FVP_BLOCK_ROWS, n - base);
     const int z0 = ia[base], nzb = ia[base + cnt] - z0;
     for (int k = threadIdx.x; k < nzb; k += blockDim.x) {
-        val[k] = a[src[z0 + k]];
+        val[k] = (T)a[src[z0 + k]];
         col[k] = ja[z0 + k];
     }
     for (int r = threadIdx.x; r < cnt; r += blockDim.x) {
@@ -276,24 +278,24 @@ __global__ void __launch_bounds__(256) bj_factor(int n, int max_nnz, const int *
         for (int t = lptr[l] + wave; t < lptr[l + 1]; t += nwaves) {
             const int r = rows[t];
             const int s0 = rp[r], len = rp[r + 1] - s0, d = dg[r] - s0;
-            double my = lane < len ? val[s0 + lane] : 0.;
+            T my = lane < len ? val[s0 + lane] : T(0);
             const int mycol = lane < len ? (int)col[s0 + lane] : -1;
             for (int kk = 0; kk < d; ++kk) {
                 const int k = __shfl(mycol, kk, 64);
-                const double lik = __shfl(my, kk, 64) * val[dg[k]];
+                const T lik = __shfl(my, kk, 64) * val[dg[k]];
                 if (lane == kk) my = lik;
                 const int ks = rp[k], klen = rp[k + 1] - ks;
                 const int kc = lane < klen ? (int)col[ks + lane] : -2;
-                const double kv = lane < klen ? val[ks + lane] : 0.;
+                const T kv = lane < klen ? val[ks + lane] : T(0);
                 for (int m = 0; m < klen; ++m) {
                     const int c = __shfl(kc, m, 64);
-                    const double v = __shfl(kv, m, 64);
+                    const T v = __shfl(kv, m, 64);
                     if (lane > kk && c == mycol) my -= lik * v;
                 }
             }
             if (lane == d) {
-                if (my == 0.) { atomicOr(flag, 1); my = 0.; }
-                else my = 1. / my;
+                if (my == T(0)) { atomicOr(flag, 1); my = T(0); }
+                else my = T(1) / my;
             }
             if (lane < len) val[s0 + lane] = my;
         }
@@ -302,36 +304,58 @@ __global__ void __launch_bounds__(256) bj_factor(int n, int max_nnz, const int *
     for (int k = threadIdx.x; k < nzb; k += blockDim.x) lu[z0 + k] = val[k];
 }
 
-__global__ void __launch_bounds__(64) bj_solve(int n, const int *__restrict__ ia, const unsigned short *__restrict__ ja,
-                                               const int *__restrict__ diag, const double *__restrict__ lu,
-                                               const int *__restrict__ blev_l, const int *__restrict__ lptr_l,
-                                               const unsigned short *__restrict__ rows_l,
-                                               const int *__restrict__ blev_u, const int *__restrict__ lptr_u,
-                                               const unsigned short *__restrict__ rows_u,
-                                               const double *__restrict__ x, double *__restrict__ y) {
-    __shared__ double v[FVP_BLOCK_ROWS];
+// Solve: the block's factor (values, local columns, row pointers, diagonal
+// offsets) and the right-hand side are staged in LDS with coalesced loads, then
+// the L and U level sets are walked with one thread per row.  T is the factor's
+// arithmetic type (double, or float for the FP32 preconditioner); x and y stay FP64.
+template <typename T>
+__global__ void __launch_bounds__(256) bj_solve(int n, int max_nnz, const int *__restrict__ ia,
+                                                const unsigned short *__restrict__ ja, const int *__restrict__ diag,
+                                                const T *__restrict__ lu, const int *__restrict__ blev_l,
+                                                const int *__restrict__ lptr_l, const unsigned short *__restrict__ rows_l,
+                                                const int *__restrict__ blev_u, const int *__restrict__ lptr_u,
+                                                const unsigned short *__restrict__ rows_u, const double *__restrict__ x,
+                                                double *__restrict__ y) {
+    extern __shared__ double smem[];
+    T *val = (T *)smem;
+    T *v = val + max_nnz;
+    unsigned short *col = (unsigned short *)(v + FVP_BLOCK_ROWS);
+    unsigned short *rp = col + max_nnz;
+    unsigned short *dg = rp + (FVP_BLOCK_ROWS + 1);
     const int b = blockIdx.x;
     const int base = b * FVP_BLOCK_ROWS;
     const int cnt = min(FVP_BLOCK_ROWS, n - base);
+    const int z0 = ia[base], nzb = ia[base + cnt] - z0;
+    for (int k = threadIdx.x; k < nzb; k += blockDim.x) {
+        val[k] = lu[z0 + k];
+        col[k] = ja[z0 + k];
+    }
+    for (int r = threadIdx.x; r < cnt; r += blockDim.x) {
+        rp[r] = (unsigned short)(ia[base + r] - z0);
+        dg[r] = (unsigned short)(diag[base + r] - z0);
+        v[r] = (T)x[base + r];
+    }
+    if (threadIdx.x == 0) rp[cnt] = (unsigned short)nzb;
+    __syncthreads();
     for (int l = blev_l[b]; l < blev_l[b + 1]; ++l) {
         for (int t = lptr_l[l] + threadIdx.x; t < lptr_l[l + 1]; t += blockDim.x) {
-            const int r = rows_l[t], i = base + r;
-            double s = x[i];
-            for (int k = ia[i]; k < diag[i]; ++k) s -= lu[k] * v[ja[k]];
+            const int r = rows_l[t];
+            T s = v[r];
+            for (int k = rp[r]; k < dg[r]; ++k) s -= val[k] * v[col[k]];
             v[r] = s;
         }
         __syncthreads();
     }
     for (int l = blev_u[b]; l < blev_u[b + 1]; ++l) {
         for (int t = lptr_u[l] + threadIdx.x; t < lptr_u[l + 1]; t += blockDim.x) {
-            const int r = rows_u[t], i = base + r;
-            double s = v[r];
-            for (int k = diag[i] + 1; k < ia[i + 1]; ++k) s -= lu[k] * v[ja[k]];
-            v[r] = s * lu[diag[i]];
+            const int r = rows_u[t];
+            T s = v[r];
+            for (int k = dg[r] + 1; k < rp[r + 1]; ++k) s -= val[k] * v[col[k]];
+            v[r] = s * val[dg[r]];
         }
         __syncthreads();
     }
-    for (int r = threadIdx.x; r < cnt; r += blockDim.x) y[base + r] = v[r];
+    for (int r = threadIdx.x; r < cnt; r += blockDim.x) y[base + r] = (double)v[r];
 }
 
 static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -547,6 +571,11 @@ static int build_levels(fvp_csr *A, const std::vector<int> &diag) {
     return 0;
 }
 
+// LDS bytes of one block's factor staging: values, local columns, row pointers, diagonal offsets
+static size_t bj_lds(int maxnz, size_t value_bytes) {
+    return (size_t)maxnz * (value_bytes + 2) + (size_t)(2 * FVP_BLOCK_ROWS + 1) * 2 + 8;
+}
+
 // block-local CSR and the per-block level sets of L and U
 static int build_bj(fvp_csr *A) {
     const int n = A->n, BS = FVP_BLOCK_ROWS;
@@ -617,16 +646,23 @@ static int build_bj(fvp_csr *A) {
     }
     if (maxrow > 64) { set_error("block-Jacobi ILU(0) supports rows of at most 64 block entries"); return -1; }
     A->b_maxnz = maxnz;
-    A->b_lds = (size_t)maxnz * (sizeof(double) + 2) + (size_t)(2 * BS + 1) * 2 + 8;
+    A->b_lds = bj_lds(maxnz, sizeof(double));
     int lds_max = 0;
     if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, A->device) != hipSuccess)
         lds_max = 64 * 1024;
-    if (A->b_lds > (size_t)lds_max) {
+    if (A->b_lds + sizeof(double) * BS > (size_t)lds_max) {
         set_error("block-Jacobi ILU(0): block needs " + std::to_string(A->b_lds) + " B of LDS, device allows " +
                   std::to_string(lds_max));
         return -1;
     }
-    (void)hipFuncSetAttribute((const void *)bj_factor, hipFuncAttributeMaxDynamicSharedMemorySize, (int)A->b_lds);
+    (void)hipFuncSetAttribute((const void *)bj_factor<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)A->b_lds);
+    (void)hipFuncSetAttribute((const void *)bj_solve<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(A->b_lds + sizeof(double) * BS));
+    (void)hipFuncSetAttribute((const void *)bj_factor<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)bj_lds(maxnz, sizeof(float)));
+    (void)hipFuncSetAttribute((const void *)bj_solve<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(bj_lds(maxnz, sizeof(float)) + sizeof(float) * BS));
     A->nblk = nb;
     A->bj = true;
     return 0;
@@ -636,19 +672,28 @@ static int precon_build_impl(fvp_csr *A, int type) {
     hipStream_t s = A->stream;
     const int n = A->n;
     if (type == FVP_PRECON_NONE) { A->ptype = type; return 0; }
-    if (type != FVP_PRECON_JACOBI && type != FVP_PRECON_ILU0 && type != FVP_PRECON_BJILU0) {
+    if (type != FVP_PRECON_JACOBI && type != FVP_PRECON_ILU0 && type != FVP_PRECON_BJILU0 &&
+        type != FVP_PRECON_BJILU0_F32) {
         set_error("invalid preconditioner type");
         return -1;
     }
     if (!A->flag.p && A->flag.alloc(1)) return -1;
     FVP_HIP(hipMemsetAsync(A->flag.p, 0, sizeof(int), s));
-    if (type == FVP_PRECON_BJILU0) {
+    if (type == FVP_PRECON_BJILU0 || type == FVP_PRECON_BJILU0_F32) {
         if (!A->bj) {
             int rc = build_bj(A);
             if (rc) { A->ptype = -1; return rc; }
         }
-        bj_factor<<<A->nblk, 256, A->b_lds, s>>>(n, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p,
-                                                  A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p, A->b_lu.p, A->flag.p);
+        if (type == FVP_PRECON_BJILU0)
+            bj_factor<double><<<A->nblk, 1024, A->b_lds, s>>>(n, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p,
+                                                              A->b_src.p, A->a.p, A->b_lev_l.p, A->b_lptr_l.p,
+                                                              A->b_rows_l.p, A->b_lu.p, A->flag.p);
+        else {
+            if (!A->b_lu32.p && A->b_lu32.alloc(A->b_lu.n)) return -1;
+            bj_factor<float><<<A->nblk, 1024, bj_lds(A->b_maxnz, sizeof(float)), s>>>(
+                n, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p, A->b_lev_l.p, A->b_lptr_l.p,
+                A->b_rows_l.p, A->b_lu32.p, A->flag.p);
+        }
     } else if (type == FVP_PRECON_JACOBI) {
         if (A->jac.n != (size_t)n && A->jac.alloc(n)) return -1;
         jacobi_build<<<nblk(n, 256), 256, 0, s>>>(n, A->ia.p, A->ja.p, A->a.p, A->jac.p, A->flag.p);
@@ -692,8 +737,14 @@ static int precon_apply_impl(fvp_csr *A, const double *x, double *y) {
         vmul<<<nblk(n, 256), 256, 0, s>>>(n, x, A->jac.p, y);
         break;
     case FVP_PRECON_BJILU0:
-        bj_solve<<<A->nblk, 64, 0, s>>>(n, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_lu.p, A->b_lev_l.p, A->b_lptr_l.p,
-                                        A->b_rows_l.p, A->b_lev_u.p, A->b_lptr_u.p, A->b_rows_u.p, x, y);
+        bj_solve<double><<<A->nblk, 256, A->b_lds + sizeof(double) * FVP_BLOCK_ROWS, s>>>(
+            n, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_lu.p, A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p,
+            A->b_lev_u.p, A->b_lptr_u.p, A->b_rows_u.p, x, y);
+        break;
+    case FVP_PRECON_BJILU0_F32:
+        bj_solve<float><<<A->nblk, 256, bj_lds(A->b_maxnz, sizeof(float)) + sizeof(float) * FVP_BLOCK_ROWS, s>>>(
+            n, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_lu32.p, A->b_lev_l.p, A->b_lptr_l.p,
+            A->b_rows_l.p, A->b_lev_u.p, A->b_lptr_u.p, A->b_rows_u.p, x, y);
         break;
     case FVP_PRECON_ILU0: {
         const int L = 8;
@@ -756,7 +807,7 @@ extern "C" int fvp_csr_destroy(fvp_csr *A) {
     A->ia.release(); A->ja.release(); A->a.release(); A->lu.release(); A->jac.release(); A->diag.release();
     A->lrows.release(); A->urows.release(); A->flag.release(); A->Q.release(); A->w.release(); A->r.release();
     A->z.release(); A->part.release(); A->scal.release(); A->hdev.release();
-    A->b_ia.release(); A->b_src.release(); A->b_diag.release(); A->b_ja.release(); A->b_lu.release();
+    A->b_ia.release(); A->b_src.release(); A->b_diag.release(); A->b_ja.release(); A->b_lu.release(); A->b_lu32.release();
     A->b_lev_l.release(); A->b_lev_u.release(); A->b_lptr_l.release(); A->b_lptr_u.release();
     A->b_rows_l.release(); A->b_rows_u.release();
     if (A->own_stream) (void)hipStreamDestroy(A->own_stream);
@@ -813,10 +864,17 @@ extern "C" int fvp_precon_values(fvp_csr *A, double *values_host) {
         FVP_HIP(hipMemcpy(values_host, A->lu.p, sizeof(double) * A->nnz, hipMemcpyDeviceToHost));
     else if (A->ptype == FVP_PRECON_JACOBI)
         FVP_HIP(hipMemcpy(values_host, A->jac.p, sizeof(double) * A->n, hipMemcpyDeviceToHost));
-    else if (A->ptype == FVP_PRECON_BJILU0) {  // scattered back to the full pattern, 0 off the blocks
+    else if (A->ptype == FVP_PRECON_BJILU0 || A->ptype == FVP_PRECON_BJILU0_F32) {
+        // scattered back to the full pattern, 0 off the blocks
         std::vector<double> lu(A->b_lu.n);
         std::vector<int> src(A->b_src.n);
-        FVP_HIP(hipMemcpy(lu.data(), A->b_lu.p, sizeof(double) * lu.size(), hipMemcpyDeviceToHost));
+        if (A->ptype == FVP_PRECON_BJILU0)
+            FVP_HIP(hipMemcpy(lu.data(), A->b_lu.p, sizeof(double) * lu.size(), hipMemcpyDeviceToHost));
+        else {
+            std::vector<float> lu32(lu.size());
+            FVP_HIP(hipMemcpy(lu32.data(), A->b_lu32.p, sizeof(float) * lu.size(), hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < lu.size(); ++k) lu[k] = lu32[k];
+        }
         FVP_HIP(hipMemcpy(src.data(), A->b_src.p, sizeof(int) * src.size(), hipMemcpyDeviceToHost));
         std::fill(values_host, values_host + A->nnz, 0.);
         for (size_t k = 0; k < lu.size(); ++k) values_host[src[k]] = lu[k];

@@ -73,7 +73,7 @@ SIGNATURES = {
     "fvp_profile_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_dbl_p, ctypes.POINTER(ctypes.c_long)]),
 }
 
-PRECON_NONE, PRECON_JACOBI, PRECON_ILU0, PRECON_BJILU0 = 0, 2, 5, 7
+PRECON_NONE, PRECON_JACOBI, PRECON_ILU0, PRECON_BJILU0, PRECON_BJILU0_F32 = 0, 2, 5, 7, 8
 BLOCK_ROWS = 512
 ERR_PIVOT, ERR_NOCONV = 1, 2
 

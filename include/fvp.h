@@ -110,6 +110,10 @@ int fvp_jv(fvp_ctx *ctx, double t, const double *psi, const double *psip, const 
  * preconditioners decomp builds per subdomain (precon_schur, schur.c:1247-1323
  * with ILU(0) on the B_i blocks).  No counterpart tag in precon.h. */
 #define FVP_PRECON_BJILU0 7
+/* The same block-Jacobi ILU(0) with the factorisation and both triangular
+ * solves in FP32 (the input matrix and the vectors at the boundary stay FP64).
+ * No counterpart in the reference. */
+#define FVP_PRECON_BJILU0_F32 8
 #define FVP_BLOCK_ROWS 512
 #define FVP_ERR_PIVOT 1
 #define FVP_ERR_NOCONV 2
@@ -188,7 +192,7 @@ typedef struct {
     int max_order;        /* 1..5 */
     double h0;            /* first step; 0 = from the initial residual */
     double h_max;         /* 0 = unlimited */
-    int precon_type;      /* FVP_PRECON_ILU0 / FVP_PRECON_JACOBI / FVP_PRECON_NONE */
+    int precon_type;      /* FVP_PRECON_BJILU0(_F32) / FVP_PRECON_ILU0 / FVP_PRECON_JACOBI / FVP_PRECON_NONE */
     int dim_krylov;       /* GMRES restart length (IDA maxl) */
     int max_restarts;
     double lin_factor;    /* linear tolerance = lin_factor * newton_tol (IDA eplifac, 0.05) */

@@ -90,3 +90,17 @@ def test_block_jacobi_run_agrees():
     a, *_ = _gpu_run(prob, 0.01)
     b, *_ = _gpu_run(prob, 0.01, precon=PRECON_BJILU0)
     assert np.abs(a - b).max() < 1e-5 * np.abs(a).max()
+
+
+def test_fp32_block_jacobi_run_agrees():
+    """FP32 preconditioner (factor and solves) against the FP64 one on the same
+    run: the residual, J v, GMRES and error control stay FP64, so the final
+    heads agree to the Newton / linear tolerances."""
+    from fvmpor_amd.lib import PRECON_BJILU0, PRECON_BJILU0_F32
+    from fvmpor_amd.problems import box3d_problem
+    prob = box3d_problem(nx=22, ny=20, nz=10, rain=0.05)
+    a, _, _, _, sa = _gpu_run(prob, 0.01, precon=PRECON_BJILU0)
+    b, _, _, _, sb = _gpu_run(prob, 0.01, precon=PRECON_BJILU0_F32)
+    rel = np.abs(a - b).max() / np.abs(a).max()
+    print(f"fp32 vs fp64 block-Jacobi: final head rel diff {rel:.2e}, gmres iters {sb['gmres_iters']} vs {sa['gmres_iters']}")
+    assert rel < 1e-6

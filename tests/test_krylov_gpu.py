@@ -193,3 +193,33 @@ def test_block_jacobi_ilu0_is_blockwise_ilu0(box_mid):
     rc, st = A.gmres(_dev(torch, b), xs, m=30, max_restarts=40, tol=1e-12, reorth=1)
     assert rc == 0 and st["converged"]
     assert np.abs(xs.cpu().numpy() - 1.0).max() < 1e-9
+
+
+def test_block_jacobi_ilu0_fp32(box_mid):
+    """FP32 factor and solves: within single-precision rounding of the blockwise
+    FP64 ILU(0) of the oracle, and GMRES (FP64 Krylov, right preconditioning)
+    still reaches the FP64 solution."""
+    import torch
+    from fvmpor_amd.lib import CsrMatrix, PRECON_BJILU0_F32, BLOCK_ROWS
+    ia, ja, a = _matrix(box_mid.mesh, seed=3, shift=1.2)
+    n = len(ia) - 1
+    A = CsrMatrix(ia, ja)
+    A.set_values(_dev(torch, a))
+    assert A.precon_build(PRECON_BJILU0_F32) == 0
+    bia, bja, ba, keep = _block_diagonal(ia, ja, a, BLOCK_ROWS)
+    lu_o, dp = O.ilu0(bia, bja, ba)
+    lu_g = A.precon_values(len(ja))
+    err = np.abs(lu_g[keep] - lu_o).max() / np.abs(lu_o).max()
+    assert 0 < err < 1e-5
+    assert np.all(lu_g[~keep] == 0)
+    x = np.random.default_rng(5).normal(size=n)
+    y = torch.empty(n, dtype=torch.float64, device=device())
+    A.precon_apply(_dev(torch, x), y)
+    torch.cuda.synchronize()
+    ref = O.ilu0_apply(bia, bja, lu_o, dp, x)
+    assert np.abs(y.cpu().numpy() - ref).max() < 1e-5 * np.abs(ref).max()
+    b = O.csr_gemv(ia, ja, a, np.ones(n))
+    xs = torch.zeros(n, dtype=torch.float64, device=device())
+    rc, st = A.gmres(_dev(torch, b), xs, m=30, max_restarts=40, tol=1e-12, reorth=1)
+    assert rc == 0 and st["converged"]
+    assert np.abs(xs.cpu().numpy() - 1.0).max() < 1e-9
