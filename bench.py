@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--rtol", type=float, default=1e-6)
     ap.add_argument("--atol", type=float, default=1e-6)
     ap.add_argument("--precon", default="bjilu0", choices=["bjilu0", "bjilu0_f32", "ilu0", "jacobi"])
+    ap.add_argument("--block-rows", type=int, default=0, help="block-Jacobi rows per block (0 = library default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--cpu-nx", type=int, default=80, help="lattice size of the CPU sample box")
     return ap.parse_args()
@@ -94,7 +95,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import torch
-    from fvmpor_amd.lib import Context, Integrator, PRECON_BJILU0, PRECON_BJILU0_F32, PRECON_ILU0, PRECON_JACOBI
+    from fvmpor_amd.lib import BLOCK_ROWS, Context, Integrator, PRECON_BJILU0, PRECON_BJILU0_F32, PRECON_ILU0, PRECON_JACOBI
     from fvmpor_amd.problems import box3d_problem
 
     t0 = time.perf_counter()
@@ -108,7 +109,8 @@ def main():
     psi0 = torch.tensor(prob.psi0, device=dev)
     pc = {"bjilu0": PRECON_BJILU0, "bjilu0_f32": PRECON_BJILU0_F32, "ilu0": PRECON_ILU0,
           "jacobi": PRECON_JACOBI}[args.precon]
-    integ = Integrator(ctx, psi0, rtol=args.rtol, atol=args.atol, max_order=3, precon=pc)
+    integ = Integrator(ctx, psi0, rtol=args.rtol, atol=args.atol, max_order=3, precon=pc,
+                       block_rows=args.block_rows)
     ncol, _ = ctx.colours()
     torch.cuda.synchronize()
     log(f"setup {time.perf_counter() - t0:.1f}s, {ncol} colours, N_edges={ctx.NE}")
@@ -158,7 +160,8 @@ def main():
             "workload": f"plain FP64 3D box {args.nx}x{args.ny}x{args.nz} lattice ({N} nodes, {T} tets), "
                         f"BDF(<=3) Newton-GMRES {args.precon}, rtol=atol={args.rtol:g}, timed steps "
                         f"{args.warmup + 1}..{args.warmup + args.steps} from t=0",
-            "nodes": N, "elements": T, "edges": NE,
+            "nodes": N, "elements": T, "edges": NE, "precon": args.precon,
+            "block_rows": args.block_rows or BLOCK_ROWS,
             "sim_days_timed": days, "t_start": st0["t"], "t_end": st1["t"],
             "newton_iters": st1["newton_iters"] - st0["newton_iters"],
             "gmres_iters": st1["gmres_iters"] - st0["gmres_iters"],

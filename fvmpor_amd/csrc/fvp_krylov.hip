@@ -43,7 +43,8 @@ struct fvp_csr {
     DevBuf<int> flag;  // device error flag (zero pivot)
     // block-Jacobi ILU(0): per-block local CSR (columns relative to the block start)
     bool bj = false;
-    int nblk = 0, b_maxnz = 0;
+    int bs = FVP_BLOCK_ROWS;                // rows per block
+    int nblk = 0, b_maxnz = 0, lds_max = 0;
     size_t b_lds = 0;
     DevBuf<int> b_ia, b_src, b_diag;        // row pointers into the block arrays, source entry, diagonal position
     DevBuf<unsigned short> b_ja;            // local column
@@ -241,14 +242,14 @@ __global__ void lincomb(long n, double *__restrict__ out, const double *__restri
     out[i] = s;
 }
 
-// ---- block-Jacobi ILU(0): one 64-lane workgroup per block of FVP_BLOCK_ROWS rows
+// ---- block-Jacobi ILU(0): one workgroup per block of bs consecutive rows
 // Factorisation: the whole block (values, local columns, row pointers, diagonal
 // offsets) is staged in LDS; one wavefront factors one row with a lane per
 // entry: for each L entry k (in column order) the multiplier l_ik = a_ik / u_kk
 // is broadcast, row k is loaded across lanes and matched against the row's
 // columns with shuffles.  Rows of one level run on different wavefronts.
 template <typename T>
-__global__ void __launch_bounds__(1024) bj_factor(int n, int max_nnz, const int *__restrict__ ia,
+__global__ void __launch_bounds__(1024) bj_factor(int n, int bs, int max_nnz, const int *__restrict__ ia,
                                                  const unsigned short *__restrict__ ja, const int *__restrict__ diag,
                                                  const int *__restrict__ src, const double *__restrict__ a,
                                                  const int *__restrict__ blev, const int *__restrict__ lptr,
@@ -258,10 +259,10 @@ __global__ void __launch_bounds__(1024) bj_factor(int n, int max_nnz, const int 
     T *val = (T *)smem;
     unsigned short *col = (unsigned short *)(val + max_nnz);
     unsigned short *rp = col + max_nnz;
-    unsigned short *dg = rp + (FVP_BLOCK_ROWS + 1);
+    unsigned short *dg = rp + (bs + 1);
     const int b = blockIdx.x;
-    const int base = b * FVP_BLOCK_ROWS;
-    const int cnt = min(FVP_BLOCK_ROWS, n - base);
+    const int base = b * bs;
+    const int cnt = min(bs, n - base);
     const int z0 = ia[base], nzb = ia[base + cnt] - z0;
     for (int k = threadIdx.x; k < nzb; k += blockDim.x) {
         val[k] = (T)a[src[z0 + k]];
@@ -309,7 +310,7 @@ __global__ void __launch_bounds__(1024) bj_factor(int n, int max_nnz, const int 
 // the L and U level sets are walked with one thread per row.  T is the factor's
 // arithmetic type (double, or float for the FP32 preconditioner); x and y stay FP64.
 template <typename T>
-__global__ void __launch_bounds__(256) bj_solve(int n, int max_nnz, const int *__restrict__ ia,
+__global__ void __launch_bounds__(256) bj_solve(int n, int bs, int max_nnz, const int *__restrict__ ia,
                                                 const unsigned short *__restrict__ ja, const int *__restrict__ diag,
                                                 const T *__restrict__ lu, const int *__restrict__ blev_l,
                                                 const int *__restrict__ lptr_l, const unsigned short *__restrict__ rows_l,
@@ -319,12 +320,12 @@ __global__ void __launch_bounds__(256) bj_solve(int n, int max_nnz, const int *_
     extern __shared__ double smem[];
     T *val = (T *)smem;
     T *v = val + max_nnz;
-    unsigned short *col = (unsigned short *)(v + FVP_BLOCK_ROWS);
+    unsigned short *col = (unsigned short *)(v + bs);
     unsigned short *rp = col + max_nnz;
-    unsigned short *dg = rp + (FVP_BLOCK_ROWS + 1);
+    unsigned short *dg = rp + (bs + 1);
     const int b = blockIdx.x;
-    const int base = b * FVP_BLOCK_ROWS;
-    const int cnt = min(FVP_BLOCK_ROWS, n - base);
+    const int base = b * bs;
+    const int cnt = min(bs, n - base);
     const int z0 = ia[base], nzb = ia[base + cnt] - z0;
     for (int k = threadIdx.x; k < nzb; k += blockDim.x) {
         val[k] = lu[z0 + k];
@@ -572,13 +573,13 @@ static int build_levels(fvp_csr *A, const std::vector<int> &diag) {
 }
 
 // LDS bytes of one block's factor staging: values, local columns, row pointers, diagonal offsets
-static size_t bj_lds(int maxnz, size_t value_bytes) {
-    return (size_t)maxnz * (value_bytes + 2) + (size_t)(2 * FVP_BLOCK_ROWS + 1) * 2 + 8;
+static size_t bj_lds(int maxnz, size_t value_bytes, int bs) {
+    return (size_t)maxnz * (value_bytes + 2) + (size_t)(2 * bs + 1) * 2 + 8;
 }
 
 // block-local CSR and the per-block level sets of L and U
 static int build_bj(fvp_csr *A) {
-    const int n = A->n, BS = FVP_BLOCK_ROWS;
+    const int n = A->n, BS = A->bs;
     const int nb = (n + BS - 1) / BS;
     std::vector<int> bia(n + 1, 0), bsrc, bdiag(n);
     std::vector<unsigned short> bja;
@@ -645,24 +646,25 @@ static int build_bj(fvp_csr *A) {
         for (int r = 0; r < cnt; ++r) maxrow = std::max(maxrow, bia[base + r + 1] - bia[base + r]);
     }
     if (maxrow > 64) { set_error("block-Jacobi ILU(0) supports rows of at most 64 block entries"); return -1; }
+    if (maxnz > 65535) { set_error("block-Jacobi ILU(0): more than 65535 entries in one block"); return -1; }
     A->b_maxnz = maxnz;
-    A->b_lds = bj_lds(maxnz, sizeof(double));
-    int lds_max = 0;
-    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, A->device) != hipSuccess)
-        lds_max = 64 * 1024;
-    if (A->b_lds + sizeof(double) * BS > (size_t)lds_max) {
-        set_error("block-Jacobi ILU(0): block needs " + std::to_string(A->b_lds) + " B of LDS, device allows " +
-                  std::to_string(lds_max));
-        return -1;
+    A->b_lds = bj_lds(maxnz, sizeof(double), BS);
+    if (hipDeviceGetAttribute(&A->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, A->device) != hipSuccess)
+        A->lds_max = 64 * 1024;
+    // the solve stages the right-hand side as well: it is the larger of the two
+    const size_t need64 = A->b_lds + sizeof(double) * BS, need32 = bj_lds(maxnz, sizeof(float), BS) + sizeof(float) * BS;
+    if (need64 <= (size_t)A->lds_max) {
+        (void)hipFuncSetAttribute((const void *)bj_factor<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)A->b_lds);
+        (void)hipFuncSetAttribute((const void *)bj_solve<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)need64);
     }
-    (void)hipFuncSetAttribute((const void *)bj_factor<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)A->b_lds);
-    (void)hipFuncSetAttribute((const void *)bj_solve<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(A->b_lds + sizeof(double) * BS));
-    (void)hipFuncSetAttribute((const void *)bj_factor<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)bj_lds(maxnz, sizeof(float)));
-    (void)hipFuncSetAttribute((const void *)bj_solve<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(bj_lds(maxnz, sizeof(float)) + sizeof(float) * BS));
+    if (need32 <= (size_t)A->lds_max) {
+        (void)hipFuncSetAttribute((const void *)bj_factor<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)bj_lds(maxnz, sizeof(float), BS));
+        (void)hipFuncSetAttribute((const void *)bj_solve<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)need32);
+    }
     A->nblk = nb;
     A->bj = true;
     return 0;
@@ -684,14 +686,22 @@ static int precon_build_impl(fvp_csr *A, int type) {
             int rc = build_bj(A);
             if (rc) { A->ptype = -1; return rc; }
         }
+        const size_t vb = type == FVP_PRECON_BJILU0 ? sizeof(double) : sizeof(float);
+        const size_t need = bj_lds(A->b_maxnz, vb, A->bs) + vb * A->bs;
+        if (need > (size_t)A->lds_max) {
+            set_error("block-Jacobi ILU(0): a block of " + std::to_string(A->bs) + " rows needs " +
+                      std::to_string(need) + " B of LDS, the device allows " + std::to_string(A->lds_max));
+            A->ptype = -1;
+            return -1;
+        }
         if (type == FVP_PRECON_BJILU0)
-            bj_factor<double><<<A->nblk, 1024, A->b_lds, s>>>(n, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p,
+            bj_factor<double><<<A->nblk, 1024, A->b_lds, s>>>(n, A->bs, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p,
                                                               A->b_src.p, A->a.p, A->b_lev_l.p, A->b_lptr_l.p,
                                                               A->b_rows_l.p, A->b_lu.p, A->flag.p);
         else {
             if (!A->b_lu32.p && A->b_lu32.alloc(A->b_lu.n)) return -1;
-            bj_factor<float><<<A->nblk, 1024, bj_lds(A->b_maxnz, sizeof(float)), s>>>(
-                n, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p, A->b_lev_l.p, A->b_lptr_l.p,
+            bj_factor<float><<<A->nblk, 1024, bj_lds(A->b_maxnz, sizeof(float), A->bs), s>>>(
+                n, A->bs, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p, A->b_lev_l.p, A->b_lptr_l.p,
                 A->b_rows_l.p, A->b_lu32.p, A->flag.p);
         }
     } else if (type == FVP_PRECON_JACOBI) {
@@ -737,13 +747,13 @@ static int precon_apply_impl(fvp_csr *A, const double *x, double *y) {
         vmul<<<nblk(n, 256), 256, 0, s>>>(n, x, A->jac.p, y);
         break;
     case FVP_PRECON_BJILU0:
-        bj_solve<double><<<A->nblk, 256, A->b_lds + sizeof(double) * FVP_BLOCK_ROWS, s>>>(
-            n, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_lu.p, A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p,
+        bj_solve<double><<<A->nblk, 256, A->b_lds + sizeof(double) * A->bs, s>>>(
+            n, A->bs, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_lu.p, A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p,
             A->b_lev_u.p, A->b_lptr_u.p, A->b_rows_u.p, x, y);
         break;
     case FVP_PRECON_BJILU0_F32:
-        bj_solve<float><<<A->nblk, 256, bj_lds(A->b_maxnz, sizeof(float)) + sizeof(float) * FVP_BLOCK_ROWS, s>>>(
-            n, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_lu32.p, A->b_lev_l.p, A->b_lptr_l.p,
+        bj_solve<float><<<A->nblk, 256, bj_lds(A->b_maxnz, sizeof(float), A->bs) + sizeof(float) * A->bs, s>>>(
+            n, A->bs, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_lu32.p, A->b_lev_l.p, A->b_lptr_l.p,
             A->b_rows_l.p, A->b_lev_u.p, A->b_lptr_u.p, A->b_rows_u.p, x, y);
         break;
     case FVP_PRECON_ILU0: {
@@ -841,6 +851,24 @@ extern "C" int fvp_csr_gemv(fvp_csr *A, double alpha, const double *x, double be
     if (!A || !x || !y) { set_error("null argument"); return -1; }
     FVP_HIP(hipSetDevice(A->device));
     return csr_gemv_impl(A, alpha, x, beta, y);
+}
+
+static int set_block_rows(fvp_csr *A, int rows) {
+    if (rows < 64 || rows > 4096 || (rows & (rows - 1))) {
+        set_error("block rows must be a power of two in [64, 4096]");
+        return -1;
+    }
+    if (rows != A->bs) {
+        A->bs = rows;
+        A->bj = false;  // block tables are rebuilt by the next block-Jacobi build
+        if (A->ptype == FVP_PRECON_BJILU0 || A->ptype == FVP_PRECON_BJILU0_F32) A->ptype = -1;
+    }
+    return 0;
+}
+
+extern "C" int fvp_precon_block_rows(fvp_csr *A, int rows) {
+    if (!A) { set_error("null matrix"); return -1; }
+    return set_block_rows(A, rows);
 }
 
 extern "C" int fvp_precon_build(fvp_csr *A, int type) {
@@ -1076,6 +1104,13 @@ extern "C" int fvp_jacobian(fvp_ctx *c, fvp_csr **A) {
     if (jac_init(c)) return -1;
     *A = c->jd->J;
     return 0;
+}
+
+extern "C" int fvp_set_block_rows(fvp_ctx *c, int rows) {
+    if (!c) { set_error("null argument"); return -1; }
+    FVP_HIP(hipSetDevice(c->device));
+    if (jac_init(c)) return -1;
+    return set_block_rows(c->jd->J, rows);
 }
 
 extern "C" int fvp_colours(fvp_ctx *c, int *n_colours, int *colour_host) {

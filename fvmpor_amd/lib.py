@@ -48,6 +48,8 @@ SIGNATURES = {
     "fvp_csr_gemv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_double,
                                     ctypes.c_void_p]),
     "fvp_precon_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "fvp_precon_block_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "fvp_set_block_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fvp_precon_apply": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fvp_precon_values": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p]),
     "fvp_gmres_csr": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -261,6 +263,9 @@ class CsrMatrix:
     def precon_build(self, ptype) -> int:
         return _check(lib().fvp_precon_build(self.h, int(ptype)), "fvp_precon_build")
 
+    def precon_block_rows(self, rows):
+        _check(lib().fvp_precon_block_rows(self.h, int(rows)), "fvp_precon_block_rows")
+
     def precon_apply(self, x, y):
         _check(lib().fvp_precon_apply(self.h, _ptr(x), _ptr(y)), "fvp_precon_apply")
 
@@ -290,6 +295,10 @@ def _ctx_jacobian(self) -> CsrMatrix:
     h = ctypes.c_void_p()
     _check(lib().fvp_jacobian(self.h, ctypes.byref(h)), "fvp_jacobian")
     return CsrMatrix(None, None, handle=h)
+
+
+def _ctx_set_block_rows(self, rows):
+    _check(lib().fvp_set_block_rows(self.h, int(rows)), "fvp_set_block_rows")
 
 
 def _ctx_colours(self):
@@ -324,6 +333,7 @@ Context.precon_setup = _ctx_precon_setup
 Context.precon_solve = _ctx_precon_solve
 Context.jacobian = _ctx_jacobian
 Context.colours = _ctx_colours
+Context.set_block_rows = _ctx_set_block_rows
 Context.newton_gmres = _ctx_newton_gmres
 
 
@@ -331,7 +341,8 @@ class IntegratorParams(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("max_order", ctypes.c_int),
                 ("h0", ctypes.c_double), ("h_max", ctypes.c_double), ("precon_type", ctypes.c_int),
                 ("dim_krylov", ctypes.c_int), ("max_restarts", ctypes.c_int), ("lin_factor", ctypes.c_double),
-                ("newton_tol", ctypes.c_double), ("max_newton", ctypes.c_int), ("max_steps", ctypes.c_int)]
+                ("newton_tol", ctypes.c_double), ("max_newton", ctypes.c_int), ("max_steps", ctypes.c_int),
+                ("block_rows", ctypes.c_int)]
 
 
 class IntegratorStats(ctypes.Structure):
@@ -349,10 +360,10 @@ class Integrator:
 
     def __init__(self, ctx: Context, psi0, t0=0.0, rtol=1e-8, atol=1e-8, max_order=3, h0=0.0, h_max=0.0,
                  precon=PRECON_ILU0, dim_krylov=20, max_restarts=5, lin_factor=0.05, newton_tol=0.33,
-                 max_newton=4, max_steps=1000000):
+                 max_newton=4, max_steps=1000000, block_rows=0):
         self.ctx = ctx
         self.params = IntegratorParams(rtol, atol, max_order, h0, h_max, precon, dim_krylov, max_restarts,
-                                       lin_factor, newton_tol, max_newton, max_steps)
+                                       lin_factor, newton_tol, max_newton, max_steps, block_rows)
         h = ctypes.c_void_p()
         _check(lib().fvp_integrator_create(ctx.h, ctypes.byref(self.params), float(t0), _ptr(psi0),
                                            ctypes.byref(h)), "fvp_integrator_create")

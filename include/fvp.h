@@ -133,6 +133,10 @@ int fvp_csr_gemv(fvp_csr *A, double alpha, const double *x, double beta, double 
 /* build the preconditioner of A (precon(), precon.c:151-195; ILU0_factor,
  * ILU.c:12-115 with the diagonal of row i inverted; precon_jacobi, MR.c:143-200) */
 int fvp_precon_build(fvp_csr *A, int type);
+/* Rows per block of FVP_PRECON_BJILU0(_F32): a power of two in [64, 4096],
+ * default FVP_BLOCK_ROWS; takes effect at the next build.  The build fails
+ * (< 0) when one block's factor does not fit the workgroup's LDS. */
+int fvp_precon_block_rows(fvp_csr *A, int rows);
 /* y = M^{-1} x (precon_apply, precon.c:112-149) */
 int fvp_precon_apply(fvp_csr *A, const double *x, double *y);
 /* host copy of the factored values (ILU(0): L unit-lower, U with inverted diagonal) */
@@ -171,6 +175,8 @@ int fvp_precon_setup(fvp_ctx *ctx, double t, double c_j, double h, const double 
 int fvp_precon_solve(fvp_ctx *ctx, const double *r, double *z);
 /* the context's Jacobian as a CSR handle (pattern = node adjacency plus diagonal) */
 int fvp_jacobian(fvp_ctx *ctx, fvp_csr **A);
+/* rows per block of the context's block-Jacobi preconditioner (see fvp_precon_block_rows) */
+int fvp_set_block_rows(fvp_ctx *ctx, int rows);
 /* number of colours of the distance-2 colouring (preconditioner_dss.cpp:72-94) */
 int fvp_colours(fvp_ctx *ctx, int *n_colours, int *colour_host);
 /* Solve J x = b, J = dF/dpsi + c_j dF/dpsi' applied matrix-free (fvp_jv) and
@@ -199,6 +205,7 @@ typedef struct {
     double newton_tol;    /* IDA epsNewt, 0.33 */
     int max_newton;       /* IDA maxcor, 4 */
     int max_steps;        /* per advance call */
+    int block_rows;       /* block-Jacobi block size, 0 = FVP_BLOCK_ROWS */
 } fvp_integrator_params;
 
 typedef struct {

@@ -169,16 +169,20 @@ def _block_diagonal(ia, ja, a, bs):
     return np.cumsum(kia).astype(np.int32), ja[keep].astype(np.int32), a[keep], keep
 
 
-def test_block_jacobi_ilu0_is_blockwise_ilu0(box_mid):
+@pytest.mark.parametrize("rows", [0, 256, 1024])
+def test_block_jacobi_ilu0_is_blockwise_ilu0(box_mid, rows):
     import torch
     from fvmpor_amd.lib import CsrMatrix, PRECON_BJILU0, BLOCK_ROWS
     ia, ja, a = _matrix(box_mid.mesh, seed=3, shift=1.2)
     n = len(ia) - 1
-    assert n > 2 * BLOCK_ROWS
+    bs = rows or BLOCK_ROWS
+    assert n > 2 * bs
     A = CsrMatrix(ia, ja)
+    if rows:
+        A.precon_block_rows(rows)
     A.set_values(_dev(torch, a))
     assert A.precon_build(PRECON_BJILU0) == 0
-    bia, bja, ba, keep = _block_diagonal(ia, ja, a, BLOCK_ROWS)
+    bia, bja, ba, keep = _block_diagonal(ia, ja, a, bs)
     lu_o, dp = O.ilu0(bia, bja, ba)
     lu_g = A.precon_values(len(ja))
     np.testing.assert_allclose(lu_g[keep], lu_o, rtol=1e-12, atol=1e-14)
@@ -223,3 +227,12 @@ def test_block_jacobi_ilu0_fp32(box_mid):
     rc, st = A.gmres(_dev(torch, b), xs, m=30, max_restarts=40, tol=1e-12, reorth=1)
     assert rc == 0 and st["converged"]
     assert np.abs(xs.cpu().numpy() - 1.0).max() < 1e-9
+
+
+def test_block_rows_rejects_invalid_sizes(box_mid):
+    from fvmpor_amd.lib import CsrMatrix, FvpError
+    ia, ja, a = _matrix(box_mid.mesh, seed=3, shift=1.2)
+    A = CsrMatrix(ia, ja)
+    for bad in (100, 32, 8192):
+        with pytest.raises(FvpError):
+            A.precon_block_rows(bad)
