@@ -73,11 +73,11 @@ __device__ __forceinline__ void load_pt(const double *__restrict__ xyz, int i, d
     x = xyz[3 * i]; y = xyz[3 * i + 1]; z = xyz[3 * i + 2];
 }
 
-// Local edge tables (mesh.cpp:409-500 element edge order).
-__constant__ int kTetA[6] = {0, 1, 2, 0, 1, 2};
-__constant__ int kTetB[6] = {1, 2, 0, 3, 3, 3};
-__constant__ int kTriA[3] = {0, 1, 2};
-__constant__ int kTriB[3] = {1, 2, 0};
+// Local edges in the element edge order of mesh.cpp:409-500: triangle edges
+// (0,1) (1,2) (2,0); tetrahedron edges (0,1) (1,2) (2,0) (0,3) (1,3) (2,3).
+// Compile-time functions, so that unrolled loops index registers directly.
+__device__ constexpr int edge_a(int j) { return j % 3; }
+__device__ constexpr int edge_b(int j) { return j < 3 ? (j + 1) % 3 : 3; }
 
 // Face fluxes of one element, summed per element node (f[k] = net inflow into
 // node k's control volume, rho0 k_r q.S over the element's CV faces).
@@ -149,8 +149,8 @@ __device__ __forceinline__ void element_fluxes(int e, int N, const ZoneTable &zt
     const unsigned orient = MODE == 1 ? eorient[e] : 0u;
 #pragma unroll
     for (int j = 0; j < NED; ++j) {
-        const int a = DIM == 3 ? kTetA[j] : kTriA[j];
-        const int b = DIM == 3 ? kTetB[j] : kTriB[j];
+        const int a = edge_a(j);
+        const int b = edge_b(j);
         double flux = sc * (g[b] - g[a]);  // flow a -> b
         double krf;
         if (MODE == 0) {
@@ -276,6 +276,11 @@ __global__ void __launch_bounds__(256) tile_lds_kernel(
         for (int q = 0; q < ntz; ++q) sk[q * wmax + i] = kr[(size_t)tz[q] * N + g];
     }
     for (int i = threadIdx.x; i < nown; i += blockDim.x) acc[i] = 0.;
+    // per-zone conductivity tensor diagonal in LDS: elements of one wavefront
+    // index it with their own zone, which a kernel-argument table would turn
+    // into select chains over every zone
+    __shared__ double zk[3 * kMaxZones];
+    if (threadIdx.x < 3 * zt.nz) zk[threadIdx.x] = zt.p[threadIdx.x / 3][ZK_X + threadIdx.x % 3];
     __syncthreads();
     constexpr int NPE = DIM + 1;
     constexpr int NED = DIM == 2 ? 3 : 6;
@@ -289,7 +294,6 @@ __global__ void __launch_bounds__(256) tile_lds_kernel(
         const double km = kmul[e];
         const unsigned mask = emask[e];
         const int slot = z == tz[0] ? 0 : z == tz[1] ? 1 : z == tz[2] ? 2 : 3;
-        const double *zp = zt.p[z];
         double px[NPE], py[NPE], pz[NPE], h[NPE];
 #pragma unroll
         for (int q = 0; q < NPE; ++q) {
@@ -322,10 +326,10 @@ __global__ void __launch_bounds__(256) tile_lds_kernel(
             gx += dh * G[q][0]; gy += dh * G[q][1]; gz += dh * G[q][2];
         }
         gx *= inv; gy *= inv; gz *= inv;
-        double qx = gx * -(zp[ZK_X] * km);
+        double qx = gx * -(zk[3 * z] * km);
         double qy, qzz;
-        if (DIM == 2) { qy = (gy + 1.) * -(zp[ZK_Y] * km); qzz = 0.; }
-        else { qy = gy * -(zp[ZK_Y] * km); qzz = (gz + 1.) * -(zp[ZK_Z] * km); }
+        if (DIM == 2) { qy = (gy + 1.) * -(zk[3 * z + 1] * km); qzz = 0.; }
+        else { qy = gy * -(zk[3 * z + 1] * km); qzz = (gz + 1.) * -(zk[3 * z + 2] * km); }
         double g[NPE];
 #pragma unroll
         for (int q = 0; q < NPE; ++q) g[q] = G[q][0] * qx + G[q][1] * qy + G[q][2] * qzz;
@@ -336,8 +340,8 @@ __global__ void __launch_bounds__(256) tile_lds_kernel(
         const double *krs = sk + slot * wmax;
 #pragma unroll
         for (int j = 0; j < NED; ++j) {
-            const int a = DIM == 3 ? kTetA[j] : kTriA[j];
-            const int b = DIM == 3 ? kTetB[j] : kTriB[j];
+            const int a = edge_a(j);
+            const int b = edge_b(j);
             const double flux = sc * (g[b] - g[a]);
             const double krf = krs[(mask >> j) & 1u ? l[a] : l[b]];
             const double M = rho0 * (krf * flux);
