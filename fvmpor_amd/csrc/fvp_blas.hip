@@ -109,6 +109,12 @@ extern "C" int fvp_jv(fvp_ctx *c, double t, const double *psi, const double *psi
     dot_partial<<<kRedBlocks, kRedThreads, 0, s>>>(N, psi, psi, part);
     dot_partial<<<kRedBlocks, kRedThreads, 0, s>>>(N, v, v, part + kRedBlocks);
     jv_sigma<<<1, kRedThreads, 0, s>>>(kRedBlocks, part, part + kRedBlocks, part + 2 * kRedBlocks);
+    FVP_HIP(hipGetLastError());
+    if (!c->averaging) {  // perturbation and difference quotient fused into the residual kernels
+        JvArgs jv;
+        jv.v = v; jv.sig = part + 2 * kRedBlocks; jv.F0 = F0; jv.cj = c_j;
+        return residual_impl(c, t, psi, psip, Jv, 0, &jv);
+    }
     jv_perturb<<<nblk(N, 256), 256, 0, s>>>(N, part + 2 * kRedBlocks, c_j, psi, psip, v, c->tmp1.p, c->tmp2.p);
     FVP_HIP(hipGetLastError());
     if (residual_impl(c, t, c->tmp1.p, c->tmp2.p, c->tmp3.p, c->averaging)) return -1;

@@ -103,7 +103,15 @@ struct fvp_ctx {
 };
 
 namespace fvp {
-int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, double *F, int averaging);
+// Directional-derivative mode of the residual: the state is psi + sig v,
+// psi' + c_j sig v, and the output is (F(state) - F0) / sig (sig is a device
+// scalar), fused into the residual kernels.
+struct JvArgs {
+    const double *v = nullptr, *sig = nullptr, *F0 = nullptr;
+    double cj = 0.;
+};
+int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, double *F, int averaging,
+                  const JvArgs *jv = nullptr);
 int tile_kernel_attrs(fvp_ctx *c);
 int upload_bc_values(fvp_ctx *c);
 void jac_free(fvp_ctx *c);

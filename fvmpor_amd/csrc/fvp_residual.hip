@@ -13,10 +13,12 @@
 // Kernels:
 //   vg_kernel      per node: van Genuchten Sw, dSw/dpsi, k_r for every zone the
 //                  CV touches, CV-averaged capacity ahh (fvmpor.h:874-945)
-//   elem_kernel    per element: gradient, 3 or 6 face fluxes, upwinded k_r,
-//                  scatter to the element's nodes
-//   finalize       F = acc / V - psi' ahh, boundary fluxes, Dirichlet rows
+//   tile_lds_kernel  per node tile: gradient, 3 or 6 face fluxes and upwinded
+//                  k_r of every element touching the tile, flux sums in LDS,
+//                  then F = acc / V - psi' ahh, boundary fluxes, Dirichlet rows
 //                  (fvmpor_ODE_impl.cpp:99-115)
+//   elem_kernel    per element, atomic scatter: the averaged-weight pass of
+//                  begin_step (fvmpor_ODE_impl.cpp:118-142), then finalize
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -47,12 +49,15 @@ __device__ __forceinline__ void vg_eval(double h, const double *zp, double &Sw, 
     Sw = s; dSw = d; krw = k;
 }
 
+template <bool JV>
 __global__ void __launch_bounds__(256) vg_kernel(int N, ZoneTable zt, double rho0, const double *__restrict__ psi,
                                                  const uint8_t *__restrict__ zmask, const double *__restrict__ zw,
-                                                 double *__restrict__ kr, double *__restrict__ cap) {
+                                                 double *__restrict__ kr, double *__restrict__ cap,
+                                                 const double *__restrict__ v, const double *__restrict__ sig) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     double h = psi[i];
+    if (JV) h = psi[i] + sig[0] * v[i];
     unsigned m = zmask[i];
     double phi = 0., Sw = 0., dSw = 0., Ss = 0.;
     for (int z = 0; z < zt.nz; ++z) {
@@ -191,51 +196,6 @@ __global__ void __launch_bounds__(256) elem_kernel(int T, int N, ZoneTable zt, d
     for (int k = 0; k < DIM + 1; ++k) atomicAdd(&acc[n[k]], f[k]);
 }
 
-// Owner-computes residual: one workgroup per node tile of kTileNodes nodes.  The
-// tile's flux sums live in LDS; every element touching the tile is evaluated
-// (elements shared by several tiles are evaluated once per tile) and only the
-// contributions to owned nodes are kept, so no global atomics are issued.  The
-// finalisation F = acc / V - psi' ahh, boundary fluxes and Dirichlet rows are
-// fused.  Tiles of one XCD are contiguous so their halo nodes share that XCD's L2.
-template <int DIM>
-__global__ void __launch_bounds__(256) tile_kernel(
-    int N, int ntiles, int per_xcd, ZoneTable zt, double rho0, const int4 *__restrict__ conn,
-    const uint8_t *__restrict__ ezone, const double *__restrict__ kmul, const uint8_t *__restrict__ emask,
-    const double *__restrict__ xyz, const double *__restrict__ psi, const double *__restrict__ kr,
-    const int *__restrict__ eptr, const int *__restrict__ hptr, const int *__restrict__ hlist,
-    const double *__restrict__ invV, const double *__restrict__ psip, const double *__restrict__ cap,
-    const int *__restrict__ bptr, const int *__restrict__ bnode, const double *__restrict__ bval,
-    const int *__restrict__ dptr, const int *__restrict__ dnode, const double *__restrict__ dval,
-    double *__restrict__ F) {
-    __shared__ double acc[kTileNodes];
-    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (tile >= ntiles) return;  // uniform for the whole workgroup
-    const int lo = tile * kTileNodes, hi = min(N, lo + kTileNodes);
-    for (int i = threadIdx.x; i < kTileNodes; i += blockDim.x) acc[i] = 0.;
-    __syncthreads();
-    const int e0 = eptr[tile], np = eptr[tile + 1] - e0;
-    const int h0 = hptr[tile], ntot = np + hptr[tile + 1] - h0;
-    for (int k = threadIdx.x; k < ntot; k += blockDim.x) {
-        const int e = k < np ? e0 + k : hlist[h0 + k - np];
-        int n[4];
-        double f[4];
-        element_fluxes<DIM, 0>(e, N, zt, rho0, conn, ezone, kmul, emask, nullptr, xyz, psi, kr, nullptr, nullptr, n,
-                               f);
-#pragma unroll
-        for (int q = 0; q < DIM + 1; ++q)
-            if (n[q] >= lo && n[q] < hi) atomicAdd(&acc[n[q] - lo], f[q]);
-    }
-    __syncthreads();
-    for (int k = bptr[tile] + threadIdx.x; k < bptr[tile + 1]; k += blockDim.x) acc[bnode[k] - lo] -= bval[k];
-    __syncthreads();
-    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) F[i] = acc[i - lo] * invV[i] - psip[i] * cap[i];
-    __syncthreads();
-    for (int k = dptr[tile] + threadIdx.x; k < dptr[tile + 1]; k += blockDim.x) {
-        const int i = dnode[k];
-        F[i] = dval[k] - psi[i];
-    }
-}
-
 // LDS-staged owner-computes residual.  A workgroup owns kTileNodes nodes; it
 // stages the coordinates, heads and relative permeabilities of its window (owned
 // nodes plus the halo nodes its elements reach) in LDS with coalesced loads for
@@ -243,7 +203,11 @@ __global__ void __launch_bounds__(256) tile_kernel(
 // touching the tile from LDS using the window-local connectivity (8 B per
 // element visit), accumulates the flux sums of owned nodes in LDS (ds_add_f64),
 // and finishes F = acc / V - psi' ahh, boundary fluxes and Dirichlet rows.
-template <int DIM>
+// Elements shared by several tiles are evaluated once per tile and only the
+// contributions to owned nodes are kept, so no global atomics are issued; the
+// tiles of one XCD are contiguous so their halo nodes share that XCD's L2.
+// JV: directional-derivative mode (state psi + sig v, output (F - F0) / sig).
+template <int DIM, bool JV>
 __global__ void __launch_bounds__(256) tile_lds_kernel(
     int N, int ntiles, int per_xcd, int wmax, int ntzmax, ZoneTable zt, double rho0, const uint8_t *__restrict__ ezone,
     const double *__restrict__ kmul, const uint8_t *__restrict__ emask, const double *__restrict__ xyz,
@@ -253,10 +217,11 @@ __global__ void __launch_bounds__(256) tile_lds_kernel(
     const int *__restrict__ tzone, const double *__restrict__ invV, const double *__restrict__ psip,
     const double *__restrict__ cap, const int *__restrict__ bptr, const int *__restrict__ bnode,
     const double *__restrict__ bval, const int *__restrict__ dptr, const int *__restrict__ dnode,
-    const double *__restrict__ dval, double *__restrict__ F) {
+    const double *__restrict__ dval, double *__restrict__ F, JvArgs jv) {
     extern __shared__ double sm[];
     const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (tile >= ntiles) return;  // uniform for the whole workgroup
+    const double sig = JV ? jv.sig[0] : 0.;
     const int lo = tile * kTileNodes, hi = min(N, lo + kTileNodes), nown = hi - lo;
     const int w0 = wptr[tile], win = nown + wptr[tile + 1] - w0;
     int tz[4], ntz = 0;
@@ -272,7 +237,7 @@ __global__ void __launch_bounds__(256) tile_lds_kernel(
         sx[i] = xyz[3 * g];
         sy[i] = xyz[3 * g + 1];
         sz[i] = xyz[3 * g + 2];
-        sp[i] = psi[g];
+        sp[i] = JV ? psi[g] + sig * jv.v[g] : psi[g];
         for (int q = 0; q < ntz; ++q) sk[q * wmax + i] = kr[(size_t)tz[q] * N + g];
     }
     for (int i = threadIdx.x; i < nown; i += blockDim.x) acc[i] = 0.;
@@ -355,11 +320,22 @@ __global__ void __launch_bounds__(256) tile_lds_kernel(
     __syncthreads();
     for (int k = bptr[tile] + threadIdx.x; k < bptr[tile + 1]; k += blockDim.x) acc[bnode[k] - lo] -= bval[k];
     __syncthreads();
-    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) F[i] = acc[i - lo] * invV[i] - psip[i] * cap[i];
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        if (JV) {
+            const double yp = psip[i] + jv.cj * (sig * jv.v[i]);
+            const double F1 = acc[i - lo] * invV[i] - yp * cap[i];
+            F[i] = (F1 - jv.F0[i]) / sig;
+        } else
+            F[i] = acc[i - lo] * invV[i] - psip[i] * cap[i];
+    }
     __syncthreads();
     for (int k = dptr[tile] + threadIdx.x; k < dptr[tile + 1]; k += blockDim.x) {
         const int i = dnode[k];
-        F[i] = dval[k] - psi[i];
+        if (JV) {
+            const double y = psi[i] + sig * jv.v[i];
+            F[i] = ((dval[k] - y) - jv.F0[i]) / sig;
+        } else
+            F[i] = dval[k] - psi[i];
     }
 }
 
@@ -432,36 +408,55 @@ int tile_kernel_attrs(fvp_ctx *c) {
                   std::to_string(lds_max));
         return -1;
     }
-    FVP_HIP(hipFuncSetAttribute((const void *)tile_lds_kernel<3>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    FVP_HIP(hipFuncSetAttribute((const void *)tile_lds_kernel<3, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)c->tile_lds));
-    FVP_HIP(hipFuncSetAttribute((const void *)tile_lds_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    FVP_HIP(hipFuncSetAttribute((const void *)tile_lds_kernel<2, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)c->tile_lds));
+    FVP_HIP(hipFuncSetAttribute((const void *)tile_lds_kernel<3, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)c->tile_lds));
+    FVP_HIP(hipFuncSetAttribute((const void *)tile_lds_kernel<2, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)c->tile_lds));
     return 0;
 }
 
-int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, double *F, int averaging) {
+template <int DIM, bool JV>
+static void launch_tile(fvp_ctx *c, const double *psi, const double *psip, double *F, const JvArgs &jv) {
+    const int per = (c->ntiles + 7) / 8;  // tiles per XCD: consecutive tiles share an XCD's L2
+    tile_lds_kernel<DIM, JV><<<per * 8, 256, c->tile_lds, c->stream>>>(
+        c->N, c->ntiles, per, c->win_max, c->ntz_max, c->zt, c->rho0, c->ezone.p, c->kmul.p, c->emask.p, c->xyz.p,
+        psi, c->kr.p, c->t_eptr.p, c->t_hptr.p, c->t_hlist.p, c->t_wptr.p, c->t_whalo.p, c->t_vptr.p,
+        (const ushort4 *)c->t_lconn.p, c->t_tz.p, c->invV.p, psip, c->cap.p, c->t_bptr.p, c->bnode.p, c->bval.p,
+        c->t_dptr.p, c->dnode.p, c->dval.p, F, jv);
+}
+
+int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, double *F, int averaging,
+                  const JvArgs *jv) {
     (void)t;
     hipStream_t s = c->stream;
     const int N = c->N, T = c->T;
+    if (jv && averaging) {
+        set_error("internal: fused J v needs the upwinded residual");
+        return -1;
+    }
     ++c->nres;
     prof_mark(c, 1, 0);
-    vg_kernel<<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p, c->cap.p);
+    if (jv)
+        vg_kernel<true><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p,
+                                                        c->cap.p, jv->v, jv->sig);
+    else
+        vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p,
+                                                         c->cap.p, nullptr, nullptr);
     if (!averaging) {
         // tiled owner-computes path: element fluxes, boundary, finalize and Dirichlet in one launch
-        const int per = (c->ntiles + 7) / 8;
         prof_mark(c, 0, 0);
-        if (c->dim == 3)
-            tile_lds_kernel<3><<<per * 8, 256, c->tile_lds, s>>>(
-                N, c->ntiles, per, c->win_max, c->ntz_max, c->zt, c->rho0, c->ezone.p, c->kmul.p, c->emask.p, c->xyz.p, psi,
-                c->kr.p, c->t_eptr.p, c->t_hptr.p, c->t_hlist.p, c->t_wptr.p, c->t_whalo.p, c->t_vptr.p,
-                (const ushort4 *)c->t_lconn.p, c->t_tz.p, c->invV.p, psip, c->cap.p, c->t_bptr.p, c->bnode.p,
-                c->bval.p, c->t_dptr.p, c->dnode.p, c->dval.p, F);
-        else
-            tile_lds_kernel<2><<<per * 8, 256, c->tile_lds, s>>>(
-                N, c->ntiles, per, c->win_max, c->ntz_max, c->zt, c->rho0, c->ezone.p, c->kmul.p, c->emask.p, c->xyz.p, psi,
-                c->kr.p, c->t_eptr.p, c->t_hptr.p, c->t_hlist.p, c->t_wptr.p, c->t_whalo.p, c->t_vptr.p,
-                (const ushort4 *)c->t_lconn.p, c->t_tz.p, c->invV.p, psip, c->cap.p, c->t_bptr.p, c->bnode.p,
-                c->bval.p, c->t_dptr.p, c->dnode.p, c->dval.p, F);
+        const JvArgs none;
+        if (c->dim == 3) {
+            if (jv) launch_tile<3, true>(c, psi, psip, F, *jv);
+            else launch_tile<3, false>(c, psi, psip, F, none);
+        } else {
+            if (jv) launch_tile<2, true>(c, psi, psip, F, *jv);
+            else launch_tile<2, false>(c, psi, psip, F, none);
+        }
         prof_mark(c, 0, 1);
     } else {
         FVP_HIP(hipMemsetAsync(c->acc.p, 0, sizeof(double) * N, s));
