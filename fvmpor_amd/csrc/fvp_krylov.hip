@@ -56,6 +56,9 @@ struct fvp_csr {
     // workspace
     int m_alloc = -1;
     DevBuf<double> Q, w, r, z, part, scal, hdev;
+    DevBuf<double> Hd, gd, csd, snd, yd;    // Hessenberg (column-major, m+1 rows), Givens pairs, rhs, solution
+    double *hstat = nullptr;                // pinned host: per Arnoldi step {residual estimate, h_{j+1,j}}
+    std::vector<hipEvent_t> ev;             // per Arnoldi step: its status copy has landed
 };
 
 namespace fvp {
@@ -174,37 +177,157 @@ __global__ void vscale_inv_norm(long n, const double *__restrict__ src, const do
     if (i < n) dst[i] = src[i] * (1. / sqrt(nsq[0]));
 }
 
-// part[c * gridDim.x + block] = sum over the block's slice of w * Q[:, c0 + c], c < nc <= 8
-__global__ void __launch_bounds__(kThreads) multidot8(long n, const double *__restrict__ w,
-                                                      const double *__restrict__ Q, long ldq, int nc,
-                                                      double *__restrict__ part) {
-    __shared__ double lds[8][kThreads / 64];
-    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        double wi = w[i];
+// dst = src * s[0] (s a device scalar: the reciprocal of the new basis vector's norm)
+__global__ void vscale_dev(long n, const double *__restrict__ src, const double *__restrict__ sc,
+                           double *__restrict__ dst) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] * sc[0];
+}
+
+constexpr int kChunk = 16;  // basis columns per Gram-Schmidt dot pass
+
+__device__ __forceinline__ double wave_reduce(double v) {
 #pragma unroll
-        for (int c = 0; c < 8; ++c)
-            if (c < nc) s[c] += wi * Q[c * ldq + i];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Block partials of w . Q[:, c] for c < nc (nc <= kChunk), and of w . w (stored
+// as column nc) when ww is set: part[c * gridDim.x + block].  Skipped entirely
+// when skip points at 0 (the second Gram-Schmidt pass was not needed).
+__global__ void __launch_bounds__(kThreads) mdot_kernel(long n, const double *__restrict__ w,
+                                                        const double *__restrict__ Q, long ldq, int nc, int ww,
+                                                        double *__restrict__ part, const double *__restrict__ skip) {
+    if (skip && skip[0] == 0.) return;
+    __shared__ double lds[kChunk + 1][kThreads / 64];
+    double acc[kChunk + 1];
+#pragma unroll
+    for (int c = 0; c <= kChunk; ++c) acc[c] = 0.;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double wi = w[i];
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c)
+            if (c < nc) acc[c] += wi * Q[c * ldq + i];
+        acc[kChunk] += wi * wi;
     }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        double v = s[c];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if (lane == 0) lds[c][wid] = v;
+    for (int c = 0; c <= kChunk; ++c) {
+        if (c < nc || (c == kChunk && ww)) {
+            const double v = wave_reduce(acc[c]);
+            if (lane == 0) lds[c][wid] = v;
+        }
     }
     __syncthreads();
-    if (threadIdx.x < nc) {
+    const int c = threadIdx.x;
+    if (c < nc || (c == kChunk && ww)) {
         double v = 0.;
-        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) v += lds[threadIdx.x][k];
-        part[threadIdx.x * gridDim.x + blockIdx.x] = v;
+        for (int k = 0; k < kThreads / 64; ++k) v += lds[c][k];
+        part[(c < nc ? c : nc) * gridDim.x + blockIdx.x] = v;
     }
 }
 
-// out[c] (+)= sum_b part[c * nb + b]; one block per column
+// w -= Q[:, 0..nc) coef, and block partials of the updated w . w into part[block]
+__global__ void __launch_bounds__(kThreads) axpy_norm(long n, double *__restrict__ w, const double *__restrict__ Q,
+                                                      long ldq, int nc, const double *__restrict__ coef,
+                                                      double *__restrict__ part, const double *__restrict__ skip) {
+    if (skip && skip[0] == 0.) return;
+    __shared__ double lds[kThreads / 64];
+    double acc = 0.;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double s = 0.;
+        for (int c = 0; c < nc; ++c) s += coef[c] * Q[c * ldq + i];
+        const double wi = w[i] - s;
+        w[i] = wi;
+        acc += wi * wi;
+    }
+    const double v = wave_reduce(acc);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.;
+        for (int k = 0; k < kThreads / 64; ++k) t += lds[k];
+        part[blockIdx.x] = t;
+    }
+}
+
+// Selective reorthogonalisation (Kahan-Parlett "twice is enough"): a second
+// classical Gram-Schmidt pass when the first one cancelled more than half of
+// ||w||^2 (mode 1), always (mode 2) or never (mode 0).
+__global__ void reorth_decide(const double *__restrict__ ww, const double *__restrict__ n1, int mode,
+                              double *__restrict__ flag) {
+    if (threadIdx.x == 0) flag[0] = mode == 2 ? 1. : mode == 0 ? 0. : (n1[0] < 0.5 * ww[0] ? 1. : 0.);
+}
+
+// netlib drotg
+__host__ __device__ inline void drotg_dev(double *da, double *db, double *c, double *s) {
+    double roe = fabs(*da) > fabs(*db) ? *da : *db;
+    double scale = fabs(*da) + fabs(*db), r, z;
+    if (scale == 0.) { *c = 1.; *s = 0.; r = 0.; z = 0.; }
+    else {
+        double a = *da / scale, b = *db / scale;
+        r = scale * sqrt(a * a + b * b);
+        r = (roe >= 0 ? 1. : -1.) * r;
+        *c = *da / r; *s = *db / r;
+        z = 1.;
+        if (fabs(*da) > fabs(*db)) z = *s;
+        if (fabs(*db) >= fabs(*da) && *c != 0.) z = 1. / *c;
+    }
+    *da = r; *db = z;
+}
+
+// New Hessenberg column j = h1 (+ h2 after a second pass), h_{j+1,j} = ||w||;
+// previous rotations, the new Givens rotation, the rotated right-hand side g,
+// the residual estimate |g_{j+1}| and h_{j+1,j} into stat, 1/h_{j+1,j} into ninv.
+__global__ void arnoldi_update(int j, int m, const double *__restrict__ h1, const double *__restrict__ h2,
+                               const double *__restrict__ n1, const double *__restrict__ n2,
+                               const double *__restrict__ flag, double *__restrict__ H, double *__restrict__ cs,
+                               double *__restrict__ sn, double *__restrict__ g, double *__restrict__ stat,
+                               double *__restrict__ ninv) {
+    if (threadIdx.x != 0) return;
+    const bool two = flag && flag[0] != 0.;
+    double *hc = H + (size_t)j * (m + 1);
+    for (int i = 0; i <= j; ++i) hc[i] = two ? h1[i] + h2[i] : h1[i];
+    const double hn = sqrt(two ? n2[0] : n1[0]);
+    hc[j + 1] = hn;
+    for (int i = 0; i < j; ++i) {
+        const double t1 = cs[i] * hc[i] + sn[i] * hc[i + 1];
+        hc[i + 1] = cs[i] * hc[i + 1] - sn[i] * hc[i];
+        hc[i] = t1;
+    }
+    double a = hc[j], b = hc[j + 1], c, sv;
+    drotg_dev(&a, &b, &c, &sv);
+    cs[j] = c; sn[j] = sv;
+    hc[j] = c * hc[j] + sv * hc[j + 1];
+    hc[j + 1] = 0.;
+    const double gj = g[j];
+    g[j] = c * gj;
+    g[j + 1] = -sv * gj;
+    stat[0] = fabs(g[j + 1]);
+    stat[1] = hn;
+    ninv[0] = hn > 0. ? 1. / hn : 0.;
+}
+
+__global__ void gmres_cycle_init(double *__restrict__ g, int len, double beta) {
+    for (int i = threadIdx.x; i < len; i += blockDim.x) g[i] = i == 0 ? beta : 0.;
+}
+
+// y = R^{-1} g on the first k rotated columns (back substitution)
+__global__ void gmres_backsolve(int k, int m, const double *__restrict__ H, const double *__restrict__ g,
+                                double *__restrict__ y) {
+    if (threadIdx.x != 0) return;
+    for (int i = k - 1; i >= 0; --i) {
+        double sum = g[i];
+        for (int l = i + 1; l < k; ++l) sum -= H[(size_t)l * (m + 1) + i] * y[l];
+        y[i] = sum / H[(size_t)i * (m + 1) + i];
+    }
+}
+
+// out[c] (+)= sum over blocks of part[c * nb + block]
 __global__ void __launch_bounds__(kThreads) reduce_cols(int nb, const double *__restrict__ part,
-                                                        double *__restrict__ out, int accumulate) {
+                                                        double *__restrict__ out, int accumulate,
+                                                        const double *__restrict__ skip) {
+    if (skip && skip[0] == 0.) return;
     __shared__ double lds[kThreads / 64];
     int c = blockIdx.x;
     double s = 0.;
@@ -221,24 +344,13 @@ __global__ void __launch_bounds__(kThreads) reduce_cols(int nb, const double *__
 }
 
 // w += sign * sum_c coef[c] Q[:, c]
-__global__ void multiaxpy(long n, double *__restrict__ w, const double *__restrict__ Q, long ldq, int nc,
-                          const double *__restrict__ coef, double sign) {
+// out = sum_c y[c] Q[:, c]  (y on the device)
+__global__ void lincomb(long n, double *__restrict__ out, const double *__restrict__ Q, long ldq, int nc,
+                        const double *__restrict__ y) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i >= n) return;
     double s = 0.;
-    for (int c = 0; c < nc; ++c) s += coef[c] * Q[c * ldq + i];
-    w[i] += sign * s;
-}
-
-struct Coef32 {
-    double v[64];
-};
-// out = sum_c y[c] Q[:, c]
-__global__ void lincomb(long n, double *__restrict__ out, const double *__restrict__ Q, long ldq, int nc, Coef32 y) {
-    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double s = 0.;
-    for (int c = 0; c < nc; ++c) s += Q[c * ldq + i] * y.v[c];
+    for (int c = 0; c < nc; ++c) s += Q[c * ldq + i] * y[c];
     out[i] = s;
 }
 
@@ -366,152 +478,129 @@ static int ensure_work(fvp_csr *A, int m) {
     if (A->m_alloc >= m) return 0;
     long n = A->n;
     if (A->Q.alloc((size_t)n * (m + 1)) || A->w.alloc(n) || A->r.alloc(n) || A->z.alloc(n) ||
-        A->part.alloc((size_t)kBlocks * 8) || A->scal.alloc(8) || A->hdev.alloc(2 * (m + 2)))
+        A->part.alloc((size_t)kBlocks * (kChunk + 1)) || A->scal.alloc(16) || A->hdev.alloc(2 * (m + 2) + 2) ||
+        A->Hd.alloc((size_t)(m + 1) * m) || A->gd.alloc(m + 1) || A->csd.alloc(m) || A->snd.alloc(m) ||
+        A->yd.alloc(m))
         return -1;
+    if (A->hstat) (void)hipHostFree(A->hstat);
+    A->hstat = nullptr;
+    if (hipHostMalloc((void **)&A->hstat, sizeof(double) * 2 * (m + 1), hipHostMallocDefault) != hipSuccess) {
+        A->hstat = nullptr;
+        set_error("pinned host allocation failed");
+        return -1;
+    }
+    while ((int)A->ev.size() < m + 1) {
+        hipEvent_t e;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { set_error("event creation failed"); return -1; }
+        A->ev.push_back(e);
+    }
     A->m_alloc = m;
     return 0;
 }
 
-// out[0..nc) = Q[:, 0..nc)^T w (device), chunks of 8 columns
+// out[0..nc) = Q[:, 0..nc)^T w and out[nc] = w . w (device), chunks of kChunk
+// columns; the w . w term rides on the last chunk.  skip: see mdot_kernel.
 static void multidot(hipStream_t s, long n, const double *w, const double *Q, long ldq, int nc, double *part,
-                     double *out, int accumulate) {
-    for (int c0 = 0; c0 < nc; c0 += 8) {
-        int k = std::min(8, nc - c0);
-        multidot8<<<kBlocks, kThreads, 0, s>>>(n, w, Q + c0 * ldq, ldq, k, part);
-        reduce_cols<<<k, kThreads, 0, s>>>(kBlocks, part, out + c0, accumulate);
+                     double *out, const double *skip) {
+    for (int c0 = 0; c0 < nc; c0 += kChunk) {
+        const int k = std::min(kChunk, nc - c0);
+        const int last = c0 + k == nc;
+        mdot_kernel<<<kBlocks, kThreads, 0, s>>>(n, w, Q + c0 * ldq, ldq, k, last, part, skip);
+        reduce_cols<<<k + last, kThreads, 0, s>>>(kBlocks, part, out + c0, 0, skip);
     }
-}
-
-static double host_norm(hipStream_t s, long n, const double *x, double *part, double *dev_out) {
-    dot_async(s, n, x, x, part, dev_out);
-    double v = 0.;
-    (void)hipMemcpyAsync(&v, dev_out, sizeof(double), hipMemcpyDeviceToHost, s);
-    (void)hipStreamSynchronize(s);
-    return std::sqrt(v);
-}
-
-// netlib drotg / drot
-static void drotg(double *da, double *db, double *c, double *s) {
-    double roe = std::fabs(*da) > std::fabs(*db) ? *da : *db;
-    double scale = std::fabs(*da) + std::fabs(*db), r, z;
-    if (scale == 0.) { *c = 1.; *s = 0.; r = 0.; z = 0.; }
-    else {
-        double a = *da / scale, b = *db / scale;
-        r = scale * std::sqrt(a * a + b * b);
-        r = (roe >= 0 ? 1. : -1.) * r;
-        *c = *da / r; *s = *db / r;
-        z = 1.;
-        if (std::fabs(*da) > std::fabs(*db)) z = *s;
-        if (std::fabs(*db) >= std::fabs(*da) && *c != 0.) z = 1. / *c;
-    }
-    *da = r; *db = z;
 }
 
 using Op = std::function<int(const double *, double *)>;
 
-// Restarted right-preconditioned GMRES (gmres.c:26-201).  Each Arnoldi step
-// stops the restart cycle early once the Givens residual estimate meets tol.
+// Restarted right-preconditioned GMRES (gmres.c:26-201) with the host kept off
+// the critical path: the Hessenberg column, the Givens rotations and the
+// residual estimate are updated on the device; each Arnoldi step's estimate is
+// copied to pinned host memory and read one step later, while the next step
+// is already queued (a cycle that has converged runs one extra, discarded
+// step).  Classical Gram-Schmidt with a second pass only when the first one
+// cancelled more than half of ||w||^2 (reorthogonalise = 1), always (2) or
+// never (0).  A cycle ends early once the estimate meets tol * ||b||.
 int gmres_device(fvp_csr *W, hipStream_t s, long n, const Op &A, const Op &M, const double *b, double *x,
-                        const fvp_gmres_params *p, fvp_gmres_stats *st) {
+                 const fvp_gmres_params *p, fvp_gmres_stats *st) {
     const int m = p->dim_krylov;
     if (m < 1 || m > 63) { set_error("dim_krylov must be 1..63"); return -1; }
+    if (p->reorthogonalise < 0 || p->reorthogonalise > 2) { set_error("reorthogonalise must be 0, 1 or 2"); return -1; }
     if (ensure_work(W, m)) return -1;
-    double *Q = W->Q.p, *w = W->w.p, *r = W->r.p, *z = W->z.p, *part = W->part.p, *sc = W->scal.p,
-           *hd = W->hdev.p;
+    double *Q = W->Q.p, *w = W->w.p, *r = W->r.p, *z = W->z.p, *part = W->part.p, *sc = W->scal.p;
+    // h1, h2: first / second pass coefficients (j+1 dots, then ||w||^2); stat: {estimate, h_{j+1,j}}
+    double *h1 = W->hdev.p, *h2 = W->hdev.p + (m + 2), *stat = W->hdev.p + 2 * (m + 2);
+    // device scalars: sc[0] ||r||^2, sc[1] ||b||^2, sc[2] first-pass ||w||^2 after projection,
+    // sc[3] second-pass ||w||^2, sc[4] reorthogonalisation flag, sc[5] 1 / h_{j+1,j}
+    double *n1 = sc + 2, *n2 = sc + 3, *flag = sc + 4, *ninv = sc + 5;
     const long ldq = n;
+    const int mode = p->reorthogonalise;
     fvp_gmres_stats stats{0, 0, 0, 0., 0.};
-    // r = b - A x
+    double hn2[2] = {0., 0.};
+    // r = b - A x; ||r||, ||b|| with one synchronisation
     if (A(x, r)) return -1;
     vrsub<<<nblk(n, 256), 256, 0, s>>>(n, b, r);
-    double beta = host_norm(s, n, r, part, sc);
-    double normb = host_norm(s, n, b, part, sc);
+    dot_async(s, n, r, r, part, sc);
+    dot_async(s, n, b, b, part, sc + 1);
+    FVP_HIP(hipMemcpyAsync(hn2, sc, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    FVP_HIP(hipStreamSynchronize(s));
+    double beta = std::sqrt(hn2[0]), normb = std::sqrt(hn2[1]);
     double error = normb > 0 ? beta / normb : 0.;
     if (beta == 0. || error < p->tol) {
         stats.converged = 1; stats.residual = beta; stats.error = error;
         if (st) *st = stats;
         return 0;
     }
-    std::vector<double> H((size_t)(m + 1) * m), g(m + 1), hcol(m + 2), hcol2(m + 2);
-    std::vector<double> cs(m), sn(m);
     int restarts = 0, converged = 0, total = 0;
-    // q0 = r / beta
-    dot_async(s, n, r, r, part, sc);
-    vscale_inv_norm<<<nblk(n, 256), 256, 0, s>>>(n, r, sc, Q);
+    vscale_inv_norm<<<nblk(n, 256), 256, 0, s>>>(n, r, sc, Q);  // q0 = r / beta
     while (restarts <= p->max_restarts && !converged) {
-        std::fill(H.begin(), H.end(), 0.);
-        std::fill(g.begin(), g.end(), 0.);
-        g[0] = beta;
-        int j = 0;
-        while (j < m) {
-            const double *qj = Q + (size_t)j * ldq;
-            if (M) { if (M(qj, z)) return -1; if (A(z, w)) return -1; }
-            else if (A(qj, w)) return -1;
-            multidot(s, n, w, Q, ldq, j + 1, part, hd, 0);
-            multiaxpy<<<nblk(n, 256), 256, 0, s>>>(n, w, Q, ldq, j + 1, hd, -1.0);
-            if (p->reorthogonalise) {  // CGS2: second sweep, coefficients added on the host
-                multidot(s, n, w, Q, ldq, j + 1, part, hd + (m + 2), 0);
-                multiaxpy<<<nblk(n, 256), 256, 0, s>>>(n, w, Q, ldq, j + 1, hd + (m + 2), -1.0);
+        gmres_cycle_init<<<1, 64, 0, s>>>(W->gd.p, m + 1, beta);
+        int k = m;
+        for (int j = 0; j <= m; ++j) {
+            if (j < m) {  // queue Arnoldi step j
+                const double *qj = Q + (size_t)j * ldq;
+                if (M) { if (M(qj, z)) return -1; if (A(z, w)) return -1; }
+                else if (A(qj, w)) return -1;
+                // first pass: h1 = Q^T w, ||w||^2 -> h1[j+1]; w -= Q h1, ||w||^2 -> n1
+                multidot(s, n, w, Q, ldq, j + 1, part, h1, nullptr);
+                axpy_norm<<<kBlocks, kThreads, 0, s>>>(n, w, Q, ldq, j + 1, h1, part, nullptr);
+                reduce_cols<<<1, kThreads, 0, s>>>(kBlocks, part, n1, 0, nullptr);
+                if (mode) {
+                    reorth_decide<<<1, 64, 0, s>>>(h1 + j + 1, n1, mode, flag);
+                    multidot(s, n, w, Q, ldq, j + 1, part, h2, flag);
+                    axpy_norm<<<kBlocks, kThreads, 0, s>>>(n, w, Q, ldq, j + 1, h2, part, flag);
+                    reduce_cols<<<1, kThreads, 0, s>>>(kBlocks, part, n2, 0, flag);
+                }
+                arnoldi_update<<<1, 64, 0, s>>>(j, m, h1, h2, n1, n2, mode ? flag : nullptr, W->Hd.p, W->csd.p,
+                                                W->snd.p, W->gd.p, stat, ninv);
+                vscale_dev<<<nblk(n, 256), 256, 0, s>>>(n, w, ninv, Q + (size_t)(j + 1) * ldq);
+                FVP_HIP(hipMemcpyAsync(W->hstat + 2 * j, stat, 2 * sizeof(double),
+                                       hipMemcpyDeviceToHost, s));
+                FVP_HIP(hipEventRecord(W->ev[j], s));
+                FVP_HIP(hipGetLastError());
             }
-            FVP_HIP(hipGetLastError());
-            dot_async(s, n, w, w, part, sc);
-            FVP_HIP(hipMemcpyAsync(hcol.data(), hd, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s));
-            if (p->reorthogonalise)
-                FVP_HIP(hipMemcpyAsync(hcol2.data(), hd + (m + 2), sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s));
-            double hn2 = 0.;
-            FVP_HIP(hipMemcpyAsync(&hn2, sc, sizeof(double), hipMemcpyDeviceToHost, s));
-            FVP_HIP(hipStreamSynchronize(s));
-            if (p->reorthogonalise)
-                for (int i = 0; i <= j; ++i) hcol[i] += hcol2[i];
-            double hn = std::sqrt(hn2);
-            for (int i = 0; i <= j; ++i) H[(size_t)j * (m + 1) + i] = hcol[i];
-            H[(size_t)j * (m + 1) + j + 1] = hn;
-            ++total;
-            if (hn < 1e-16) { ++j; break; }
-            vscale_inv_norm<<<nblk(n, 256), 256, 0, s>>>(n, w, sc, Q + (size_t)(j + 1) * ldq);
-            // Givens on the new column for the residual estimate
-            for (int i = 0; i < j; ++i) {
-                double *hc = &H[(size_t)j * (m + 1)];
-                double t1 = cs[i] * hc[i] + sn[i] * hc[i + 1];
-                hc[i + 1] = cs[i] * hc[i + 1] - sn[i] * hc[i];
-                hc[i] = t1;
+            if (j > 0) {  // read step j-1 while step j runs
+                FVP_HIP(hipEventSynchronize(W->ev[j - 1]));
+                ++total;
+                const double est = W->hstat[2 * (j - 1)], hn = W->hstat[2 * (j - 1) + 1];
+                if (est < p->tol * normb || !(hn > 1e-16)) { k = j; break; }
             }
-            {
-                double *hc = &H[(size_t)j * (m + 1)];
-                double a = hc[j], bb = hc[j + 1], c, sv;
-                drotg(&a, &bb, &c, &sv);
-                cs[j] = c; sn[j] = sv;
-                hc[j] = c * hc[j] + sv * hc[j + 1];
-                hc[j + 1] = 0.;
-                double gj = g[j];
-                g[j] = c * gj;
-                g[j + 1] = -sv * gj;
-            }
-            ++j;
-            if (std::fabs(g[j]) < p->tol * normb) break;
         }
-        // back substitution on the rotated (upper triangular) H
-        std::vector<double> y(j, 0.);
-        for (int i = j - 1; i >= 0; --i) {
-            double sum = g[i];
-            for (int k = i + 1; k < j; ++k) sum -= H[(size_t)k * (m + 1) + i] * y[k];
-            y[i] = sum / H[(size_t)i * (m + 1) + i];
-        }
-        Coef32 yc{};
-        for (int i = 0; i < j; ++i) yc.v[i] = y[i];
-        lincomb<<<nblk(n, 256), 256, 0, s>>>(n, w, Q, ldq, j, yc);
+        gmres_backsolve<<<1, 64, 0, s>>>(k, m, W->Hd.p, W->gd.p, W->yd.p);
+        lincomb<<<nblk(n, 256), 256, 0, s>>>(n, w, Q, ldq, k, W->yd.p);
         if (M) { if (M(w, z)) return -1; }
         else vcopy<<<nblk(n, 256), 256, 0, s>>>(n, w, z);
         vadd<<<nblk(n, 256), 256, 0, s>>>(n, z, x);
         if (A(x, r)) return -1;
         vrsub<<<nblk(n, 256), 256, 0, s>>>(n, b, r);
-        beta = host_norm(s, n, r, part, sc);
+        dot_async(s, n, r, r, part, sc);
+        FVP_HIP(hipMemcpyAsync(hn2, sc, sizeof(double), hipMemcpyDeviceToHost, s));
+        FVP_HIP(hipStreamSynchronize(s));
+        beta = std::sqrt(hn2[0]);
         error = beta / normb;
         if (error < p->tol) converged = 1;
         else ++restarts;
-        if (!converged && restarts <= p->max_restarts) {
-            dot_async(s, n, r, r, part, sc);
+        if (!converged && restarts <= p->max_restarts)
             vscale_inv_norm<<<nblk(n, 256), 256, 0, s>>>(n, r, sc, Q);
-        }
     }
     stats.restarts = converged ? restarts : restarts - 1;
     stats.iterations = total;
@@ -817,6 +906,9 @@ extern "C" int fvp_csr_destroy(fvp_csr *A) {
     A->ia.release(); A->ja.release(); A->a.release(); A->lu.release(); A->jac.release(); A->diag.release();
     A->lrows.release(); A->urows.release(); A->flag.release(); A->Q.release(); A->w.release(); A->r.release();
     A->z.release(); A->part.release(); A->scal.release(); A->hdev.release();
+    A->Hd.release(); A->gd.release(); A->csd.release(); A->snd.release(); A->yd.release();
+    if (A->hstat) (void)hipHostFree(A->hstat);
+    for (hipEvent_t e : A->ev) (void)hipEventDestroy(e);
     A->b_ia.release(); A->b_src.release(); A->b_diag.release(); A->b_ja.release(); A->b_lu.release(); A->b_lu32.release();
     A->b_lev_l.release(); A->b_lev_u.release(); A->b_lptr_l.release(); A->b_lptr_u.release();
     A->b_rows_l.release(); A->b_rows_u.release();

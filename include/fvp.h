@@ -146,7 +146,8 @@ int fvp_precon_values(fvp_csr *A, double *values_host);
 typedef struct {
     int dim_krylov;       /* restart length m */
     int max_restarts;
-    int reorthogonalise;  /* 1: classical Gram-Schmidt applied twice (CGS2) */
+    int reorthogonalise;  /* classical Gram-Schmidt: 0 one pass, 1 a second pass when the first
+                             cancels more than half of ||w||^2, 2 always two passes (CGS2) */
     double tol;           /* ||b - A x|| / ||b|| */
 } fvp_gmres_params;
 typedef struct {

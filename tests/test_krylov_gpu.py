@@ -89,7 +89,8 @@ def test_jacobi_and_zero_pivot(mats):
 
 @pytest.mark.parametrize("which", ["cassion", "box3d"])
 @pytest.mark.parametrize("ptype", [0, 2, 5])
-def test_gmres_known_answer(mats, which, ptype):
+@pytest.mark.parametrize("reorth", [0, 1, 2])
+def test_gmres_known_answer(mats, which, ptype, reorth):
     import torch
     from fvmpor_amd.lib import CsrMatrix
     ia, ja, a = mats[which]
@@ -99,7 +100,7 @@ def test_gmres_known_answer(mats, which, ptype):
     assert A.precon_build(ptype) == 0
     b = O.csr_gemv(ia, ja, a, np.ones(n))
     x = torch.zeros(n, dtype=torch.float64, device=device())
-    rc, st = A.gmres(_dev(torch, b), x, m=30, max_restarts=40, tol=1e-12, reorth=1)
+    rc, st = A.gmres(_dev(torch, b), x, m=30, max_restarts=40, tol=1e-12, reorth=reorth)
     assert rc == 0 and st["converged"], st
     assert np.abs(x.cpu().numpy() - 1.0).max() < 1e-9
     xo, sto = O.gmres_csr(ia, ja, a, b, precon=ptype, m=10, max_restarts=100, tol=1e-12)
@@ -236,3 +237,23 @@ def test_block_rows_rejects_invalid_sizes(box_mid):
     for bad in (100, 32, 8192):
         with pytest.raises(FvpError):
             A.precon_block_rows(bad)
+
+
+def test_gmres_full_krylov_space_is_an_exact_solve():
+    """m >= n: the Krylov space fills up (h_{n,n-1} -> 0, the lucky breakdown)
+    and GMRES returns the exact solution of a small nonsymmetric system."""
+    import torch
+    from fvmpor_amd.lib import CsrMatrix
+    rng = np.random.default_rng(11)
+    n = 40
+    D = rng.normal(size=(n, n)) * (rng.random((n, n)) < 0.3) + np.diag(rng.uniform(2, 4, n))
+    S = sp.csr_matrix(D)
+    S.sort_indices()
+    A = CsrMatrix(S.indptr.astype(np.int32), S.indices.astype(np.int32))
+    A.set_values(_dev(torch, S.data))
+    assert A.precon_build(0) == 0
+    b = rng.normal(size=n)
+    x = torch.zeros(n, dtype=torch.float64, device=device())
+    rc, st = A.gmres(_dev(torch, b), x, m=50, max_restarts=0, tol=1e-13, reorth=1)
+    assert rc == 0 and st["converged"] and st["iterations"] <= n, st
+    np.testing.assert_allclose(x.cpu().numpy(), np.linalg.solve(D, b), rtol=1e-10, atol=1e-11)
