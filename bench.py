@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--precon", default="bjilu0", choices=["bjilu0", "bjilu0_f32", "ilu0", "jacobi"])
     ap.add_argument("--block-rows", type=int, default=0, help="block-Jacobi rows per block (0 = library default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
+    ap.add_argument("--no-fp32", action="store_true",
+                    help="skip the FP32-preconditioner run over the same simulated window")
     ap.add_argument("--cpu-nx", type=int, default=80, help="lattice size of the CPU sample box")
     return ap.parse_args()
 
@@ -85,6 +87,55 @@ def cpu_baseline(args, gpu_window_days, gpu_window_residuals, n_nodes_gpu):
                    f"generator ({per_eval*1e3:.1f} ms each), scaled by node count to {per_eval_full:.2f} s per "
                    f"1M-node residual and by the {gpu_window_residuals} residual evaluations the GPU run "
                    f"needed for the timed steps; preconditioner and Krylov vector work not counted"),
+    }
+
+
+def fp32_window(args, prob, ctx, psi0, st0, st1, wall64, psi64):
+    """The FP32-preconditioner variant over the same simulated window: a second
+    integrator on the same context from t = 0 takes the same W untimed warm-up
+    steps, then timed steps with the FP64 run's end time as stop time, so both
+    runs end at the same t.  Reports its rate, the GMRES-iteration delta, the
+    final-head difference to the FP64 run and both runs' mass-balance errors."""
+    import torch
+    from fvmpor_amd.diagnostics import inflow_rate, total_mass
+    from fvmpor_amd.lib import Integrator, PRECON_BJILU0_F32
+    tb = st1["t"]
+    integ = Integrator(ctx, psi0, rtol=args.rtol, atol=args.atol, max_order=3, precon=PRECON_BJILU0_F32,
+                       block_rows=args.block_rows)
+    for _ in range(args.warmup):
+        integ.step(1.0)
+    torch.cuda.synchronize()
+    s0 = integ.stats()
+    w0 = time.perf_counter()
+    while integ.stats()["t"] < tb:
+        integ.step(tb)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    s1 = integ.stats()
+    out = torch.empty_like(psi0)
+    integ.advance(tb, out)
+    torch.cuda.synchronize()
+    psi32 = out.cpu().numpy()
+    ta = s0["t"]
+    m0, q = total_mass(prob, prob.psi0), inflow_rate(prob)
+
+    def mb(psi):
+        return (total_mass(prob, psi) - m0 - q * tb) / (q * tb)
+
+    d = lambda k: s1[k] - s0[k]
+    return {
+        "value": (tb - ta) / wall,
+        "unit": "sim-days/wall-s",
+        "steps": d("steps"),
+        "ms_per_step": wall / max(d("steps"), 1) * 1e3,
+        "speedup_vs_fp64": wall64 / wall,
+        "gmres_iters": d("gmres_iters"),
+        "gmres_iters_fp64": st1["gmres_iters"] - st0["gmres_iters"],
+        "newton_iters": d("newton_iters"),
+        "head_rel_err_vs_fp64": float(np.abs(psi32 - psi64).max() / np.abs(psi64).max()),
+        "mass_balance_err": float(mb(psi32)),
+        "mass_balance_err_fp64": float(mb(psi64)),
+        "t_window": [ta, tb],
     }
 
 
@@ -134,6 +185,13 @@ def main():
     rs_ms, rs_n = ctx.profile_get(1)
     pc_ms, pc_n = ctx.profile_get(2)
     ctx.profile(False)
+    out64 = torch.empty_like(psi0)
+    integ.advance(st1["t"], out64)
+    torch.cuda.synchronize()
+    psi_ref = out64.cpu().numpy()
+    fp32 = None
+    if not args.no_fp32 and args.precon == "bjilu0":
+        fp32 = fp32_window(args, prob, ctx, psi0, st0, st1, wall, psi_ref)
     days = st1["t"] - st0["t"]
     N, T, NE = ctx.N, ctx.T, ctx.NE
     bf = 48 * N + 28 * T + NE
@@ -184,6 +242,8 @@ def main():
             "launches": ek_n,
         },
     }
+    if fp32 is not None:
+        out["fp32_precon"] = fp32
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(args, days, window_res, N)
