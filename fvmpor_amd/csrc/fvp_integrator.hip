@@ -46,6 +46,11 @@ struct fvp_integrator {
     bool need_setup = true;
     long nres0 = 0;
     fvp_integrator_stats st{};
+    // independent instances stored back to back (an ensemble): norms are the
+    // maximum over instances of the per-instance weighted RMS
+    int nseg = 0, seg_min = 0;
+    DevBuf<int> seg;
+    DevBuf<double> seg_sum;
 };
 
 namespace fvp {
@@ -98,14 +103,54 @@ __global__ void zero_rows(int nd, const int *__restrict__ rows, double *__restri
 
 static inline int nb(long n) { return (int)((n + 255) / 256); }
 
-static double wrms(fvp_integrator *I, const double *v) {
+// sum over each instance's nodes of (v ewt)^2: one workgroup per instance
+__global__ void __launch_bounds__(256) seg_sq(const int *__restrict__ seg, const double *__restrict__ v,
+                                              const double *__restrict__ ewt, double *__restrict__ out) {
+    __shared__ double lds[4];
+    const int g = blockIdx.x;
+    double acc = 0.;
+    for (int i = seg[g] + threadIdx.x; i < seg[g + 1]; i += blockDim.x) {
+        const double x = v[i] * ewt[i];
+        acc += x * x;
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) out[g] = lds[0] + lds[1] + lds[2] + lds[3];
+}
+
+// out[0] = max over instances of sqrt(sum / size), out[1] = total sum
+__global__ void seg_fold(int nseg, const int *__restrict__ seg, const double *__restrict__ sums,
+                         double *__restrict__ out) {
+    if (threadIdx.x != 0) return;
+    double mx = 0., tot = 0.;
+    for (int g = 0; g < nseg; ++g) {
+        mx = fmax(mx, sqrt(sums[g] / (seg[g + 1] - seg[g])));
+        tot += sums[g];
+    }
+    out[0] = mx;
+    out[1] = tot;
+}
+
+// Weighted RMS norm ||v||_WRMS with the error weights (the maximum over the
+// instances of an ensemble); l2 (optional) gets the plain 2-norm of v * ewt.
+static double wrms(fvp_integrator *I, const double *v, double *l2 = nullptr) {
     hipStream_t s = I->c->stream;
+    double r[2] = {0., 0.};
+    if (I->nseg > 1) {
+        seg_sq<<<I->nseg, 256, 0, s>>>(I->seg.p, v, I->ewt.p, I->seg_sum.p);
+        seg_fold<<<1, 64, 0, s>>>(I->nseg, I->seg.p, I->seg_sum.p, I->scal.p);
+        (void)hipMemcpyAsync(r, I->scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
+        (void)hipStreamSynchronize(s);
+        if (l2) *l2 = std::sqrt(r[1]);
+        return r[0];
+    }
     mul_kernel<<<nb(I->N), 256, 0, s>>>(I->N, v, I->ewt.p, I->tmp.p);
     dot_async(s, I->N, I->tmp.p, I->tmp.p, I->part.p, I->scal.p);
-    double r = 0.;
-    (void)hipMemcpyAsync(&r, I->scal.p, sizeof(double), hipMemcpyDeviceToHost, s);
+    (void)hipMemcpyAsync(r, I->scal.p, sizeof(double), hipMemcpyDeviceToHost, s);
     (void)hipStreamSynchronize(s);
-    return std::sqrt(r / I->N);
+    if (l2) *l2 = std::sqrt(r[0]);
+    return std::sqrt(r[0] / I->N);
 }
 
 static void lincomb(fvp_integrator *I, int m, const double *c, const double *const *x, double *out) {
@@ -202,7 +247,9 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
     }
     fvp_csr *W = fvp::jacobian_handle(c);
     if (!W) return -1;
-    const double tol_abs = I->p.lin_factor * I->p.newton_tol * std::sqrt((double)N);
+    // linear tolerance in the scaled 2-norm; for an ensemble the smallest
+    // instance's share, so that every instance meets its own tolerance
+    const double tol_abs = I->p.lin_factor * I->p.newton_tol * std::sqrt((double)(I->nseg > 1 ? I->seg_min : N));
     double oldnrm = 0.;
     bool conv = false;
     for (int m = 0; m < I->p.max_newton; ++m) {
@@ -214,7 +261,8 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
         mul_kernel<<<nb(N), 256, 0, s>>>(N, I->G.p, I->ewt.p, I->b.p);
         axpby_kernel<<<nb(N), 256, 0, s>>>(N, 0., I->b.p, -1., I->b.p);
         FVP_HIP(hipMemsetAsync(I->delta.p, 0, sizeof(double) * N, s));
-        double bn = wrms(I, I->G.p) * std::sqrt((double)N);
+        double bn = 0.;
+        (void)wrms(I, I->G.p, &bn);
         fvp_gmres_params gp{I->p.dim_krylov, I->p.max_restarts, 1, bn > 0 ? tol_abs / bn : 1.};
         fvp_gmres_stats gs{};
         const double *yv = I->y.p, *ypv = I->ypn.p, *Gv = I->G.p, *ew = I->ewt.p;
@@ -398,6 +446,7 @@ extern "C" int fvp_integrator_destroy(fvp_integrator *I) {
     I->yp.release(); I->ewt.release(); I->y.release(); I->ypn.release(); I->ypred.release(); I->ypred2.release();
     I->G.release(); I->b.release(); I->delta.release(); I->beta.release(); I->tmp.release(); I->part.release();
     I->scal.release();
+    I->seg.release(); I->seg_sum.release();
     delete I;
     return 0;
 }
@@ -432,6 +481,22 @@ extern "C" int fvp_integrator_advance(fvp_integrator *I, double t_out, double *p
     }
     if (psi_out)
         FVP_HIP(hipMemcpyAsync(psi_out, H(I, 0), sizeof(double) * I->N, hipMemcpyDeviceToDevice, I->c->stream));
+    return 0;
+}
+
+extern "C" int fvp_integrator_set_segments(fvp_integrator *I, int nseg, const int *offsets) {
+    if (!I || (nseg > 0 && !offsets)) { set_error("null argument"); return -1; }
+    if (nseg <= 1) { I->nseg = 0; return 0; }
+    if (offsets[0] != 0 || offsets[nseg] != I->N) { set_error("segments must start at 0 and end at n_nodes"); return -1; }
+    int mn = I->N;
+    for (int g = 0; g < nseg; ++g) {
+        if (offsets[g + 1] <= offsets[g]) { set_error("segment offsets must increase"); return -1; }
+        mn = std::min(mn, offsets[g + 1] - offsets[g]);
+    }
+    FVP_HIP(hipSetDevice(I->c->device));
+    if (I->seg.upload(offsets, nseg + 1) || I->seg_sum.alloc(nseg)) return -1;
+    I->nseg = nseg;
+    I->seg_min = mn;
     return 0;
 }
 

@@ -14,7 +14,7 @@ namespace fvp {
 void set_error(const std::string &msg);
 
 constexpr int kMaxZones = 16;
-constexpr int kMaxTags = 64;
+constexpr int kMaxTags = 1 << 16;
 constexpr int kTileNodes = 512;  // nodes owned by one workgroup of the tiled residual kernel
 
 // Zone parameters passed by value to kernels: K_xx, K_yy, K_zz, phi, alpha, n, m, S_r, S_s.
@@ -65,8 +65,10 @@ struct fvp_ctx {
     fvp::DevBuf<int> bnode;
     fvp::DevBuf<double> bval;
     std::vector<int> bnode_h;
-    std::vector<double> barea_h;   // nb x ntags summed CV-face area per tag
-    std::vector<double> bnrm_h;    // nb x ntags x 3 summed area^2 * unit normal per tag
+    // per boundary node, the tags of its boundary CV faces (ascending) with their
+    // summed area and summed area^2 * unit normal: CSR over the node slots
+    std::vector<int> bent_ptr, bent_tag;
+    std::vector<double> bent_area, bent_nrm;
     // Dirichlet rows
     int ndir = 0;
     fvp::DevBuf<int> dnode;

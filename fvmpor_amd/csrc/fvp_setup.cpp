@@ -9,6 +9,7 @@
 #include <cstring>
 #include <mutex>
 #include <numeric>
+#include <unordered_map>
 
 #include "fvp_internal.h"
 
@@ -69,15 +70,16 @@ int upload_bc_values(fvp_ctx *c) {
     std::vector<double> val(c->nb, 0.0);
     for (int k = 0; k < c->nb; ++k) {
         double s = 0.0;
-        for (int tag = 0; tag < c->ntags; ++tag) {
-            const double *b = &c->bc[tag * FVP_BC_FIELDS];
-            double a = c->barea_h[(size_t)k * c->ntags + tag];
+        for (int q = c->bent_ptr[k]; q < c->bent_ptr[k + 1]; ++q) {
+            const int tag = c->bent_tag[q];
+            const double *b = &c->bc[(size_t)tag * FVP_BC_FIELDS];
+            const double a = c->bent_area[q];
             if (a == 0.0) continue;
             switch (c->bc_type[tag]) {
             case 3:
             case 7: s += b[0] * a; break;
             case 6: {
-                const double *nv = &c->bnrm_h[((size_t)k * c->ntags + tag) * 3];
+                const double *nv = &c->bent_nrm[3 * (size_t)q];
                 s += nv[0] * b[3] + nv[1] * b[4] + nv[2] * b[5];
                 break;
             }
@@ -106,7 +108,7 @@ static int validate(const fvp_problem *p) {
         set_error("missing mesh arrays"); return -1;
     }
     if (p->n_zones <= 0 || p->n_zones > kMaxZones) { set_error("n_zones must be 1..16"); return -1; }
-    if (p->n_tags < 0 || p->n_tags > kMaxTags) { set_error("n_tags must be 0..64"); return -1; }
+    if (p->n_tags < 0 || p->n_tags > kMaxTags) { set_error("n_tags must be 0..65536"); return -1; }
     int npe = p->dim + 1;
     for (long i = 0; i < (long)p->n_elements * npe; ++i)
         if (p->conn[i] < 0 || p->conn[i] >= p->n_nodes) { set_error("connectivity out of range"); return -1; }
@@ -286,6 +288,9 @@ extern "C" int fvp_create(const fvp_problem *p, int device, fvp_ctx **out) {
     // ---- boundary CV faces aggregated per (node, tag)
     {
         std::vector<int> slot_of(N, -1);
+        std::unordered_map<long, int> ent_of;  // (slot, tag) -> entry
+        std::vector<int> e_slot, e_tag;
+        std::vector<double> e_area, e_nrm;
         const int(*ftab)[3] = dim == 2 ? TRI_FACE : TET_FACE;
         const int nfe = npe, nfn = dim;
         for (int e = 0; e < T; ++e)
@@ -325,33 +330,53 @@ extern "C" int fvp_create(const fvp_problem *p, int device, fvp_ctx **out) {
                     if (slot_of[i] < 0) {
                         slot_of[i] = c->nb++;
                         c->bnode_h.push_back(i);
-                        c->barea_h.resize((size_t)c->nb * c->ntags, 0.0);
-                        c->bnrm_h.resize((size_t)c->nb * c->ntags * 3, 0.0);
                     }
-                    size_t s = (size_t)slot_of[i] * c->ntags + tag;
-                    c->barea_h[s] += Ai;
-                    c->bnrm_h[3 * s] += Ai * Ai * nx;
-                    c->bnrm_h[3 * s + 1] += Ai * Ai * ny;
-                    c->bnrm_h[3 * s + 2] += Ai * Ai * nzv;
+                    const long key = (long)slot_of[i] * c->ntags + tag;
+                    auto it = ent_of.find(key);
+                    int q;
+                    if (it == ent_of.end()) {
+                        q = (int)e_slot.size();
+                        ent_of.emplace(key, q);
+                        e_slot.push_back(slot_of[i]);
+                        e_tag.push_back(tag);
+                        e_area.push_back(0.0);
+                        e_nrm.insert(e_nrm.end(), {0.0, 0.0, 0.0});
+                    } else q = it->second;
+                    e_area[q] += Ai;
+                    e_nrm[3 * (size_t)q] += Ai * Ai * nx;
+                    e_nrm[3 * (size_t)q + 1] += Ai * Ai * ny;
+                    e_nrm[3 * (size_t)q + 2] += Ai * Ai * nzv;
                 }
             }
         // boundary slots in ascending node order (per-tile ranges below)
         std::vector<int> ord(c->nb);
         std::iota(ord.begin(), ord.end(), 0);
         std::sort(ord.begin(), ord.end(), [&](int a, int b) { return c->bnode_h[a] < c->bnode_h[b]; });
-        std::vector<int> bn(c->nb);
-        std::vector<double> ba((size_t)c->nb * c->ntags), bv((size_t)c->nb * c->ntags * 3);
+        std::vector<int> bn(c->nb), rank(c->nb);
         for (int k = 0; k < c->nb; ++k) {
             bn[k] = c->bnode_h[ord[k]];
-            for (int tg = 0; tg < c->ntags; ++tg) {
-                ba[(size_t)k * c->ntags + tg] = c->barea_h[(size_t)ord[k] * c->ntags + tg];
-                for (int d = 0; d < 3; ++d)
-                    bv[((size_t)k * c->ntags + tg) * 3 + d] = c->bnrm_h[((size_t)ord[k] * c->ntags + tg) * 3 + d];
-            }
+            rank[ord[k]] = k;
         }
         c->bnode_h.swap(bn);
-        c->barea_h.swap(ba);
-        c->bnrm_h.swap(bv);
+        // entries grouped by sorted slot, ascending tag within a slot
+        std::vector<int> eo(e_slot.size());
+        std::iota(eo.begin(), eo.end(), 0);
+        std::sort(eo.begin(), eo.end(), [&](int a, int b) {
+            const int ra = rank[e_slot[a]], rb = rank[e_slot[b]];
+            return ra != rb ? ra < rb : e_tag[a] < e_tag[b];
+        });
+        c->bent_ptr.assign(c->nb + 1, 0);
+        c->bent_tag.resize(eo.size());
+        c->bent_area.resize(eo.size());
+        c->bent_nrm.resize(3 * eo.size());
+        for (size_t q = 0; q < eo.size(); ++q) {
+            const int src = eo[q];
+            c->bent_ptr[rank[e_slot[src]] + 1]++;
+            c->bent_tag[q] = e_tag[src];
+            c->bent_area[q] = e_area[src];
+            for (int d = 0; d < 3; ++d) c->bent_nrm[3 * q + d] = e_nrm[3 * (size_t)src + d];
+        }
+        for (int k = 0; k < c->nb; ++k) c->bent_ptr[k + 1] += c->bent_ptr[k];
         if (c->bnode.upload(c->bnode_h.data(), c->nb)) return fail();
         if (upload_bc_values(c)) return fail();
     }

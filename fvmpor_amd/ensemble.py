@@ -1,0 +1,166 @@
+"""Batched Monte Carlo ensembles of independent Richards problems.
+
+An ensemble is many independent realisations (own mesh, own log-normal K
+field, own infiltration rate).  On one GPU they are advanced together as one
+block-diagonal problem: the instances' nodes and elements are stored back to
+back in one context, so every residual, J v, Gram-Schmidt pass, colouring
+sweep and preconditioner launch covers all instances at once (one launch per
+kernel per ensemble instead of one per instance), node tiles and
+block-Jacobi blocks never couple different instances (there are no entries
+between them), and the integrator measures every error and Newton norm per
+instance (the maximum over instances of each one's weighted RMS,
+fvp_integrator_set_segments) while all instances share t, h and the BDF order.
+
+The reference runs one problem per process; this module has no counterpart in
+it.  Instances of one ensemble must share the spatial dimension (the residual
+kernel is specialised per dimension); a mixed 2D/3D draw runs as two batches.
+"""
+from __future__ import annotations
+
+import dataclasses
+import time
+
+import numpy as np
+
+from .meshgen import Mesh, box_mesh_2d, gaussian_field
+from .problems import SOIL_CLASSES, Problem, _zone_row, box3d_problem
+
+
+@dataclasses.dataclass
+class Draw:
+    index: int
+    dim: int
+    n_nodes: int
+    sigma_lnK: float
+    rain: float
+    seed: int
+
+
+def draw_instances(n=256, seed=4242, n_min=5000, n_max=60000):
+    """Per-instance parameters: node count ~ U[n_min, n_max], sigma_lnK ~
+    U[0.5, 2.0], infiltration ~ U[0.01, 0.1] m/day, jitter seed 1000 + i,
+    2D for even seeds and 3D for odd ones."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        s = 1000 + i
+        out.append(Draw(i, 2 if s % 2 == 0 else 3, int(rng.integers(n_min, n_max + 1)),
+                        float(rng.uniform(0.5, 2.0)), float(rng.uniform(0.01, 0.1)), s))
+    return out
+
+
+def slice_instance(d: Draw) -> Problem:
+    """2D slice 10 m x 5 m of ~n_nodes nodes (jittered lattice, Delaunay),
+    homogeneous loam with a log-normal K field (mean ln 0.25 m/day, 2 m / 0.5 m
+    correlation), psi0 = 1 - y, infiltration on the strip x in [4, 6] m."""
+    nx = max(5, int(round(np.sqrt(2.0 * d.n_nodes))))
+    ny = max(3, int(round(d.n_nodes / nx)))
+    mesh = box_mesh_2d(nx, ny, 10.0, 5.0, seed=d.seed)
+    spacing = (10.0 / (nx - 1), 5.0 / (ny - 1))
+    field = gaussian_field((nx, ny), spacing, (2.0, 0.5), d.sigma_lnK, d.seed + 1) + np.log(0.25)
+    cen = mesh.xyz[mesh.conn].mean(axis=1)
+    idx = [np.clip(np.rint(cen[:, a] / spacing[a]).astype(np.int64), 0, (nx, ny)[a] - 1) for a in range(2)]
+    alpha, n, tr, ts = SOIL_CLASSES["loam"]
+    zones = np.array([_zone_row(1.0, ts, alpha, n, tr / ts, 1e-5)])
+    bc_type = np.array([0, 3, 3], dtype=np.int32)
+    bc = np.zeros((3, 6))
+    bc[2, 0] = -d.rain
+    return Problem(f"slice{d.index}", mesh, zones, np.zeros(mesh.n_elements, dtype=np.int32),
+                   np.exp(field[idx[0], idx[1]]), bc_type, bc, 1000.0, 1.0 - mesh.xyz[:, 1], 0.5, "day")
+
+
+def box_instance(d: Draw) -> Problem:
+    """3D box of the 1M-node workload's statistics at ~n_nodes nodes
+    (4 nz x 4 nz x nz lattice)."""
+    nz = max(4, int(round((d.n_nodes / 16.0) ** (1.0 / 3.0))))
+    p = box3d_problem(nx=4 * nz, ny=4 * nz, nz=nz, seed_mesh=d.seed, seed_field=d.seed + 1,
+                      seed_layers=d.seed + 2, sigma_lnK=d.sigma_lnK, rain=d.rain)
+    p.name = f"box{d.index}"
+    p.t_end = 0.5
+    return p
+
+
+def make_instance(d: Draw) -> Problem:
+    return slice_instance(d) if d.dim == 2 else box_instance(d)
+
+
+def union_problem(problems):
+    """One problem holding the instances back to back.  Tag 1 (no flow) is
+    shared, each instance's tag 2 (its infiltration face) becomes tag 2 + i;
+    zone tables are merged by row.  Returns (problem, node offsets)."""
+    dims = {p.mesh.dim for p in problems}
+    if len(dims) != 1:
+        raise ValueError("an ensemble batch needs instances of one dimension")
+    dim = dims.pop()
+    rho0 = problems[0].rho0
+    if any(p.rho0 != rho0 for p in problems):
+        raise ValueError("instances must share rho0")
+    zone_rows, ezone, xyz, conn, ftags, nptr, ntags, zfield, ekmul, psi0 = [], [], [], [], [], [], [], [], [], []
+    offsets = [0]
+    bc_type = [0, 3]
+    bc = [np.zeros(6), np.zeros(6)]
+    for i, p in enumerate(problems):
+        if not (np.array_equal(p.bc_type, [0, 3, 3]) and p.bc[1, 0] == 0.0):
+            raise ValueError("instances must have tags 0 (interior), 1 (no flow), 2 (prescribed flux)")
+        zmap = []
+        for row in p.zones:
+            for k, r in enumerate(zone_rows):
+                if np.array_equal(r, row):
+                    zmap.append(k)
+                    break
+            else:
+                zone_rows.append(row.copy())
+                zmap.append(len(zone_rows) - 1)
+        off = offsets[-1]
+        m = p.mesh
+        ezone.append(np.asarray(zmap, dtype=np.int32)[p.ezone])
+        xyz.append(m.xyz)
+        conn.append(m.conn + off)
+        ft = m.face_tags.copy()
+        ft[ft == 2] = 2 + i
+        ftags.append(ft)
+        nt = m.node_tags.copy()
+        nt[nt == 2] = 2 + i
+        ntags.append(nt)
+        nptr.append(m.node_tag_ptr[1:] + (nptr[-1][-1] if nptr else 0))
+        zfield.append(m.zone)
+        ekmul.append(p.ekmul)
+        psi0.append(p.psi0)
+        bc_type.append(3)
+        bc.append(p.bc[2].copy())
+        offsets.append(off + m.n_nodes)
+    if len(zone_rows) > 16:
+        raise ValueError("more than 16 distinct zones")
+    mesh = Mesh(dim, np.vstack(xyz), np.vstack(conn).astype(np.int32), np.vstack(ftags).astype(np.int32),
+                np.concatenate([[0]] + nptr).astype(np.int32), np.concatenate(ntags).astype(np.int32),
+                np.concatenate(zfield).astype(np.int32))
+    prob = Problem("ensemble", mesh, np.array(zone_rows), np.concatenate(ezone), np.concatenate(ekmul),
+                   np.array(bc_type, dtype=np.int32), np.array(bc), rho0, np.concatenate(psi0),
+                   problems[0].t_end, problems[0].time_unit)
+    return prob, np.array(offsets, dtype=np.int64)
+
+
+def run_batch(problems, t_end=0.5, rtol=1e-6, atol=1e-6, precon=None, block_rows=0, stream=None):
+    """Advance one batch (instances of one dimension) to t_end on cuda:0.
+    Returns per-instance final heads, the integrator statistics and the wall
+    time of the advance (set-up excluded)."""
+    import torch
+
+    from .lib import PRECON_BJILU0_F32, Context, Integrator
+    union, offsets = union_problem(problems)
+    ctx = Context(union)
+    if stream is not None:
+        ctx.set_stream(stream)
+    psi0 = torch.tensor(union.psi0, device="cuda:0")
+    integ = Integrator(ctx, psi0, rtol=rtol, atol=atol, max_order=3,
+                       precon=PRECON_BJILU0_F32 if precon is None else precon, block_rows=block_rows)
+    integ.set_segments(offsets)
+    out = torch.empty_like(psi0)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    integ.advance(t_end, out)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    psi = out.cpu().numpy()
+    heads = [psi[offsets[i]:offsets[i + 1]] for i in range(len(problems))]
+    return heads, integ.stats(), wall

@@ -50,6 +50,7 @@ SIGNATURES = {
     "fvp_precon_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fvp_precon_block_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fvp_set_block_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "fvp_integrator_set_segments": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_int_p]),
     "fvp_precon_apply": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fvp_precon_values": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p]),
     "fvp_gmres_csr": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -376,6 +377,11 @@ class Integrator:
 
     def __del__(self):
         self.close()
+
+    def set_segments(self, offsets):
+        off = np.ascontiguousarray(offsets, dtype=np.int32)
+        _check(lib().fvp_integrator_set_segments(self.h, len(off) - 1, off.ctypes.data_as(_c_int_p)),
+               "fvp_integrator_set_segments")
 
     def step(self, t_stop) -> int:
         return _check(lib().fvp_integrator_step(self.h, float(t_stop)), "fvp_integrator_step")

@@ -226,6 +226,14 @@ int fvp_integrator_destroy(fvp_integrator *I);
 int fvp_integrator_step(fvp_integrator *I, double t_stop);
 /* Step until t_out (Solver::advance(t), solver.h:176-180); psi(t_out) into psi_out (device). */
 int fvp_integrator_advance(fvp_integrator *I, double t_out, double *psi_out);
+/* Ensemble of independent instances stored back to back (node ranges
+ * [offsets[g], offsets[g+1]), offsets[0] = 0, offsets[nseg] = n_nodes): the
+ * error test and the Newton test use the maximum over instances of each
+ * instance's weighted RMS norm, and the linear tolerance the smallest
+ * instance's share, so every instance meets its own tolerances while all of
+ * them share t, h and the order.  No counterpart in the reference (one
+ * problem per run); nseg <= 1 restores the single-problem norms. */
+int fvp_integrator_set_segments(fvp_integrator *I, int nseg, const int *offsets);
 /* current state (device pointers owned by the integrator) */
 int fvp_integrator_state(fvp_integrator *I, double *t, const double **psi, const double **psip);
 int fvp_integrator_get_stats(const fvp_integrator *I, fvp_integrator_stats *st);
