@@ -15,7 +15,7 @@ void set_error(const std::string &msg);
 
 constexpr int kMaxZones = 16;
 constexpr int kMaxTags = 1 << 16;
-constexpr int kTileNodes = 512;  // nodes owned by one workgroup of the tiled residual kernel
+constexpr int kTileNodes = 512;  // default nodes per tile of the tiled residual kernel
 
 // Zone parameters passed by value to kernels: K_xx, K_yy, K_zz, phi, alpha, n, m, S_r, S_s.
 struct ZoneTable {
@@ -74,7 +74,8 @@ struct fvp_ctx {
     fvp::DevBuf<int> dnode;
     fvp::DevBuf<double> dval;
     // node tiles of the residual kernel (see fvp_setup.cpp)
-    int ntiles = 0;
+    int ntiles = 0, tile_nodes = fvp::kTileNodes, xcd_map = 1;  // tile size, XCD-grouped tile order
+    int rvariant = FVP_RESIDUAL_OWNER;  // residual kernel variant
     long tile_elems = 0;            // element visits per residual (primary + halo)
     fvp::DevBuf<int> t_eptr, t_hptr, t_hlist, t_bptr, t_dptr;
     fvp::DevBuf<int> t_wptr, t_whalo, t_vptr, t_tz;  // window halo nodes, visits, zones per tile
@@ -112,6 +113,7 @@ struct JvArgs {
     const double *v = nullptr, *sig = nullptr, *F0 = nullptr;
     double cj = 0.;
 };
+int build_tiles(fvp_ctx *c, int tile_nodes);
 int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, double *F, int averaging,
                   const JvArgs *jv = nullptr);
 int tile_kernel_attrs(fvp_ctx *c);

@@ -740,20 +740,11 @@ static int build_bj(fvp_csr *A) {
     A->b_lds = bj_lds(maxnz, sizeof(double), BS);
     if (hipDeviceGetAttribute(&A->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, A->device) != hipSuccess)
         A->lds_max = 64 * 1024;
-    // the solve stages the right-hand side as well: it is the larger of the two
-    const size_t need64 = A->b_lds + sizeof(double) * BS, need32 = bj_lds(maxnz, sizeof(float), BS) + sizeof(float) * BS;
-    if (need64 <= (size_t)A->lds_max) {
-        (void)hipFuncSetAttribute((const void *)bj_factor<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)A->b_lds);
-        (void)hipFuncSetAttribute((const void *)bj_solve<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)need64);
-    }
-    if (need32 <= (size_t)A->lds_max) {
-        (void)hipFuncSetAttribute((const void *)bj_factor<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)bj_lds(maxnz, sizeof(float), BS));
-        (void)hipFuncSetAttribute((const void *)bj_solve<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)need32);
-    }
+    // per-kernel attributes (shared by every matrix): allow the device maximum;
+    // each build checks its own block against it
+    for (const void *f : {(const void *)bj_factor<double>, (const void *)bj_solve<double>,
+                          (const void *)bj_factor<float>, (const void *)bj_solve<float>})
+        (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, A->lds_max);
     A->nblk = nb;
     A->bj = true;
     return 0;

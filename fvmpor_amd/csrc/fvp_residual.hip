@@ -89,12 +89,14 @@ __device__ constexpr int edge_b(int j) { return j < 3 ? (j + 1) % 3 : 3; }
 // MODE 0: upwind k_r from the element's mask bits; MODE 1: averaged k_r
 // (0.5 front + 0.5 back) and the averaged face fluxes are also written per
 // (element, local edge) in back->front orientation for the edge-flux sign test.
-template <int DIM, int MODE>
-__device__ __forceinline__ void element_fluxes(int e, int N, const ZoneTable &zt, double rho0,
+// zk: per-zone K_xx, K_yy, K_zz in LDS.  JV: the state is psi + sig v.
+template <int DIM, int MODE, bool JV = false>
+__device__ __forceinline__ void element_fluxes(int e, int N, const double *zk, double rho0,
                                                const int4 *__restrict__ conn, const uint8_t *__restrict__ ezone,
                                                const double *__restrict__ kmul, const uint8_t *__restrict__ emask,
                                                const uint8_t *__restrict__ eorient,
                                                const double *__restrict__ xyz, const double *__restrict__ psi,
+                                               const double *__restrict__ v, double sig,
                                                const double *__restrict__ kr, double *__restrict__ qface,
                                                double *__restrict__ aface, int n[4], double f[4]) {
     constexpr int NPE = DIM + 1;
@@ -105,10 +107,9 @@ __device__ __forceinline__ void element_fluxes(int e, int N, const ZoneTable &zt
 #pragma unroll
     for (int k = 0; k < NPE; ++k) {
         load_pt(xyz, n[k], px[k], py[k], pz[k]);
-        h[k] = psi[n[k]];
+        h[k] = JV ? psi[n[k]] + sig * v[n[k]] : psi[n[k]];
     }
     const int z = ezone[e];
-    const double *zp = zt.p[z];
     const double km = kmul[e];
     double G[NPE][3];  // adjugate rows: grad N_k * det
     double det;
@@ -139,10 +140,10 @@ __device__ __forceinline__ void element_fluxes(int e, int N, const ZoneTable &zt
     }
     gx *= inv; gy *= inv; gz *= inv;
     // Darcy flux without k_r: q = -K (grad psi + e_gravity), gravity on y (2D) / z (3D)
-    double qx = gx * -(zp[ZK_X] * km);
+    double qx = gx * -(zk[3 * z] * km);
     double qy, qz;
-    if (DIM == 2) { qy = (gy + 1.) * -(zp[ZK_Y] * km); qz = 0.; }
-    else { qy = gy * -(zp[ZK_Y] * km); qz = (gz + 1.) * -(zp[ZK_Z] * km); }
+    if (DIM == 2) { qy = (gy + 1.) * -(zk[3 * z + 1] * km); qz = 0.; }
+    else { qy = gy * -(zk[3 * z + 1] * km); qz = (gz + 1.) * -(zk[3 * z + 2] * km); }
     double g[NPE];
 #pragma unroll
     for (int k = 0; k < NPE; ++k) g[k] = G[k][0] * qx + G[k][1] * qy + G[k][2] * qz;
@@ -187,16 +188,60 @@ __global__ void __launch_bounds__(256) elem_kernel(int T, int N, ZoneTable zt, d
                                                    const double *__restrict__ xyz, const double *__restrict__ psi,
                                                    const double *__restrict__ kr, double *__restrict__ acc,
                                                    double *__restrict__ qface, double *__restrict__ aface) {
+    __shared__ double zk[3 * kMaxZones];
+    if (threadIdx.x < 3 * zt.nz) zk[threadIdx.x] = zt.p[threadIdx.x / 3][ZK_X + threadIdx.x % 3];
+    __syncthreads();
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= T) return;
     int n[4];
     double f[4];
-    element_fluxes<DIM, MODE>(e, N, zt, rho0, conn, ezone, kmul, emask, eorient, xyz, psi, kr, qface, aface, n, f);
+    element_fluxes<DIM, MODE>(e, N, zk, rho0, conn, ezone, kmul, emask, eorient, xyz, psi, nullptr, 0., kr, qface,
+                              aface, n, f);
 #pragma unroll
     for (int k = 0; k < DIM + 1; ++k) atomicAdd(&acc[n[k]], f[k]);
 }
 
-// LDS-staged owner-computes residual.  A workgroup owns kTileNodes nodes; it
+// Element-owner residual: a workgroup owns the nodes of one tile and evaluates
+// the tile's primary elements (lowest node in the tile; every element has
+// exactly one primary tile, so each element is evaluated once), gathering node
+// data through the caches.  Contributions to owned nodes accumulate in LDS
+// (ds_add_f64); those to nodes of other tiles go to the global accumulator with
+// hardware FP64 atomics, as do the owned sums at the end (after the tile's
+// boundary fluxes).  The finalisation is a separate node pass.
+template <int DIM, bool JV>
+__global__ void __launch_bounds__(256) owner_kernel(
+    int N, int ntiles, int tile_nodes, int per_xcd, ZoneTable zt, double rho0, const int4 *__restrict__ conn,
+    const uint8_t *__restrict__ ezone, const double *__restrict__ kmul, const uint8_t *__restrict__ emask,
+    const double *__restrict__ xyz, const double *__restrict__ psi, const double *__restrict__ kr,
+    const int *__restrict__ eptr, const int *__restrict__ bptr, const int *__restrict__ bnode,
+    const double *__restrict__ bval, double *__restrict__ acc_g, JvArgs jv) {
+    extern __shared__ double sm[];
+    const int tile = per_xcd ? (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3) : blockIdx.x;
+    if (tile >= ntiles) return;  // uniform for the whole workgroup
+    const double sig = JV ? jv.sig[0] : 0.;
+    const int lo = tile * tile_nodes, hi = min(N, lo + tile_nodes), nown = hi - lo;
+    for (int i = threadIdx.x; i < nown; i += blockDim.x) sm[i] = 0.;
+    __shared__ double zk[3 * kMaxZones];
+    if (threadIdx.x < 3 * zt.nz) zk[threadIdx.x] = zt.p[threadIdx.x / 3][ZK_X + threadIdx.x % 3];
+    __syncthreads();
+    for (int e = eptr[tile] + threadIdx.x; e < eptr[tile + 1]; e += blockDim.x) {
+        int n[4];
+        double f[4];
+        element_fluxes<DIM, 0, JV>(e, N, zk, rho0, conn, ezone, kmul, emask, nullptr, xyz, psi, jv.v, sig, kr,
+                                   nullptr, nullptr, n, f);
+#pragma unroll
+        for (int q = 0; q < DIM + 1; ++q) {
+            if (n[q] >= lo && n[q] < hi) atomicAdd(&sm[n[q] - lo], f[q]);
+            else atomicAdd(&acc_g[n[q]], f[q]);
+        }
+    }
+    __syncthreads();
+    for (int k = bptr[tile] + threadIdx.x; k < bptr[tile + 1]; k += blockDim.x) sm[bnode[k] - lo] -= bval[k];
+    __syncthreads();
+    for (int i = threadIdx.x; i < nown; i += blockDim.x) atomicAdd(&acc_g[lo + i], sm[i]);
+}
+
+// LDS-staged owner-computes residual.  A workgroup owns tile_nodes nodes; it
 // stages the coordinates, heads and relative permeabilities of its window (owned
 // nodes plus the halo nodes its elements reach) in LDS with coalesced loads for
 // the owned range and one gather per halo node, then evaluates every element
@@ -209,7 +254,7 @@ __global__ void __launch_bounds__(256) elem_kernel(int T, int N, ZoneTable zt, d
 // JV: directional-derivative mode (state psi + sig v, output (F - F0) / sig).
 template <int DIM, bool JV>
 __global__ void __launch_bounds__(256) tile_lds_kernel(
-    int N, int ntiles, int per_xcd, int wmax, int ntzmax, ZoneTable zt, double rho0, const uint8_t *__restrict__ ezone,
+    int N, int ntiles, int tile_nodes, int per_xcd, int wmax, int ntzmax, ZoneTable zt, double rho0, const uint8_t *__restrict__ ezone,
     const double *__restrict__ kmul, const uint8_t *__restrict__ emask, const double *__restrict__ xyz,
     const double *__restrict__ psi, const double *__restrict__ kr, const int *__restrict__ eptr,
     const int *__restrict__ hptr, const int *__restrict__ hlist, const int *__restrict__ wptr,
@@ -219,10 +264,12 @@ __global__ void __launch_bounds__(256) tile_lds_kernel(
     const double *__restrict__ bval, const int *__restrict__ dptr, const int *__restrict__ dnode,
     const double *__restrict__ dval, double *__restrict__ F, JvArgs jv) {
     extern __shared__ double sm[];
-    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    // workgroups are dealt to the 8 XCDs round-robin: with per_xcd > 0 the
+    // tiles of one XCD are contiguous (their halos share its L2)
+    const int tile = per_xcd ? (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3) : blockIdx.x;
     if (tile >= ntiles) return;  // uniform for the whole workgroup
     const double sig = JV ? jv.sig[0] : 0.;
-    const int lo = tile * kTileNodes, hi = min(N, lo + kTileNodes), nown = hi - lo;
+    const int lo = tile * tile_nodes, hi = min(N, lo + tile_nodes), nown = hi - lo;
     const int w0 = wptr[tile], win = nown + wptr[tile + 1] - w0;
     int tz[4], ntz = 0;
 #pragma unroll
@@ -345,20 +392,33 @@ __global__ void boundary_kernel(int nb, const int *__restrict__ bnode, const dou
     if (k < nb) acc[bnode[k]] -= bval[k];
 }
 
+template <bool JV = false>
 __global__ void finalize_kernel(int N, const double *__restrict__ acc, const double *__restrict__ invV,
                                 const double *__restrict__ psip, const double *__restrict__ cap,
-                                double *__restrict__ F) {
+                                double *__restrict__ F, JvArgs jv = JvArgs()) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    F[i] = acc[i] * invV[i] - psip[i] * cap[i];
+    if (JV) {
+        const double sig = jv.sig[0];
+        const double yp = psip[i] + jv.cj * (sig * jv.v[i]);
+        const double F1 = acc[i] * invV[i] - yp * cap[i];
+        F[i] = (F1 - jv.F0[i]) / sig;
+    } else
+        F[i] = acc[i] * invV[i] - psip[i] * cap[i];
 }
 
+template <bool JV = false>
 __global__ void dirichlet_kernel(int nd, const int *__restrict__ dnode, const double *__restrict__ dval,
-                                 const double *__restrict__ psi, double *__restrict__ F) {
+                                 const double *__restrict__ psi, double *__restrict__ F, JvArgs jv = JvArgs()) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < nd) {
         int i = dnode[k];
-        F[i] = dval[k] - psi[i];
+        if (JV) {
+            const double sig = jv.sig[0];
+            const double y = psi[i] + sig * jv.v[i];
+            F[i] = ((dval[k] - y) - jv.F0[i]) / sig;
+        } else
+            F[i] = dval[k] - psi[i];
     }
 }
 
@@ -403,30 +463,47 @@ int tile_kernel_attrs(fvp_ctx *c) {
     int lds_max = 0;
     if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess)
         lds_max = 64 * 1024;
-    if (c->tile_lds > (size_t)lds_max) {
+    // the static zone table shares the workgroup's LDS with the dynamic window
+    const size_t static_lds = sizeof(double) * 3 * kMaxZones;
+    if (c->tile_lds + static_lds > (size_t)lds_max) {
         set_error("residual tile window needs " + std::to_string(c->tile_lds) + " B of LDS, device allows " +
                   std::to_string(lds_max));
         return -1;
     }
-    FVP_HIP(hipFuncSetAttribute((const void *)tile_lds_kernel<3, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)c->tile_lds));
-    FVP_HIP(hipFuncSetAttribute((const void *)tile_lds_kernel<2, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)c->tile_lds));
-    FVP_HIP(hipFuncSetAttribute((const void *)tile_lds_kernel<3, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)c->tile_lds));
-    FVP_HIP(hipFuncSetAttribute((const void *)tile_lds_kernel<2, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)c->tile_lds));
+    // the attribute is per kernel, not per context: allow the device maximum so
+    // that contexts with different windows can coexist
+    const int dyn = lds_max - (int)static_lds;
+    FVP_HIP(hipFuncSetAttribute((const void *)tile_lds_kernel<3, false>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+    FVP_HIP(hipFuncSetAttribute((const void *)tile_lds_kernel<2, false>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+    FVP_HIP(hipFuncSetAttribute((const void *)tile_lds_kernel<3, true>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+    FVP_HIP(hipFuncSetAttribute((const void *)tile_lds_kernel<2, true>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
     return 0;
 }
 
 template <int DIM, bool JV>
 static void launch_tile(fvp_ctx *c, const double *psi, const double *psip, double *F, const JvArgs &jv) {
     const int per = (c->ntiles + 7) / 8;  // tiles per XCD: consecutive tiles share an XCD's L2
-    tile_lds_kernel<DIM, JV><<<per * 8, 256, c->tile_lds, c->stream>>>(
-        c->N, c->ntiles, per, c->win_max, c->ntz_max, c->zt, c->rho0, c->ezone.p, c->kmul.p, c->emask.p, c->xyz.p,
+    hipStream_t s = c->stream;
+    if (c->rvariant == FVP_RESIDUAL_OWNER) {
+        const int N = c->N;
+        (void)hipMemsetAsync(c->acc.p, 0, sizeof(double) * N, s);
+        prof_mark(c, 0, 0);
+        owner_kernel<DIM, JV><<<c->xcd_map ? per * 8 : c->ntiles, 256, sizeof(double) * c->tile_nodes, s>>>(
+            N, c->ntiles, c->tile_nodes, c->xcd_map ? per : 0, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
+            c->emask.p, c->xyz.p, psi, c->kr.p, c->t_eptr.p, c->t_bptr.p, c->bnode.p, c->bval.p, c->acc.p, jv);
+        prof_mark(c, 0, 1);
+        finalize_kernel<JV><<<blocks(N, 256), 256, 0, s>>>(N, c->acc.p, c->invV.p, psip, c->cap.p, F, jv);
+        if (c->ndir)
+            dirichlet_kernel<JV><<<blocks(c->ndir, 256), 256, 0, s>>>(c->ndir, c->dnode.p, c->dval.p, psi, F, jv);
+        return;
+    }
+    prof_mark(c, 0, 0);
+    tile_lds_kernel<DIM, JV><<<c->xcd_map ? per * 8 : c->ntiles, 256, c->tile_lds, c->stream>>>(
+        c->N, c->ntiles, c->tile_nodes, c->xcd_map ? per : 0, c->win_max, c->ntz_max, c->zt, c->rho0, c->ezone.p, c->kmul.p, c->emask.p, c->xyz.p,
         psi, c->kr.p, c->t_eptr.p, c->t_hptr.p, c->t_hlist.p, c->t_wptr.p, c->t_whalo.p, c->t_vptr.p,
         (const ushort4 *)c->t_lconn.p, c->t_tz.p, c->invV.p, psip, c->cap.p, c->t_bptr.p, c->bnode.p, c->bval.p,
         c->t_dptr.p, c->dnode.p, c->dval.p, F, jv);
+    prof_mark(c, 0, 1);
 }
 
 int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, double *F, int averaging,
@@ -447,8 +524,7 @@ int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, d
         vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p,
                                                          c->cap.p, nullptr, nullptr);
     if (!averaging) {
-        // tiled owner-computes path: element fluxes, boundary, finalize and Dirichlet in one launch
-        prof_mark(c, 0, 0);
+        // tiled paths (element fluxes with the boundary fluxes, then the finalisation)
         const JvArgs none;
         if (c->dim == 3) {
             if (jv) launch_tile<3, true>(c, psi, psip, F, *jv);
@@ -457,7 +533,6 @@ int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, d
             if (jv) launch_tile<2, true>(c, psi, psip, F, *jv);
             else launch_tile<2, false>(c, psi, psip, F, none);
         }
-        prof_mark(c, 0, 1);
     } else {
         FVP_HIP(hipMemsetAsync(c->acc.p, 0, sizeof(double) * N, s));
         if (c->dim == 3)
@@ -596,5 +671,71 @@ extern "C" int fvp_profile_get(fvp_ctx *c, int kind, double *total_ms, long *cou
     prof_fold(c, kind);
     if (total_ms) *total_ms = c->prof_ms[kind];
     if (count) *count = c->prof_n[kind];
+    return 0;
+}
+
+extern "C" int fvp_set_residual_launch(fvp_ctx *c, int variant, int tile_nodes, int xcd_map) {
+    if (!c) { set_error("null context"); return -1; }
+    if (variant != FVP_RESIDUAL_OWNER && variant != FVP_RESIDUAL_WINDOW) { set_error("unknown residual variant"); return -1; }
+    if (tile_nodes < 64 || tile_nodes > 4096 || (tile_nodes & (tile_nodes - 1))) {
+        set_error("tile_nodes must be a power of two in [64, 4096]");
+        return -1;
+    }
+    FVP_HIP(hipSetDevice(c->device));
+    FVP_HIP(hipStreamSynchronize(c->stream));
+    const int old = c->tile_nodes, old_v = c->rvariant;
+    c->rvariant = variant;  // the window variant's LDS check depends on it
+    if ((tile_nodes != old || variant != old_v) && build_tiles(c, tile_nodes)) {
+        const std::string why = fvp_last_error();
+        c->rvariant = old_v;
+        if (build_tiles(c, old)) return -1;  // keep the context usable
+        set_error(why);
+        return -1;
+    }
+    c->xcd_map = xcd_map ? 1 : 0;
+    return 0;
+}
+
+// Launch-shape autotuner for the residual: every (tile size, XCD grouping)
+// candidate that fits the LDS is built and timed with HIP events over a few
+// upwinded residual evaluations at the given state; the fastest is kept.
+extern "C" int fvp_autotune(fvp_ctx *c, double t, const double *psi, const double *psip, int *variant,
+                            int *tile_nodes, int *xcd_map, double *best_us) {
+    if (!c || !psi || !psip) { set_error("null argument"); return -1; }
+    FVP_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    if (fvp_begin_step(c, t, psi, psip)) return -1;
+    hipEvent_t e0, e1;
+    FVP_HIP(hipEventCreate(&e0));
+    FVP_HIP(hipEventCreate(&e1));
+    const int tiles[] = {256, 512, 1024, 2048};
+    int best_t = c->tile_nodes, best_x = c->xcd_map, best_v = c->rvariant;
+    double best = 1e300;
+    const int reps = 10;
+    for (int v : {FVP_RESIDUAL_OWNER, FVP_RESIDUAL_WINDOW})
+    for (int tn : tiles) {
+        if (fvp_set_residual_launch(c, v, tn, 1)) continue;  // window does not fit the LDS
+        for (int x = 0; x <= 1; ++x) {
+            c->xcd_map = x;
+            for (int w = 0; w < 3; ++w)
+                if (residual_impl(c, t, psi, psip, c->tmp3.p, 0)) return -1;
+            FVP_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r)
+                if (residual_impl(c, t, psi, psip, c->tmp3.p, 0)) return -1;
+            FVP_HIP(hipEventRecord(e1, s));
+            FVP_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            FVP_HIP(hipEventElapsedTime(&ms, e0, e1));
+            const double us = 1e3 * ms / reps;
+            if (us < best) { best = us; best_t = tn; best_x = x; best_v = v; }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (fvp_set_residual_launch(c, best_v, best_t, best_x)) return -1;
+    if (variant) *variant = best_v;
+    if (tile_nodes) *tile_nodes = best_t;
+    if (xcd_map) *xcd_map = best_x;
+    if (best_us) *best_us = best;
     return 0;
 }

@@ -91,6 +91,125 @@ int upload_bc_values(fvp_ctx *c) {
     return c->bval.upload(val.data(), val.size());
 }
 
+// Node tiles: tile t owns nodes [t*TILE, (t+1)*TILE); it processes its primary
+// elements (lowest node inside, a contiguous range) and the halo elements that
+// reach into it from other tiles.  Rebuilt when the tile size changes.
+int build_tiles(fvp_ctx *c, int TILE) {
+    const int N = c->N, T = c->T, npe = c->npe;
+    std::vector<int4> conn4(T);
+    std::vector<uint8_t> ez(T);
+    FVP_HIP(hipMemcpy(conn4.data(), c->conn.p, sizeof(int4) * T, hipMemcpyDeviceToHost));
+    FVP_HIP(hipMemcpy(ez.data(), c->ezone.p, T, hipMemcpyDeviceToHost));
+    {
+        const int nt = (N + TILE - 1) / TILE;
+        c->ntiles = nt;
+        std::vector<int> eptr(nt + 1, 0), hcnt(nt + 1, 0);
+        std::vector<int> first_tile(T);
+        for (int e = 0; e < T; ++e) {
+            const int4 q = conn4[e];
+            int n[4] = {q.x, q.y, q.z, q.w};
+            int m = n[0];
+            for (int k = 1; k < npe; ++k) m = std::min(m, n[k]);
+            first_tile[e] = m / TILE;
+            eptr[m / TILE + 1]++;
+        }
+        for (int t = 0; t < nt; ++t) eptr[t + 1] += eptr[t];
+        const bool window = c->rvariant == FVP_RESIDUAL_WINDOW;
+        std::vector<std::vector<int>> halo(window ? nt : 0);
+        for (int e = 0; window && e < T; ++e) {
+            const int4 q = conn4[e];
+            int n[4] = {q.x, q.y, q.z, q.w};
+            int tl[4], cnt = 0;
+            for (int k = 0; k < npe; ++k) {
+                int t = n[k] / TILE;
+                if (t == first_tile[e]) continue;
+                bool seen = false;
+                for (int r = 0; r < cnt; ++r) seen |= tl[r] == t;
+                if (!seen) tl[cnt++] = t;
+            }
+            for (int r = 0; r < cnt; ++r) halo[tl[r]].push_back(e);
+        }
+        std::vector<int> hptr(nt + 1, 0), hlist;
+        for (int t = 0; window && t < nt; ++t) hptr[t + 1] = hptr[t] + (int)halo[t].size();
+        hlist.reserve(hptr[nt]);
+        for (int t = 0; window && t < nt; ++t) hlist.insert(hlist.end(), halo[t].begin(), halo[t].end());
+        // window of each tile = owned nodes [lo, hi) then its halo nodes (ascending);
+        // per element visit (primary elements, then halo elements) the window-local
+        // node indices, and the zones the tile's elements use (at most 4 per tile)
+        std::vector<int> wptr(nt + 1, 0), whalo, vptr(nt + 1, 0), tz((size_t)nt * 4, -1);
+        std::vector<unsigned short> lconn;
+        lconn.reserve((size_t)4 * (T + hptr[nt]));
+        std::vector<int> mark(N, -1), loc(N, 0);
+        int wmax = 0, ntzmax = 1;
+        for (int t = 0; window && t < nt; ++t) {
+            const int lo = t * TILE, hi = std::min(N, lo + TILE), nown = hi - lo;
+            std::vector<int> hal;
+            unsigned zones = 0;
+            auto scan = [&](int e) {
+                const int4 q = conn4[e];
+                int n[4] = {q.x, q.y, q.z, q.w};
+                for (int k = 0; k < npe; ++k)
+                    if ((n[k] < lo || n[k] >= hi) && mark[n[k]] != t) { mark[n[k]] = t; hal.push_back(n[k]); }
+                zones |= 1u << ez[e];
+            };
+            for (int e = eptr[t]; e < eptr[t + 1]; ++e) scan(e);
+            for (int e : halo[t]) scan(e);
+            std::sort(hal.begin(), hal.end());
+            for (size_t j = 0; j < hal.size(); ++j) loc[hal[j]] = nown + (int)j;
+            const int win = nown + (int)hal.size();
+            if (win > 65535) { set_error("tile window exceeds 65535 nodes"); return -1; }
+            wmax = std::max(wmax, win);
+            int nzt = 0;
+            for (int z = 0; z < c->nz; ++z)
+                if (zones & (1u << z)) {
+                    if (nzt == 4) { set_error("a node tile touches more than 4 zones"); return -1; }
+                    tz[(size_t)t * 4 + nzt++] = z;
+                }
+            ntzmax = std::max(ntzmax, nzt);
+            auto emit = [&](int e) {
+                const int4 q = conn4[e];
+                int n[4] = {q.x, q.y, q.z, q.w};
+                for (int k = 0; k < 4; ++k) {
+                    int l = 0;
+                    if (k < npe) l = (n[k] >= lo && n[k] < hi) ? n[k] - lo : loc[n[k]];
+                    lconn.push_back((unsigned short)l);
+                }
+            };
+            for (int e = eptr[t]; e < eptr[t + 1]; ++e) emit(e);
+            for (int e : halo[t]) emit(e);
+            whalo.insert(whalo.end(), hal.begin(), hal.end());
+            wptr[t + 1] = (int)whalo.size();
+            vptr[t + 1] = (int)(lconn.size() / 4);
+        }
+        c->win_max = wmax;
+        c->ntz_max = ntzmax;
+        c->tile_lds = window ? (size_t)(4 + ntzmax) * wmax * sizeof(double) + (size_t)TILE * sizeof(double)
+                             : (size_t)TILE * sizeof(double);
+        if (window) {
+            if (c->t_wptr.upload(wptr.data(), nt + 1) || c->t_whalo.upload(whalo.data(), whalo.size()) ||
+                c->t_vptr.upload(vptr.data(), nt + 1) || c->t_lconn.upload(lconn.data(), lconn.size()) ||
+                c->t_tz.upload(tz.data(), tz.size()))
+                return -1;
+        } else {  // the element-owner variant reads none of the window tables
+            c->t_wptr.release(); c->t_whalo.release(); c->t_vptr.release(); c->t_lconn.release(); c->t_tz.release();
+        }
+        std::vector<int> bptr(nt + 1, 0), dptr(nt + 1, 0);
+        for (int k = 0; k < c->nb; ++k) bptr[c->bnode_h[k] / TILE + 1]++;
+        std::vector<int> dn_h(c->ndir);
+        if (c->ndir) FVP_HIP(hipMemcpy(dn_h.data(), c->dnode.p, sizeof(int) * c->ndir, hipMemcpyDeviceToHost));
+        for (int k = 0; k < c->ndir; ++k) dptr[dn_h[k] / TILE + 1]++;
+        for (int t = 0; t < nt; ++t) { bptr[t + 1] += bptr[t]; dptr[t + 1] += dptr[t]; }
+        c->tile_elems = (long)T + hptr[nt];
+        if (c->t_eptr.upload(eptr.data(), nt + 1) || c->t_hptr.upload(hptr.data(), nt + 1) ||
+            (window && c->t_hlist.upload(hlist.data(), hlist.size())) || c->t_bptr.upload(bptr.data(), nt + 1) ||
+            c->t_dptr.upload(dptr.data(), nt + 1))
+            return -1;
+        if (!window) c->t_hlist.release();
+    }
+    c->tile_nodes = TILE;
+    return tile_kernel_attrs(c);
+}
+
 }  // namespace fvp
 
 using namespace fvp;
@@ -402,110 +521,7 @@ extern "C" int fvp_create(const fvp_problem *p, int device, fvp_ctx **out) {
         c->ndir = (int)dn.size();
         if (c->dnode.upload(dn.data(), dn.size()) || c->dval.upload(dv.data(), dv.size())) return fail();
     }
-    // ---- node tiles: tile t owns nodes [t*TILE, (t+1)*TILE); it processes its
-    // primary elements (lowest node inside, a contiguous range) and the halo
-    // elements that reach into it from earlier tiles
-    {
-        const int TILE = kTileNodes;
-        const int nt = (N + TILE - 1) / TILE;
-        c->ntiles = nt;
-        std::vector<int> eptr(nt + 1, 0), hcnt(nt + 1, 0);
-        std::vector<int> first_tile(T);
-        for (int e = 0; e < T; ++e) {
-            const int4 q = conn4[e];
-            int n[4] = {q.x, q.y, q.z, q.w};
-            int m = n[0];
-            for (int k = 1; k < npe; ++k) m = std::min(m, n[k]);
-            first_tile[e] = m / TILE;
-            eptr[m / TILE + 1]++;
-        }
-        for (int t = 0; t < nt; ++t) eptr[t + 1] += eptr[t];
-        std::vector<std::vector<int>> halo(nt);
-        for (int e = 0; e < T; ++e) {
-            const int4 q = conn4[e];
-            int n[4] = {q.x, q.y, q.z, q.w};
-            int tl[4], cnt = 0;
-            for (int k = 0; k < npe; ++k) {
-                int t = n[k] / TILE;
-                if (t == first_tile[e]) continue;
-                bool seen = false;
-                for (int r = 0; r < cnt; ++r) seen |= tl[r] == t;
-                if (!seen) tl[cnt++] = t;
-            }
-            for (int r = 0; r < cnt; ++r) halo[tl[r]].push_back(e);
-        }
-        std::vector<int> hptr(nt + 1, 0), hlist;
-        for (int t = 0; t < nt; ++t) hptr[t + 1] = hptr[t] + (int)halo[t].size();
-        hlist.reserve(hptr[nt]);
-        for (int t = 0; t < nt; ++t) hlist.insert(hlist.end(), halo[t].begin(), halo[t].end());
-        // window of each tile = owned nodes [lo, hi) then its halo nodes (ascending);
-        // per element visit (primary elements, then halo elements) the window-local
-        // node indices, and the zones the tile's elements use (at most 4 per tile)
-        std::vector<int> wptr(nt + 1, 0), whalo, vptr(nt + 1, 0), tz((size_t)nt * 4, -1);
-        std::vector<unsigned short> lconn;
-        lconn.reserve((size_t)4 * (T + hptr[nt]));
-        std::vector<int> mark(N, -1), loc(N, 0);
-        int wmax = 0, ntzmax = 1;
-        for (int t = 0; t < nt; ++t) {
-            const int lo = t * TILE, hi = std::min(N, lo + TILE), nown = hi - lo;
-            std::vector<int> hal;
-            unsigned zones = 0;
-            auto scan = [&](int e) {
-                const int4 q = conn4[e];
-                int n[4] = {q.x, q.y, q.z, q.w};
-                for (int k = 0; k < npe; ++k)
-                    if ((n[k] < lo || n[k] >= hi) && mark[n[k]] != t) { mark[n[k]] = t; hal.push_back(n[k]); }
-                zones |= 1u << ez[e];
-            };
-            for (int e = eptr[t]; e < eptr[t + 1]; ++e) scan(e);
-            for (int e : halo[t]) scan(e);
-            std::sort(hal.begin(), hal.end());
-            for (size_t j = 0; j < hal.size(); ++j) loc[hal[j]] = nown + (int)j;
-            const int win = nown + (int)hal.size();
-            if (win > 65535) { set_error("tile window exceeds 65535 nodes"); return fail(); }
-            wmax = std::max(wmax, win);
-            int nzt = 0;
-            for (int z = 0; z < c->nz; ++z)
-                if (zones & (1u << z)) {
-                    if (nzt == 4) { set_error("a node tile touches more than 4 zones"); return fail(); }
-                    tz[(size_t)t * 4 + nzt++] = z;
-                }
-            ntzmax = std::max(ntzmax, nzt);
-            auto emit = [&](int e) {
-                const int4 q = conn4[e];
-                int n[4] = {q.x, q.y, q.z, q.w};
-                for (int k = 0; k < 4; ++k) {
-                    int l = 0;
-                    if (k < npe) l = (n[k] >= lo && n[k] < hi) ? n[k] - lo : loc[n[k]];
-                    lconn.push_back((unsigned short)l);
-                }
-            };
-            for (int e = eptr[t]; e < eptr[t + 1]; ++e) emit(e);
-            for (int e : halo[t]) emit(e);
-            whalo.insert(whalo.end(), hal.begin(), hal.end());
-            wptr[t + 1] = (int)whalo.size();
-            vptr[t + 1] = (int)(lconn.size() / 4);
-        }
-        c->win_max = wmax;
-        c->ntz_max = ntzmax;
-        c->tile_lds = (size_t)(4 + ntzmax) * wmax * sizeof(double) + (size_t)TILE * sizeof(double);
-        if (c->t_wptr.upload(wptr.data(), nt + 1) || c->t_whalo.upload(whalo.data(), whalo.size()) ||
-            c->t_vptr.upload(vptr.data(), nt + 1) || c->t_lconn.upload(lconn.data(), lconn.size()) ||
-            c->t_tz.upload(tz.data(), tz.size()))
-            return fail();
-        std::vector<int> bptr(nt + 1, 0), dptr(nt + 1, 0);
-        for (int k = 0; k < c->nb; ++k) bptr[c->bnode_h[k] / TILE + 1]++;
-        std::vector<int> dn_h(c->ndir);
-        if (c->ndir) FVP_HIP(hipMemcpy(dn_h.data(), c->dnode.p, sizeof(int) * c->ndir, hipMemcpyDeviceToHost));
-        for (int k = 0; k < c->ndir; ++k) dptr[dn_h[k] / TILE + 1]++;
-        for (int t = 0; t < nt; ++t) { bptr[t + 1] += bptr[t]; dptr[t + 1] += dptr[t]; }
-        c->tile_elems = (long)T + hptr[nt];
-        if (c->t_eptr.upload(eptr.data(), nt + 1) || c->t_hptr.upload(hptr.data(), nt + 1) ||
-            c->t_hlist.upload(hlist.data(), hlist.size()) || c->t_bptr.upload(bptr.data(), nt + 1) ||
-            c->t_dptr.upload(dptr.data(), nt + 1))
-            return fail();
-    }
-    if (tile_kernel_attrs(c)) return fail();
+    if (build_tiles(c, c->tile_nodes)) return fail();
     if (c->tmp1.alloc(N) || c->tmp2.alloc(N) || c->tmp3.alloc(N) || c->red.alloc(4096)) return fail();
     c->red_h.resize(4096);
     *out = c;

@@ -50,6 +50,9 @@ SIGNATURES = {
     "fvp_precon_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fvp_precon_block_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fvp_set_block_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "fvp_set_residual_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "fvp_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, _c_int_p,
+                                    _c_int_p, _c_int_p, _c_dbl_p]),
     "fvp_integrator_set_segments": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_int_p]),
     "fvp_precon_apply": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fvp_precon_values": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p]),
@@ -77,6 +80,7 @@ SIGNATURES = {
 }
 
 PRECON_NONE, PRECON_JACOBI, PRECON_ILU0, PRECON_BJILU0, PRECON_BJILU0_F32 = 0, 2, 5, 7, 8
+RESIDUAL_OWNER, RESIDUAL_WINDOW = 0, 1
 BLOCK_ROWS = 512
 ERR_PIVOT, ERR_NOCONV = 1, 2
 
@@ -302,6 +306,18 @@ def _ctx_set_block_rows(self, rows):
     _check(lib().fvp_set_block_rows(self.h, int(rows)), "fvp_set_block_rows")
 
 
+def _ctx_set_residual_launch(self, variant=0, tile_nodes=512, xcd_map=1):
+    _check(lib().fvp_set_residual_launch(self.h, int(variant), int(tile_nodes), int(xcd_map)),
+           "fvp_set_residual_launch")
+
+
+def _ctx_autotune(self, t, psi, psip):
+    v, tn, xm, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    _check(lib().fvp_autotune(self.h, float(t), _ptr(psi), _ptr(psip), ctypes.byref(v), ctypes.byref(tn),
+                              ctypes.byref(xm), ctypes.byref(us)), "fvp_autotune")
+    return {"variant": v.value, "tile_nodes": tn.value, "xcd_map": xm.value, "residual_us": us.value}
+
+
 def _ctx_colours(self):
     nc = ctypes.c_int()
     col = np.zeros(self.N, dtype=np.int32)
@@ -335,6 +351,8 @@ Context.precon_solve = _ctx_precon_solve
 Context.jacobian = _ctx_jacobian
 Context.colours = _ctx_colours
 Context.set_block_rows = _ctx_set_block_rows
+Context.set_residual_launch = _ctx_set_residual_launch
+Context.autotune = _ctx_autotune
 Context.newton_gmres = _ctx_newton_gmres
 
 

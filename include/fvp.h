@@ -239,6 +239,25 @@ int fvp_integrator_state(fvp_integrator *I, double *t, const double **psi, const
 int fvp_integrator_get_stats(const fvp_integrator *I, fvp_integrator_stats *st);
 
 /* ---------------------------------------------------------------------------
+ * Launch shape of the residual.  Nodes are processed in tiles of tile_nodes
+ * consecutive ids (a power of two in [64, 4096], default 512), one workgroup
+ * per tile; xcd_map = 1 deals consecutive tiles to one XCD (shared L2 for
+ * their halos).  Variants: FVP_RESIDUAL_OWNER evaluates each element once, in
+ * the tile of its lowest node, with LDS sums for owned nodes and FP64 atomics
+ * for the others, then a node pass finalises (default); FVP_RESIDUAL_WINDOW
+ * stages every tile's node window in LDS and evaluates each element once per
+ * tile it touches, with no global atomics and a fused finalisation (needs the
+ * window to fit the LDS).  fvp_autotune times every variant / tile size / XCD
+ * grouping at the state (psi, psi') and keeps the fastest; it refreshes the
+ * upwind weights from that state (fvp_begin_step).  No counterpart in the
+ * reference. */
+#define FVP_RESIDUAL_OWNER 0
+#define FVP_RESIDUAL_WINDOW 1
+int fvp_set_residual_launch(fvp_ctx *ctx, int variant, int tile_nodes, int xcd_map);
+int fvp_autotune(fvp_ctx *ctx, double t, const double *psi, const double *psip, int *variant, int *tile_nodes,
+                 int *xcd_map, double *best_us);
+
+/* ---------------------------------------------------------------------------
  * Live kernel timing with HIP events on the context's stream (the reference's
  * util::TimerCuda, timer_cuda.h:11-55).  While enabled every residual
  * evaluation records its element-flux kernel and the whole residual; kind 0 =

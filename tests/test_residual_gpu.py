@@ -137,3 +137,45 @@ def test_jv_matches_oracle_difference_quotient(which, request):
     err = np.abs(Jv.cpu().numpy() - Jc).max() / np.abs(Jc).max()
     print(f"{which} Jv rel={err:.3e}")
     assert err < 1e-6
+
+
+@pytest.mark.parametrize("which", CASES)
+def test_every_launch_shape_and_the_autotuner_keep_parity(which, request):
+    """Both residual variants (element-owner with atomics, LDS window), several
+    tile sizes and both tile-to-XCD mappings: residual and J v against the
+    oracle; then the autotuner's pick."""
+    from fvmpor_amd.lib import RESIDUAL_OWNER, RESIDUAL_WINDOW
+    prob = request.getfixturevalue(which)
+    ctx, dev, torch = _gpu(prob)
+    psi, psip = _state(prob, 3)
+    op = OracleProblem(prob)
+    tp, tpp = torch.tensor(psi, device=dev), torch.tensor(psip, device=dev)
+    ctx.begin_step(0.0, tp, tpp)
+    wf, _ = ctx.get_upwind()
+    op.set_weights(wf)
+    Fc = op.residual(0.2, psi, psip)
+    s = op.flux_scale()
+    v = np.random.default_rng(9).normal(size=len(psi))
+    cj = 2.5
+    sig = np.sqrt(2.0 ** -52) * np.linalg.norm(psi) / np.linalg.norm(v)
+    Jc = (op.residual(0.2, psi + sig * v, psip + cj * (sig * v)) - Fc) / sig
+    tv = torch.tensor(v, device=dev)
+    F, Jv = torch.empty_like(tp), torch.empty_like(tp)
+    shapes = [(RESIDUAL_OWNER, t, x) for t in (64, 256, 512, 1024) for x in (0, 1)] + [(RESIDUAL_WINDOW, 512, 1)]
+
+    def check(label):
+        ctx.residual(0.2, tp, tpp, F)
+        ctx.jv(0.2, tp, tpp, F, cj, tv, Jv)
+        torch.cuda.synchronize()
+        rel = _rel(F.cpu().numpy(), Fc, s)
+        jrel = np.abs(Jv.cpu().numpy() - Jc).max() / np.abs(Jc).max()
+        assert rel <= TOL, (label, rel)
+        assert jrel < 1e-6, (label, jrel)
+        return rel
+
+    for shape in shapes:
+        ctx.set_residual_launch(*shape)
+        check(shape)
+    pick = ctx.autotune(0.0, tp, tpp)
+    rel = check(pick)
+    print(f"{which}: autotuner picked {pick}, rel={rel:.2e}")
