@@ -356,26 +356,36 @@ __global__ void lincomb(long n, double *__restrict__ out, const double *__restri
 
 // ---- block-Jacobi ILU(0): one workgroup per block of bs consecutive rows
 // Factorisation: the whole block (values, local columns, row pointers, diagonal
-// offsets) is staged in LDS; one wavefront factors one row with a lane per
-// entry: for each L entry k (in column order) the multiplier l_ik = a_ik / u_kk
-// is broadcast, row k is loaded across lanes and matched against the row's
-// columns with shuffles.  Rows of one level run on different wavefronts.
+// offsets) is staged in LDS; one wavefront factors one row (IKJ order): for
+// each L entry k of row i in column order, l_ik = a_ik / u_kk, then the lanes
+// walk row k's U part and subtract l_ik u_kj from a_ij where (i, j) is in the
+// pattern, found through a per-wavefront column -> position map of row i.
+// Rows of one level run on different wavefronts.
+constexpr int kBjFactorThreads = 512;
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename T>
-__global__ void __launch_bounds__(1024) bj_factor(int n, int bs, int max_nnz, const int *__restrict__ ia,
-                                                 const unsigned short *__restrict__ ja, const int *__restrict__ diag,
-                                                 const int *__restrict__ src, const double *__restrict__ a,
-                                                 const int *__restrict__ blev, const int *__restrict__ lptr,
-                                                 const unsigned short *__restrict__ rows, T *__restrict__ lu,
-                                                 int *__restrict__ flag) {
+__global__ void __launch_bounds__(kBjFactorThreads) bj_factor(
+    int n, int bs, int max_nnz, const int *__restrict__ ia, const unsigned short *__restrict__ ja,
+    const int *__restrict__ diag, const int *__restrict__ src, const double *__restrict__ a,
+    const int *__restrict__ blev, const int *__restrict__ lptr, const unsigned short *__restrict__ rows,
+    T *__restrict__ lu, int *__restrict__ flag) {
     extern __shared__ double smem[];
     T *val = (T *)smem;
     unsigned short *col = (unsigned short *)(val + max_nnz);
     unsigned short *rp = col + max_nnz;
     unsigned short *dg = rp + (bs + 1);
+    short *pmap = (short *)(dg + bs);  // per wavefront: position of column j in its current row, -1 if absent
     const int b = blockIdx.x;
     const int base = b * bs;
     const int cnt = min(bs, n - base);
     const int z0 = ia[base], nzb = ia[base + cnt] - z0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
     for (int k = threadIdx.x; k < nzb; k += blockDim.x) {
         val[k] = (T)a[src[z0 + k]];
         col[k] = ja[z0 + k];
@@ -384,90 +394,130 @@ __global__ void __launch_bounds__(1024) bj_factor(int n, int bs, int max_nnz, co
         rp[r] = (unsigned short)(ia[base + r] - z0);
         dg[r] = (unsigned short)(diag[base + r] - z0);
     }
+    for (int k = threadIdx.x; k < nwaves * bs; k += blockDim.x) pmap[k] = -1;
     if (threadIdx.x == 0) rp[cnt] = (unsigned short)nzb;
     __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    short *map = pmap + wave * bs;
     for (int l = blev[b]; l < blev[b + 1]; ++l) {
         for (int t = lptr[l] + wave; t < lptr[l + 1]; t += nwaves) {
             const int r = rows[t];
             const int s0 = rp[r], len = rp[r + 1] - s0, d = dg[r] - s0;
-            T my = lane < len ? val[s0 + lane] : T(0);
-            const int mycol = lane < len ? (int)col[s0 + lane] : -1;
+            if (lane < len) map[col[s0 + lane]] = (short)lane;
+            wave_lds_sync();
             for (int kk = 0; kk < d; ++kk) {
-                const int k = __shfl(mycol, kk, 64);
-                const T lik = __shfl(my, kk, 64) * val[dg[k]];
-                if (lane == kk) my = lik;
-                const int ks = rp[k], klen = rp[k + 1] - ks;
-                const int kc = lane < klen ? (int)col[ks + lane] : -2;
-                const T kv = lane < klen ? val[ks + lane] : T(0);
-                for (int m = 0; m < klen; ++m) {
-                    const int c = __shfl(kc, m, 64);
-                    const T v = __shfl(kv, m, 64);
-                    if (lane > kk && c == mycol) my -= lik * v;
+                const int k = col[s0 + kk];
+                const T lik = val[s0 + kk] * val[dg[k]];  // u_kk is stored inverted
+                const int ks = dg[k] + 1, ke = rp[k + 1];
+                for (int m = ks + lane; m < ke; m += 64) {
+                    const int p = map[col[m]];
+                    if (p > kk) val[s0 + p] -= lik * val[m];
                 }
+                wave_lds_sync();
+                if (lane == 0) val[s0 + kk] = lik;
+                wave_lds_sync();
             }
-            if (lane == d) {
-                if (my == T(0)) { atomicOr(flag, 1); my = T(0); }
-                else my = T(1) / my;
+            if (lane == 0) {
+                const T piv = val[s0 + d];
+                if (piv == T(0)) { atomicOr(flag, 1); val[s0 + d] = T(0); }
+                else val[s0 + d] = T(1) / piv;
             }
-            if (lane < len) val[s0 + lane] = my;
+            if (lane < len) map[col[s0 + lane]] = -1;
+            wave_lds_sync();
         }
         __syncthreads();
     }
     for (int k = threadIdx.x; k < nzb; k += blockDim.x) lu[z0 + k] = val[k];
 }
 
-// Solve: the block's factor (values, local columns, row pointers, diagonal
-// offsets) and the right-hand side are staged in LDS with coalesced loads, then
-// the L and U level sets are walked with one thread per row.  T is the factor's
-// arithmetic type (double, or float for the FP32 preconditioner); x and y stay FP64.
+// Solve: one wavefront per block.  Only the block's solution lives in LDS (so
+// that every block of the matrix is resident at once and the CU interleaves
+// their dependency chains); the factor streams from memory.  Levels are walked
+// with a thread per row; the next level's row (index, bounds and first kBjRow
+// entries) is loaded before the current level's barrier so that only the LDS
+// reads of the solution and the FMA chain sit on the critical path.  The
+// entries of a row are subtracted in column order, as the sequential solve
+// does.  T is the factor's arithmetic type (double, or float for the FP32
+// preconditioner); x and y stay FP64.
+constexpr int kBjRow = 8;
+
 template <typename T>
-__global__ void __launch_bounds__(256) bj_solve(int n, int bs, int max_nnz, const int *__restrict__ ia,
-                                                const unsigned short *__restrict__ ja, const int *__restrict__ diag,
-                                                const T *__restrict__ lu, const int *__restrict__ blev_l,
-                                                const int *__restrict__ lptr_l, const unsigned short *__restrict__ rows_l,
-                                                const int *__restrict__ blev_u, const int *__restrict__ lptr_u,
-                                                const unsigned short *__restrict__ rows_u, const double *__restrict__ x,
-                                                double *__restrict__ y) {
+struct BjRow {
+    int r = -1, k0 = 0, k1 = 0;
+    T a[kBjRow];
+    int c[kBjRow];
+};
+
+template <typename T, bool UPPER>
+__device__ __forceinline__ void bj_load_row(BjRow<T> &w, int t, int p0, int cnt, int base,
+                                            const unsigned short *__restrict__ rows, const int *__restrict__ ia,
+                                            const int *__restrict__ diag, const unsigned short *__restrict__ ja,
+                                            const T *__restrict__ lu) {
+    w.r = -1;
+    if (t >= cnt) return;
+    w.r = rows[p0 + t];
+    const int gi = base + w.r;
+    w.k0 = UPPER ? diag[gi] + 1 : ia[gi];
+    w.k1 = UPPER ? ia[gi + 1] : diag[gi];
+#pragma unroll
+    for (int u = 0; u < kBjRow; ++u) {
+        const int k = w.k0 + u;
+        const bool ok = k < w.k1;
+        w.a[u] = ok ? lu[k] : T(0);
+        w.c[u] = ok ? (int)ja[k] : w.r;  // padding: 0 * v[r]
+    }
+}
+
+template <typename T, bool UPPER>
+__device__ __forceinline__ void bj_row_update(const BjRow<T> &w, T *v, int base, const int *__restrict__ diag,
+                                              const unsigned short *__restrict__ ja, const T *__restrict__ lu) {
+    T s = v[w.r];
+#pragma unroll
+    for (int u = 0; u < kBjRow; ++u) s -= w.a[u] * v[w.c[u]];
+    for (int k = w.k0 + kBjRow; k < w.k1; ++k) s -= lu[k] * v[ja[k]];
+    if (UPPER) s *= lu[diag[base + w.r]];  // inverted diagonal
+    v[w.r] = s;
+}
+
+template <typename T, bool UPPER>
+__device__ __forceinline__ void bj_sweep(T *v, int base, int l0, int l1, const int *__restrict__ lptr,
+                                         const unsigned short *__restrict__ rows, const int *__restrict__ ia,
+                                         const int *__restrict__ diag, const unsigned short *__restrict__ ja,
+                                         const T *__restrict__ lu) {
+    const int t = threadIdx.x;
+    BjRow<T> nxt;
+    if (l0 < l1) bj_load_row<T, UPPER>(nxt, t, lptr[l0], lptr[l0 + 1] - lptr[l0], base, rows, ia, diag, ja, lu);
+    for (int l = l0; l < l1; ++l) {
+        const BjRow<T> cur = nxt;
+        const int p0 = lptr[l], cnt = lptr[l + 1] - p0;
+        if (l + 1 < l1)
+            bj_load_row<T, UPPER>(nxt, t, lptr[l + 1], lptr[l + 2] - lptr[l + 1], base, rows, ia, diag, ja, lu);
+        if (cur.r >= 0) bj_row_update<T, UPPER>(cur, v, base, diag, ja, lu);
+        for (int tt = t + 64; tt < cnt; tt += 64) {  // levels wider than a wavefront
+            BjRow<T> w;
+            bj_load_row<T, UPPER>(w, tt, p0, cnt, base, rows, ia, diag, ja, lu);
+            bj_row_update<T, UPPER>(w, v, base, diag, ja, lu);
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) bj_solve(int n, int bs, const int *__restrict__ ia,
+                                               const unsigned short *__restrict__ ja, const int *__restrict__ diag,
+                                               const T *__restrict__ lu, const int *__restrict__ blev_l,
+                                               const int *__restrict__ lptr_l, const unsigned short *__restrict__ rows_l,
+                                               const int *__restrict__ blev_u, const int *__restrict__ lptr_u,
+                                               const unsigned short *__restrict__ rows_u, const double *__restrict__ x,
+                                               double *__restrict__ y) {
     extern __shared__ double smem[];
-    T *val = (T *)smem;
-    T *v = val + max_nnz;
-    unsigned short *col = (unsigned short *)(v + bs);
-    unsigned short *rp = col + max_nnz;
-    unsigned short *dg = rp + (bs + 1);
+    T *v = (T *)smem;
     const int b = blockIdx.x;
     const int base = b * bs;
     const int cnt = min(bs, n - base);
-    const int z0 = ia[base], nzb = ia[base + cnt] - z0;
-    for (int k = threadIdx.x; k < nzb; k += blockDim.x) {
-        val[k] = lu[z0 + k];
-        col[k] = ja[z0 + k];
-    }
-    for (int r = threadIdx.x; r < cnt; r += blockDim.x) {
-        rp[r] = (unsigned short)(ia[base + r] - z0);
-        dg[r] = (unsigned short)(diag[base + r] - z0);
-        v[r] = (T)x[base + r];
-    }
-    if (threadIdx.x == 0) rp[cnt] = (unsigned short)nzb;
+    for (int r = threadIdx.x; r < cnt; r += blockDim.x) v[r] = (T)x[base + r];
     __syncthreads();
-    for (int l = blev_l[b]; l < blev_l[b + 1]; ++l) {
-        for (int t = lptr_l[l] + threadIdx.x; t < lptr_l[l + 1]; t += blockDim.x) {
-            const int r = rows_l[t];
-            T s = v[r];
-            for (int k = rp[r]; k < dg[r]; ++k) s -= val[k] * v[col[k]];
-            v[r] = s;
-        }
-        __syncthreads();
-    }
-    for (int l = blev_u[b]; l < blev_u[b + 1]; ++l) {
-        for (int t = lptr_u[l] + threadIdx.x; t < lptr_u[l + 1]; t += blockDim.x) {
-            const int r = rows_u[t];
-            T s = v[r];
-            for (int k = dg[r] + 1; k < rp[r + 1]; ++k) s -= val[k] * v[col[k]];
-            v[r] = s * val[dg[r]];
-        }
-        __syncthreads();
-    }
+    bj_sweep<T, false>(v, base, blev_l[b], blev_l[b + 1], lptr_l, rows_l, ia, diag, ja, lu);
+    bj_sweep<T, true>(v, base, blev_u[b], blev_u[b + 1], lptr_u, rows_u, ia, diag, ja, lu);
     for (int r = threadIdx.x; r < cnt; r += blockDim.x) y[base + r] = (double)v[r];
 }
 
@@ -663,7 +713,8 @@ static int build_levels(fvp_csr *A, const std::vector<int> &diag) {
 
 // LDS bytes of one block's factor staging: values, local columns, row pointers, diagonal offsets
 static size_t bj_lds(int maxnz, size_t value_bytes, int bs) {
-    return (size_t)maxnz * (value_bytes + 2) + (size_t)(2 * bs + 1) * 2 + 8;
+    return (size_t)maxnz * (value_bytes + 2) + (size_t)(2 * bs + 1) * 2 + 8 +
+           (size_t)(kBjFactorThreads / 64) * bs * sizeof(short);
 }
 
 // block-local CSR and the per-block level sets of L and U
@@ -767,7 +818,7 @@ static int precon_build_impl(fvp_csr *A, int type) {
             if (rc) { A->ptype = -1; return rc; }
         }
         const size_t vb = type == FVP_PRECON_BJILU0 ? sizeof(double) : sizeof(float);
-        const size_t need = bj_lds(A->b_maxnz, vb, A->bs) + vb * A->bs;
+        const size_t need = bj_lds(A->b_maxnz, vb, A->bs);  // the factorisation's staging
         if (need > (size_t)A->lds_max) {
             set_error("block-Jacobi ILU(0): a block of " + std::to_string(A->bs) + " rows needs " +
                       std::to_string(need) + " B of LDS, the device allows " + std::to_string(A->lds_max));
@@ -775,12 +826,12 @@ static int precon_build_impl(fvp_csr *A, int type) {
             return -1;
         }
         if (type == FVP_PRECON_BJILU0)
-            bj_factor<double><<<A->nblk, 1024, A->b_lds, s>>>(n, A->bs, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p,
+            bj_factor<double><<<A->nblk, kBjFactorThreads, A->b_lds, s>>>(n, A->bs, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p,
                                                               A->b_src.p, A->a.p, A->b_lev_l.p, A->b_lptr_l.p,
                                                               A->b_rows_l.p, A->b_lu.p, A->flag.p);
         else {
             if (!A->b_lu32.p && A->b_lu32.alloc(A->b_lu.n)) return -1;
-            bj_factor<float><<<A->nblk, 1024, bj_lds(A->b_maxnz, sizeof(float), A->bs), s>>>(
+            bj_factor<float><<<A->nblk, kBjFactorThreads, bj_lds(A->b_maxnz, sizeof(float), A->bs), s>>>(
                 n, A->bs, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p, A->b_lev_l.p, A->b_lptr_l.p,
                 A->b_rows_l.p, A->b_lu32.p, A->flag.p);
         }
@@ -827,13 +878,13 @@ static int precon_apply_impl(fvp_csr *A, const double *x, double *y) {
         vmul<<<nblk(n, 256), 256, 0, s>>>(n, x, A->jac.p, y);
         break;
     case FVP_PRECON_BJILU0:
-        bj_solve<double><<<A->nblk, 256, A->b_lds + sizeof(double) * A->bs, s>>>(
-            n, A->bs, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_lu.p, A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p,
+        bj_solve<double><<<A->nblk, 64, sizeof(double) * A->bs, s>>>(
+            n, A->bs, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_lu.p, A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p,
             A->b_lev_u.p, A->b_lptr_u.p, A->b_rows_u.p, x, y);
         break;
     case FVP_PRECON_BJILU0_F32:
-        bj_solve<float><<<A->nblk, 256, bj_lds(A->b_maxnz, sizeof(float), A->bs) + sizeof(float) * A->bs, s>>>(
-            n, A->bs, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_lu32.p, A->b_lev_l.p, A->b_lptr_l.p,
+        bj_solve<float><<<A->nblk, 64, sizeof(float) * A->bs, s>>>(
+            n, A->bs, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_lu32.p, A->b_lev_l.p, A->b_lptr_l.p,
             A->b_rows_l.p, A->b_lev_u.p, A->b_lptr_u.p, A->b_rows_u.p, x, y);
         break;
     case FVP_PRECON_ILU0: {
