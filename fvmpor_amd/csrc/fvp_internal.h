@@ -85,10 +85,8 @@ struct fvp_ctx {
     // time-step upwinding
     int averaging = 1;             // 1: all edge weights 0.5
     fvp::DevBuf<int> eedge;        // T x ned global edge of each local edge
-    fvp::DevBuf<int> slot_ptr;     // NE + 1
-    fvp::DevBuf<int> slot;         // T x ned (e*ned + j) sorted by (edge, element)
-    fvp::DevBuf<double> qface;     // T x ned averaged q.n per (element, local edge), back->front
-    fvp::DevBuf<double> aface;     // T x ned CV face area
+    fvp::DevBuf<double> eq;        // NE: averaged q.n summed over the edge's CV faces, back->front
+    fvp::DevBuf<double> ea;        // NE: summed CV face area
     fvp::DevBuf<double> eflux;     // NE
     fvp::DevBuf<double> wf;        // NE front weight
     // scratch for Jv / reductions

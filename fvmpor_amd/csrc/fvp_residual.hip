@@ -87,8 +87,9 @@ __device__ constexpr int edge_b(int j) { return j < 3 ? (j + 1) % 3 : 3; }
 // Face fluxes of one element, summed per element node (f[k] = net inflow into
 // node k's control volume, rho0 k_r q.S over the element's CV faces).
 // MODE 0: upwind k_r from the element's mask bits; MODE 1: averaged k_r
-// (0.5 front + 0.5 back) and the averaged face fluxes are also written per
-// (element, local edge) in back->front orientation for the edge-flux sign test.
+// (0.5 front + 0.5 back); MODE 2: averaged k_r, and instead of node sums the
+// face flux (back -> front) and face area are added to per-edge totals
+// eq / ea (eedge: global edge of each local edge) for the edge-flux sign test.
 // zk: per-zone K_xx, K_yy, K_zz in LDS.  JV: the state is psi + sig v.
 template <int DIM, int MODE, bool JV = false>
 __device__ __forceinline__ void element_fluxes(int e, int N, const double *zk, double rho0,
@@ -97,8 +98,9 @@ __device__ __forceinline__ void element_fluxes(int e, int N, const double *zk, d
                                                const uint8_t *__restrict__ eorient,
                                                const double *__restrict__ xyz, const double *__restrict__ psi,
                                                const double *__restrict__ v, double sig,
-                                               const double *__restrict__ kr, double *__restrict__ qface,
-                                               double *__restrict__ aface, int n[4], double f[4]) {
+                                               const double *__restrict__ kr, double *__restrict__ eq,
+                                               double *__restrict__ ea, const int *__restrict__ eedge, int n[4],
+                                               double f[4]) {
     constexpr int NPE = DIM + 1;
     constexpr int NED = DIM == 2 ? 3 : 6;
     int4 c4 = conn[e];
@@ -152,7 +154,7 @@ __device__ __forceinline__ void element_fluxes(int e, int N, const double *zk, d
 #pragma unroll
     for (int k = 0; k < 4; ++k) f[k] = 0.;
     const unsigned mask = MODE == 0 ? emask[e] : 0u;
-    const unsigned orient = MODE == 1 ? eorient[e] : 0u;
+    const unsigned orient = MODE != 0 ? eorient[e] : 0u;
 #pragma unroll
     for (int j = 0; j < NED; ++j) {
         const int a = edge_a(j);
@@ -168,10 +170,13 @@ __device__ __forceinline__ void element_fluxes(int e, int N, const double *zk, d
             double kb = kr[(size_t)z * N + (a_front ? n[b] : n[a])];
             krf = kf * 0.5;
             krf += kb * 0.5;
-            double qn = krf * (a_front ? -flux : flux);  // back -> front
-            qface[(size_t)e * NED + j] = qn;
-            double sx = G[b][0] - G[a][0], sy = G[b][1] - G[a][1], sz = G[b][2] - G[a][2];
-            aface[(size_t)e * NED + j] = fabs(sc) * sqrt(sx * sx + sy * sy + sz * sz);
+            if (MODE == 2) {
+                const double qn = krf * (a_front ? -flux : flux);  // back -> front
+                const double sx = G[b][0] - G[a][0], sy = G[b][1] - G[a][1], sz = G[b][2] - G[a][2];
+                const int ed = eedge[(size_t)e * NED + j];
+                atomicAdd(&eq[ed], qn);
+                atomicAdd(&ea[ed], fabs(sc) * sqrt(sx * sx + sy * sy + sz * sz));
+            }
         }
         double M = rho0 * (krf * flux);
         f[a] -= M;
@@ -179,7 +184,8 @@ __device__ __forceinline__ void element_fluxes(int e, int N, const double *zk, d
     }
 }
 
-// Atomic scatter variant (averaged weights in begin_step).
+// Atomic scatter variant: the averaged-weight residual (MODE 1, before the
+// first begin_step) and begin_step's per-edge face sums (MODE 2).
 template <int DIM, int MODE>
 __global__ void __launch_bounds__(256) elem_kernel(int T, int N, ZoneTable zt, double rho0,
                                                    const int4 *__restrict__ conn, const uint8_t *__restrict__ ezone,
@@ -187,7 +193,8 @@ __global__ void __launch_bounds__(256) elem_kernel(int T, int N, ZoneTable zt, d
                                                    const uint8_t *__restrict__ eorient,
                                                    const double *__restrict__ xyz, const double *__restrict__ psi,
                                                    const double *__restrict__ kr, double *__restrict__ acc,
-                                                   double *__restrict__ qface, double *__restrict__ aface) {
+                                                   double *__restrict__ eq, double *__restrict__ ea,
+                                                   const int *__restrict__ eedge) {
     __shared__ double zk[3 * kMaxZones];
     if (threadIdx.x < 3 * zt.nz) zk[threadIdx.x] = zt.p[threadIdx.x / 3][ZK_X + threadIdx.x % 3];
     __syncthreads();
@@ -195,10 +202,12 @@ __global__ void __launch_bounds__(256) elem_kernel(int T, int N, ZoneTable zt, d
     if (e >= T) return;
     int n[4];
     double f[4];
-    element_fluxes<DIM, MODE>(e, N, zk, rho0, conn, ezone, kmul, emask, eorient, xyz, psi, nullptr, 0., kr, qface,
-                              aface, n, f);
+    element_fluxes<DIM, MODE>(e, N, zk, rho0, conn, ezone, kmul, emask, eorient, xyz, psi, nullptr, 0., kr, eq, ea,
+                              eedge, n, f);
+    if (MODE != 2) {
 #pragma unroll
-    for (int k = 0; k < DIM + 1; ++k) atomicAdd(&acc[n[k]], f[k]);
+        for (int k = 0; k < DIM + 1; ++k) atomicAdd(&acc[n[k]], f[k]);
+    }
 }
 
 // Element-owner residual: a workgroup owns the nodes of one tile and evaluates
@@ -228,7 +237,7 @@ __global__ void __launch_bounds__(256) owner_kernel(
         int n[4];
         double f[4];
         element_fluxes<DIM, 0, JV>(e, N, zk, rho0, conn, ezone, kmul, emask, nullptr, xyz, psi, jv.v, sig, kr,
-                                   nullptr, nullptr, n, f);
+                                   nullptr, nullptr, nullptr, n, f);
 #pragma unroll
         for (int q = 0; q < DIM + 1; ++q) {
             if (n[q] >= lo && n[q] < hi) atomicAdd(&sm[n[q] - lo], f[q]);
@@ -424,16 +433,11 @@ __global__ void dirichlet_kernel(int nd, const int *__restrict__ dnode, const do
 
 // edge flux = sum over the edge's CV faces of q.n / sum of their areas
 // (flux_lim_matrix, fvmpor.h:1033-1071), then the upwind test (fvmpor.h:783-794)
-__global__ void edge_flux_kernel(int NE, const int *__restrict__ slot_ptr, const int *__restrict__ slot,
-                                 const double *__restrict__ qface, const double *__restrict__ aface,
+__global__ void edge_flux_kernel(int NE, const double *__restrict__ eq, const double *__restrict__ ea,
                                  double *__restrict__ eflux, double *__restrict__ wf) {
     int ed = blockIdx.x * blockDim.x + threadIdx.x;
     if (ed >= NE) return;
-    double tot = 0.;
-    for (int s = slot_ptr[ed]; s < slot_ptr[ed + 1]; ++s) tot += aface[slot[s]];
-    double w = 1. / tot;
-    double sum = 0.;
-    for (int s = slot_ptr[ed]; s < slot_ptr[ed + 1]; ++s) sum += w * qface[slot[s]];
+    const double sum = (1. / ea[ed]) * eq[ed];
     eflux[ed] = sum;
     wf[ed] = sum < 0. ? 1. : 0.;
 }
@@ -538,11 +542,11 @@ int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, d
         if (c->dim == 3)
             elem_kernel<3, 1><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
                                                            c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p,
-                                                           c->acc.p, c->qface.p, c->aface.p);
+                                                           c->acc.p, nullptr, nullptr, nullptr);
         else
             elem_kernel<2, 1><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
                                                            c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p,
-                                                           c->acc.p, c->qface.p, c->aface.p);
+                                                           c->acc.p, nullptr, nullptr, nullptr);
         if (c->nb) boundary_kernel<<<blocks(c->nb, 256), 256, 0, s>>>(c->nb, c->bnode.p, c->bval.p, c->acc.p);
         finalize_kernel<<<blocks(N, 256), 256, 0, s>>>(N, c->acc.p, c->invV.p, psip, c->cap.p, F);
         if (c->ndir) dirichlet_kernel<<<blocks(c->ndir, 256), 256, 0, s>>>(c->ndir, c->dnode.p, c->dval.p, psi, F);
@@ -605,10 +609,22 @@ extern "C" int fvp_begin_step(fvp_ctx *c, double t, const double *psi, const dou
     if (!c || !psi || !psip) { set_error("null argument"); return -1; }
     FVP_HIP(hipSetDevice(c->device));
     hipStream_t s = c->stream;
-    // averaged-weight evaluation writes the per-face fluxes; its F is discarded
-    if (residual_impl(c, t, psi, psip, c->tmp3.p, 1)) return -1;
-    edge_flux_kernel<<<blocks(c->NE, 256), 256, 0, s>>>(c->NE, c->slot_ptr.p, c->slot.p, c->qface.p, c->aface.p,
-                                                        c->eflux.p, c->wf.p);
+    (void)t; (void)psip;
+    const int N = c->N, T = c->T;
+    // k_r at the state, then the averaged-weight face fluxes summed per edge
+    vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p, c->cap.p,
+                                                     nullptr, nullptr);
+    FVP_HIP(hipMemsetAsync(c->eq.p, 0, sizeof(double) * c->NE, s));
+    FVP_HIP(hipMemsetAsync(c->ea.p, 0, sizeof(double) * c->NE, s));
+    if (c->dim == 3)
+        elem_kernel<3, 2><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
+                                                       c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p, nullptr,
+                                                       c->eq.p, c->ea.p, c->eedge.p);
+    else
+        elem_kernel<2, 2><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
+                                                       c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p, nullptr,
+                                                       c->eq.p, c->ea.p, c->eedge.p);
+    edge_flux_kernel<<<blocks(c->NE, 256), 256, 0, s>>>(c->NE, c->eq.p, c->ea.p, c->eflux.p, c->wf.p);
     if (c->dim == 3)
         mask_kernel<3><<<blocks(c->T, 256), 256, 0, s>>>(c->T, c->eedge.p, c->eorient.p, c->wf.p, c->emask.p);
     else

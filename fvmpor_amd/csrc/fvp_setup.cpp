@@ -356,45 +356,34 @@ extern "C" int fvp_create(const fvp_problem *p, int device, fvp_ctx **out) {
         std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
             return key[a] != key[b] ? key[a] < key[b] : a < b;
         });
-        std::vector<int> eedge(ns), slot(ns), slot_ptr;
+        std::vector<int> eedge(ns);
         std::vector<uint8_t> orient(T, 0);
-        slot_ptr.reserve(ns / 4 + 2);
         int ne = -1;
         int front = -1;
         for (size_t r = 0; r < ns; ++r) {
             uint32_t s = order[r];
             if (r == 0 || key[s] != key[order[r - 1]]) {
                 ++ne;
-                slot_ptr.push_back((int)r);
                 int e = (int)(s / ned), j = (int)(s % ned);
                 front = p->conn[(size_t)e * npe + tab[j][0]];  // first inserter's first node
             }
             int e = (int)(s / ned), j = (int)(s % ned);
             eedge[s] = ne;
-            slot[r] = (int)s;
             if (p->conn[(size_t)e * npe + tab[j][0]] == front) orient[e] |= (uint8_t)(1u << j);
         }
-        slot_ptr.push_back((int)ns);
         c->NE = ne + 1;
-        // move the per-element tables to the sorted element order; slots keep their
-        // (edge, original element) order, the order the reference sums CV faces in
-        std::vector<int> eedge_n(ns), slot_n(ns);
+        // move the per-element tables to the sorted element order
+        std::vector<int> eedge_n(ns);
         std::vector<uint8_t> orient_n(T);
         for (int e = 0; e < T; ++e) {
             const int o = eperm[e];
             orient_n[e] = orient[o];
             for (int j = 0; j < ned; ++j) eedge_n[(size_t)e * ned + j] = eedge[(size_t)o * ned + j];
         }
-        for (size_t r = 0; r < ns; ++r) {
-            int o = slot[r] / ned, j = slot[r] % ned;
-            slot_n[r] = einv[o] * ned + j;
-        }
-        if (c->eedge.upload(eedge_n.data(), ns) || c->slot.upload(slot_n.data(), ns) ||
-            c->slot_ptr.upload(slot_ptr.data(), slot_ptr.size()) || c->eorient.upload(orient_n.data(), T))
-            return fail();
+        if (c->eedge.upload(eedge_n.data(), ns) || c->eorient.upload(orient_n.data(), T)) return fail();
     }
-    if (c->emask.alloc(T) || c->qface.alloc((size_t)T * ned) || c->aface.alloc((size_t)T * ned) ||
-        c->eflux.alloc(c->NE) || c->wf.alloc(c->NE))
+    if (c->emask.alloc(T) || c->eq.alloc(c->NE) || c->ea.alloc(c->NE) || c->eflux.alloc(c->NE) ||
+        c->wf.alloc(c->NE))
         return fail();
     FVP_HIP(hipMemset(c->emask.p, 0, T));
     FVP_HIP(hipMemset(c->eflux.p, 0, sizeof(double) * c->NE));
@@ -539,8 +528,8 @@ extern "C" int fvp_destroy(fvp_ctx *c) {
     c->cap.release(); c->acc.release(); c->bnode.release(); c->bval.release(); c->dnode.release();
     c->dval.release(); c->t_wptr.release(); c->t_whalo.release(); c->t_vptr.release(); c->t_lconn.release();
     c->t_tz.release(); c->t_eptr.release(); c->t_hptr.release(); c->t_hlist.release(); c->t_bptr.release();
-    c->t_dptr.release(); c->eedge.release(); c->slot_ptr.release(); c->slot.release(); c->qface.release();
-    c->aface.release(); c->eflux.release(); c->wf.release(); c->tmp1.release(); c->tmp2.release();
+    c->t_dptr.release(); c->eedge.release(); c->eq.release();
+    c->ea.release(); c->eflux.release(); c->wf.release(); c->tmp1.release(); c->tmp2.release();
     c->tmp3.release(); c->red.release();
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
