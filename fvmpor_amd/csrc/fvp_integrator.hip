@@ -486,13 +486,15 @@ extern "C" int fvp_integrator_advance(fvp_integrator *I, double t_out, double *p
 
 extern "C" int fvp_integrator_set_segments(fvp_integrator *I, int nseg, const int *offsets) {
     if (!I || (nseg > 0 && !offsets)) { set_error("null argument"); return -1; }
-    if (nseg <= 1) { I->nseg = 0; return 0; }
+    if (nseg < 0) { set_error("nseg must be >= 0"); return -1; }
+    if (nseg == 0) { I->nseg = 0; return 0; }
     if (offsets[0] != 0 || offsets[nseg] != I->N) { set_error("segments must start at 0 and end at n_nodes"); return -1; }
     int mn = I->N;
     for (int g = 0; g < nseg; ++g) {
         if (offsets[g + 1] <= offsets[g]) { set_error("segment offsets must increase"); return -1; }
         mn = std::min(mn, offsets[g + 1] - offsets[g]);
     }
+    if (nseg == 1) { I->nseg = 0; return 0; }  // one instance: the plain norms
     FVP_HIP(hipSetDevice(I->c->device));
     if (I->seg.upload(offsets, nseg + 1) || I->seg_sum.alloc(nseg)) return -1;
     I->nseg = nseg;

@@ -78,6 +78,12 @@ struct fvp_ctx {
     int rvariant = FVP_RESIDUAL_OWNER;  // residual kernel variant
     long tile_elems = 0;            // element visits per residual (primary + halo)
     fvp::DevBuf<int> t_eptr, t_hptr, t_hlist, t_bptr, t_dptr;
+    // element-owner variant: per element its first export slot relative to the
+    // tile's (t_xptr), per node the export slots it gathers (CSR), the slots
+    fvp::DevBuf<unsigned short> t_xoff;
+    fvp::DevBuf<int> t_xptr, x_iptr, x_ilist;
+    fvp::DevBuf<double> xf;
+    long nexport = 0;
     fvp::DevBuf<int> t_wptr, t_whalo, t_vptr, t_tz;  // window halo nodes, visits, zones per tile
     fvp::DevBuf<unsigned short> t_lconn;             // 4 window-local node indices per visit
     int win_max = 0, ntz_max = 1;
@@ -85,8 +91,10 @@ struct fvp_ctx {
     // time-step upwinding
     int averaging = 1;             // 1: all edge weights 0.5
     fvp::DevBuf<int> eedge;        // T x ned global edge of each local edge
-    fvp::DevBuf<double> eq;        // NE: averaged q.n summed over the edge's CV faces, back->front
-    fvp::DevBuf<double> ea;        // NE: summed CV face area
+    fvp::DevBuf<int> slot_ptr;     // NE + 1
+    fvp::DevBuf<int> slot;         // T x ned (e*ned + j) sorted by (edge, element)
+    fvp::DevBuf<double> qface;     // T x ned averaged q.n per (element, local edge), back->front
+    fvp::DevBuf<double> ea;        // NE: summed area of the edge's CV faces (geometry, set-up)
     fvp::DevBuf<double> eflux;     // NE
     fvp::DevBuf<double> wf;        // NE front weight
     // scratch for Jv / reductions
@@ -112,6 +120,7 @@ struct JvArgs {
     double cj = 0.;
 };
 int build_tiles(fvp_ctx *c, int tile_nodes);
+int edge_areas(fvp_ctx *c);
 int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, double *F, int averaging,
                   const JvArgs *jv = nullptr);
 int tile_kernel_attrs(fvp_ctx *c);

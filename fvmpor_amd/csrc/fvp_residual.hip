@@ -88,8 +88,8 @@ __device__ constexpr int edge_b(int j) { return j < 3 ? (j + 1) % 3 : 3; }
 // node k's control volume, rho0 k_r q.S over the element's CV faces).
 // MODE 0: upwind k_r from the element's mask bits; MODE 1: averaged k_r
 // (0.5 front + 0.5 back); MODE 2: averaged k_r, and instead of node sums the
-// face flux (back -> front) and face area are added to per-edge totals
-// eq / ea (eedge: global edge of each local edge) for the edge-flux sign test.
+// face fluxes (back -> front) are stored per (element, local edge) in qface
+// for the edge-flux sign test.
 // zk: per-zone K_xx, K_yy, K_zz in LDS.  JV: the state is psi + sig v.
 template <int DIM, int MODE, bool JV = false>
 __device__ __forceinline__ void element_fluxes(int e, int N, const double *zk, double rho0,
@@ -98,9 +98,8 @@ __device__ __forceinline__ void element_fluxes(int e, int N, const double *zk, d
                                                const uint8_t *__restrict__ eorient,
                                                const double *__restrict__ xyz, const double *__restrict__ psi,
                                                const double *__restrict__ v, double sig,
-                                               const double *__restrict__ kr, double *__restrict__ eq,
-                                               double *__restrict__ ea, const int *__restrict__ eedge, int n[4],
-                                               double f[4]) {
+                                               const double *__restrict__ kr, double *__restrict__ qface,
+                                               int n[4], double f[4]) {
     constexpr int NPE = DIM + 1;
     constexpr int NED = DIM == 2 ? 3 : 6;
     int4 c4 = conn[e];
@@ -170,13 +169,7 @@ __device__ __forceinline__ void element_fluxes(int e, int N, const double *zk, d
             double kb = kr[(size_t)z * N + (a_front ? n[b] : n[a])];
             krf = kf * 0.5;
             krf += kb * 0.5;
-            if (MODE == 2) {
-                const double qn = krf * (a_front ? -flux : flux);  // back -> front
-                const double sx = G[b][0] - G[a][0], sy = G[b][1] - G[a][1], sz = G[b][2] - G[a][2];
-                const int ed = eedge[(size_t)e * NED + j];
-                atomicAdd(&eq[ed], qn);
-                atomicAdd(&ea[ed], fabs(sc) * sqrt(sx * sx + sy * sy + sz * sz));
-            }
+            if (MODE == 2) qface[(size_t)e * NED + j] = krf * (a_front ? -flux : flux);  // back -> front
         }
         double M = rho0 * (krf * flux);
         f[a] -= M;
@@ -193,8 +186,7 @@ __global__ void __launch_bounds__(256) elem_kernel(int T, int N, ZoneTable zt, d
                                                    const uint8_t *__restrict__ eorient,
                                                    const double *__restrict__ xyz, const double *__restrict__ psi,
                                                    const double *__restrict__ kr, double *__restrict__ acc,
-                                                   double *__restrict__ eq, double *__restrict__ ea,
-                                                   const int *__restrict__ eedge) {
+                                                   double *__restrict__ qface) {
     __shared__ double zk[3 * kMaxZones];
     if (threadIdx.x < 3 * zt.nz) zk[threadIdx.x] = zt.p[threadIdx.x / 3][ZK_X + threadIdx.x % 3];
     __syncthreads();
@@ -202,8 +194,8 @@ __global__ void __launch_bounds__(256) elem_kernel(int T, int N, ZoneTable zt, d
     if (e >= T) return;
     int n[4];
     double f[4];
-    element_fluxes<DIM, MODE>(e, N, zk, rho0, conn, ezone, kmul, emask, eorient, xyz, psi, nullptr, 0., kr, eq, ea,
-                              eedge, n, f);
+    element_fluxes<DIM, MODE>(e, N, zk, rho0, conn, ezone, kmul, emask, eorient, xyz, psi, nullptr, 0., kr, qface,
+                              n, f);
     if (MODE != 2) {
 #pragma unroll
         for (int k = 0; k < DIM + 1; ++k) atomicAdd(&acc[n[k]], f[k]);
@@ -214,16 +206,18 @@ __global__ void __launch_bounds__(256) elem_kernel(int T, int N, ZoneTable zt, d
 // the tile's primary elements (lowest node in the tile; every element has
 // exactly one primary tile, so each element is evaluated once), gathering node
 // data through the caches.  Contributions to owned nodes accumulate in LDS
-// (ds_add_f64); those to nodes of other tiles go to the global accumulator with
-// hardware FP64 atomics, as do the owned sums at the end (after the tile's
-// boundary fluxes).  The finalisation is a separate node pass.
+// (ds_add_f64) and are stored, with the tile's boundary fluxes, to acc; those
+// to nodes of other tiles are stored to export slots (the element's first
+// slot: tile base + xoff[e], then one per foreign node in local order), which
+// the finalisation gathers per node.  No global atomics.
 template <int DIM, bool JV>
 __global__ void __launch_bounds__(256) owner_kernel(
     int N, int ntiles, int tile_nodes, int per_xcd, ZoneTable zt, double rho0, const int4 *__restrict__ conn,
     const uint8_t *__restrict__ ezone, const double *__restrict__ kmul, const uint8_t *__restrict__ emask,
     const double *__restrict__ xyz, const double *__restrict__ psi, const double *__restrict__ kr,
     const int *__restrict__ eptr, const int *__restrict__ bptr, const int *__restrict__ bnode,
-    const double *__restrict__ bval, double *__restrict__ acc_g, JvArgs jv) {
+    const double *__restrict__ bval, const unsigned short *__restrict__ xoff, const int *__restrict__ xptr,
+    double *__restrict__ xf, double *__restrict__ acc_g, JvArgs jv) {
     extern __shared__ double sm[];
     const int tile = per_xcd ? (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3) : blockIdx.x;
     if (tile >= ntiles) return;  // uniform for the whole workgroup
@@ -233,21 +227,44 @@ __global__ void __launch_bounds__(256) owner_kernel(
     __shared__ double zk[3 * kMaxZones];
     if (threadIdx.x < 3 * zt.nz) zk[threadIdx.x] = zt.p[threadIdx.x / 3][ZK_X + threadIdx.x % 3];
     __syncthreads();
+    double *xt = xf + xptr[tile];
     for (int e = eptr[tile] + threadIdx.x; e < eptr[tile + 1]; e += blockDim.x) {
         int n[4];
         double f[4];
         element_fluxes<DIM, 0, JV>(e, N, zk, rho0, conn, ezone, kmul, emask, nullptr, xyz, psi, jv.v, sig, kr,
-                                   nullptr, nullptr, nullptr, n, f);
+                                   nullptr, n, f);
+        int x = xoff[e];
 #pragma unroll
         for (int q = 0; q < DIM + 1; ++q) {
             if (n[q] >= lo && n[q] < hi) atomicAdd(&sm[n[q] - lo], f[q]);
-            else atomicAdd(&acc_g[n[q]], f[q]);
+            else xt[x++] = f[q];
         }
     }
     __syncthreads();
     for (int k = bptr[tile] + threadIdx.x; k < bptr[tile + 1]; k += blockDim.x) sm[bnode[k] - lo] -= bval[k];
     __syncthreads();
-    for (int i = threadIdx.x; i < nown; i += blockDim.x) atomicAdd(&acc_g[lo + i], sm[i]);
+    for (int i = threadIdx.x; i < nown; i += blockDim.x) acc_g[lo + i] = sm[i];
+}
+
+// Finalisation of the element-owner residual: the node's own sum plus the
+// contributions exported to it, then F = acc / V - psi' ahh (JV: the
+// difference quotient).
+template <bool JV>
+__global__ void owner_finalize(int N, const double *__restrict__ acc, const int *__restrict__ iptr,
+                               const int *__restrict__ ilist, const double *__restrict__ xf,
+                               const double *__restrict__ invV, const double *__restrict__ psip,
+                               const double *__restrict__ cap, double *__restrict__ F, JvArgs jv) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double a = acc[i];
+    for (int k = iptr[i]; k < iptr[i + 1]; ++k) a += xf[ilist[k]];
+    if (JV) {
+        const double sig = jv.sig[0];
+        const double yp = psip[i] + jv.cj * (sig * jv.v[i]);
+        const double F1 = a * invV[i] - yp * cap[i];
+        F[i] = (F1 - jv.F0[i]) / sig;
+    } else
+        F[i] = a * invV[i] - psip[i] * cap[i];
 }
 
 // LDS-staged owner-computes residual.  A workgroup owns tile_nodes nodes; it
@@ -432,14 +449,65 @@ __global__ void dirichlet_kernel(int nd, const int *__restrict__ dnode, const do
 }
 
 // edge flux = sum over the edge's CV faces of q.n / sum of their areas
-// (flux_lim_matrix, fvmpor.h:1033-1071), then the upwind test (fvmpor.h:783-794)
-__global__ void edge_flux_kernel(int NE, const double *__restrict__ eq, const double *__restrict__ ea,
+// (flux_lim_matrix, fvmpor.h:1033-1071; the faces in the reference's (edge,
+// element) order), then the upwind test (fvmpor.h:783-794)
+__global__ void edge_flux_kernel(int NE, const int *__restrict__ slot_ptr, const int *__restrict__ slot,
+                                 const double *__restrict__ qface, const double *__restrict__ ea,
                                  double *__restrict__ eflux, double *__restrict__ wf) {
     int ed = blockIdx.x * blockDim.x + threadIdx.x;
     if (ed >= NE) return;
-    const double sum = (1. / ea[ed]) * eq[ed];
+    const double w = 1. / ea[ed];
+    double sum = 0.;
+    for (int s = slot_ptr[ed]; s < slot_ptr[ed + 1]; ++s) sum += w * qface[slot[s]];
     eflux[ed] = sum;
     wf[ed] = sum < 0. ? 1. : 0.;
+}
+
+// CV face areas |S_ab| = |e| / (d+1) |grad N_b - grad N_a| per (element, local edge)
+template <int DIM>
+__global__ void face_area_kernel(int T, const int4 *__restrict__ conn, const double *__restrict__ xyz,
+                                 double *__restrict__ aface) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= T) return;
+    constexpr int NPE = DIM + 1;
+    constexpr int NED = DIM == 2 ? 3 : 6;
+    const int4 c4 = conn[e];
+    const int n[4] = {c4.x, c4.y, c4.z, c4.w};
+    double px[NPE], py[NPE], pz[NPE];
+#pragma unroll
+    for (int k = 0; k < NPE; ++k) load_pt(xyz, n[k], px[k], py[k], pz[k]);
+    double G[NPE][3];
+    if (DIM == 3) {
+        double ux = px[1] - px[0], uy = py[1] - py[0], uz = pz[1] - pz[0];
+        double vx = px[2] - px[0], vy = py[2] - py[0], vz = pz[2] - pz[0];
+        double wx = px[3] - px[0], wy = py[3] - py[0], wz = pz[3] - pz[0];
+        G[1][0] = vy * wz - vz * wy; G[1][1] = vz * wx - vx * wz; G[1][2] = vx * wy - vy * wx;
+        G[2][0] = wy * uz - wz * uy; G[2][1] = wz * ux - wx * uz; G[2][2] = wx * uy - wy * ux;
+        G[3][0] = uy * vz - uz * vy; G[3][1] = uz * vx - ux * vz; G[3][2] = ux * vy - uy * vx;
+    } else {
+        double ux = px[1] - px[0], uy = py[1] - py[0];
+        double vx = px[2] - px[0], vy = py[2] - py[0];
+        G[1][0] = vy; G[1][1] = -vx; G[1][2] = 0.;
+        G[2][0] = -uy; G[2][1] = ux; G[2][2] = 0.;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) G[0][d] = -(G[1][d] + G[2][d] + (DIM == 3 ? G[NPE - 1][d] : 0.));
+    const double sc = 1.0 / (DIM == 3 ? 24.0 : 6.0);
+#pragma unroll
+    for (int j = 0; j < NED; ++j) {
+        const int a = edge_a(j), b = edge_b(j);
+        const double sx = G[b][0] - G[a][0], sy = G[b][1] - G[a][1], sz = G[b][2] - G[a][2];
+        aface[(size_t)e * NED + j] = sc * sqrt(sx * sx + sy * sy + sz * sz);
+    }
+}
+
+__global__ void edge_area_kernel(int NE, const int *__restrict__ slot_ptr, const int *__restrict__ slot,
+                                 const double *__restrict__ aface, double *__restrict__ ea) {
+    int ed = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ed >= NE) return;
+    double tot = 0.;
+    for (int s = slot_ptr[ed]; s < slot_ptr[ed + 1]; ++s) tot += aface[slot[s]];
+    ea[ed] = tot;
 }
 
 // per-element upwind bits: bit j set when the first local node of edge j is upwind
@@ -490,13 +558,14 @@ static void launch_tile(fvp_ctx *c, const double *psi, const double *psip, doubl
     hipStream_t s = c->stream;
     if (c->rvariant == FVP_RESIDUAL_OWNER) {
         const int N = c->N;
-        (void)hipMemsetAsync(c->acc.p, 0, sizeof(double) * N, s);
         prof_mark(c, 0, 0);
         owner_kernel<DIM, JV><<<c->xcd_map ? per * 8 : c->ntiles, 256, sizeof(double) * c->tile_nodes, s>>>(
             N, c->ntiles, c->tile_nodes, c->xcd_map ? per : 0, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
-            c->emask.p, c->xyz.p, psi, c->kr.p, c->t_eptr.p, c->t_bptr.p, c->bnode.p, c->bval.p, c->acc.p, jv);
+            c->emask.p, c->xyz.p, psi, c->kr.p, c->t_eptr.p, c->t_bptr.p, c->bnode.p, c->bval.p, c->t_xoff.p,
+            c->t_xptr.p, c->xf.p, c->acc.p, jv);
         prof_mark(c, 0, 1);
-        finalize_kernel<JV><<<blocks(N, 256), 256, 0, s>>>(N, c->acc.p, c->invV.p, psip, c->cap.p, F, jv);
+        owner_finalize<JV><<<blocks(N, 256), 256, 0, s>>>(N, c->acc.p, c->x_iptr.p, c->x_ilist.p, c->xf.p, c->invV.p,
+                                                          psip, c->cap.p, F, jv);
         if (c->ndir)
             dirichlet_kernel<JV><<<blocks(c->ndir, 256), 256, 0, s>>>(c->ndir, c->dnode.p, c->dval.p, psi, F, jv);
         return;
@@ -542,17 +611,30 @@ int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, d
         if (c->dim == 3)
             elem_kernel<3, 1><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
                                                            c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p,
-                                                           c->acc.p, nullptr, nullptr, nullptr);
+                                                           c->acc.p, nullptr);
         else
             elem_kernel<2, 1><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
                                                            c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p,
-                                                           c->acc.p, nullptr, nullptr, nullptr);
+                                                           c->acc.p, nullptr);
         if (c->nb) boundary_kernel<<<blocks(c->nb, 256), 256, 0, s>>>(c->nb, c->bnode.p, c->bval.p, c->acc.p);
         finalize_kernel<<<blocks(N, 256), 256, 0, s>>>(N, c->acc.p, c->invV.p, psip, c->cap.p, F);
         if (c->ndir) dirichlet_kernel<<<blocks(c->ndir, 256), 256, 0, s>>>(c->ndir, c->dnode.p, c->dval.p, psi, F);
     }
     prof_mark(c, 1, 1);
     FVP_HIP(hipGetLastError());
+    return 0;
+}
+
+// Set-up: summed CV face area of every edge (geometry only)
+int edge_areas(fvp_ctx *c) {
+    DevBuf<double> aface;
+    if (aface.alloc((size_t)c->T * c->ned)) return -1;
+    if (c->dim == 3) face_area_kernel<3><<<blocks(c->T, 256), 256, 0, c->stream>>>(c->T, c->conn.p, c->xyz.p, aface.p);
+    else face_area_kernel<2><<<blocks(c->T, 256), 256, 0, c->stream>>>(c->T, c->conn.p, c->xyz.p, aface.p);
+    edge_area_kernel<<<blocks(c->NE, 256), 256, 0, c->stream>>>(c->NE, c->slot_ptr.p, c->slot.p, aface.p, c->ea.p);
+    FVP_HIP(hipGetLastError());
+    FVP_HIP(hipStreamSynchronize(c->stream));
+    aface.release();
     return 0;
 }
 
@@ -614,17 +696,16 @@ extern "C" int fvp_begin_step(fvp_ctx *c, double t, const double *psi, const dou
     // k_r at the state, then the averaged-weight face fluxes summed per edge
     vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p, c->cap.p,
                                                      nullptr, nullptr);
-    FVP_HIP(hipMemsetAsync(c->eq.p, 0, sizeof(double) * c->NE, s));
-    FVP_HIP(hipMemsetAsync(c->ea.p, 0, sizeof(double) * c->NE, s));
     if (c->dim == 3)
         elem_kernel<3, 2><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
                                                        c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p, nullptr,
-                                                       c->eq.p, c->ea.p, c->eedge.p);
+                                                       c->qface.p);
     else
         elem_kernel<2, 2><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
                                                        c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p, nullptr,
-                                                       c->eq.p, c->ea.p, c->eedge.p);
-    edge_flux_kernel<<<blocks(c->NE, 256), 256, 0, s>>>(c->NE, c->eq.p, c->ea.p, c->eflux.p, c->wf.p);
+                                                       c->qface.p);
+    edge_flux_kernel<<<blocks(c->NE, 256), 256, 0, s>>>(c->NE, c->slot_ptr.p, c->slot.p, c->qface.p, c->ea.p,
+                                                        c->eflux.p, c->wf.p);
     if (c->dim == 3)
         mask_kernel<3><<<blocks(c->T, 256), 256, 0, s>>>(c->T, c->eedge.p, c->eorient.p, c->wf.p, c->emask.p);
     else

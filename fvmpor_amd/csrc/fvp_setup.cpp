@@ -205,6 +205,39 @@ int build_tiles(fvp_ctx *c, int TILE) {
             c->t_dptr.upload(dptr.data(), nt + 1))
             return -1;
         if (!window) c->t_hlist.release();
+        // element-owner variant: contributions of a tile's elements to nodes of
+        // other tiles are stored in export slots (numbered per tile, element order,
+        // local node order) and gathered by their nodes in the finalisation
+        if (!window) {
+            std::vector<unsigned short> xoff(T);
+            std::vector<int> xptr(nt + 1, 0), target;
+            for (int t = 0; t < nt; ++t) {
+                const int lo = t * TILE, hi = std::min(N, lo + TILE);
+                int run = 0;
+                for (int e = eptr[t]; e < eptr[t + 1]; ++e) {
+                    if (run > 65535) { set_error("a residual tile exports more than 65535 contributions"); return -1; }
+                    xoff[e] = (unsigned short)run;
+                    const int4 q = conn4[e];
+                    const int n[4] = {q.x, q.y, q.z, q.w};
+                    for (int k = 0; k < npe; ++k)
+                        if (n[k] < lo || n[k] >= hi) { target.push_back(n[k]); ++run; }
+                }
+                xptr[t + 1] = xptr[t] + run;
+            }
+            std::vector<int> iptr(N + 1, 0), ilist(target.size());
+            for (int tn : target) iptr[tn + 1]++;
+            for (int i = 0; i < N; ++i) iptr[i + 1] += iptr[i];
+            std::vector<int> fill(iptr.begin(), iptr.end() - 1);
+            for (size_t k = 0; k < target.size(); ++k) ilist[fill[target[k]]++] = (int)k;
+            c->nexport = (long)target.size();
+            if (c->t_xoff.upload(xoff.data(), T) || c->t_xptr.upload(xptr.data(), nt + 1) ||
+                c->x_iptr.upload(iptr.data(), N + 1) || c->x_ilist.upload(ilist.data(), ilist.size()) ||
+                c->xf.alloc(std::max<size_t>(1, target.size())))
+                return -1;
+        } else {
+            c->t_xoff.release(); c->t_xptr.release(); c->x_iptr.release(); c->x_ilist.release(); c->xf.release();
+            c->nexport = 0;
+        }
     }
     c->tile_nodes = TILE;
     return tile_kernel_attrs(c);
@@ -356,33 +389,44 @@ extern "C" int fvp_create(const fvp_problem *p, int device, fvp_ctx **out) {
         std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
             return key[a] != key[b] ? key[a] < key[b] : a < b;
         });
-        std::vector<int> eedge(ns);
+        std::vector<int> eedge(ns), slot(ns), slot_ptr;
         std::vector<uint8_t> orient(T, 0);
+        slot_ptr.reserve(ns / 4 + 2);
         int ne = -1;
         int front = -1;
         for (size_t r = 0; r < ns; ++r) {
             uint32_t s = order[r];
             if (r == 0 || key[s] != key[order[r - 1]]) {
                 ++ne;
+                slot_ptr.push_back((int)r);
                 int e = (int)(s / ned), j = (int)(s % ned);
                 front = p->conn[(size_t)e * npe + tab[j][0]];  // first inserter's first node
             }
             int e = (int)(s / ned), j = (int)(s % ned);
             eedge[s] = ne;
+            slot[r] = (int)s;
             if (p->conn[(size_t)e * npe + tab[j][0]] == front) orient[e] |= (uint8_t)(1u << j);
         }
+        slot_ptr.push_back((int)ns);
         c->NE = ne + 1;
-        // move the per-element tables to the sorted element order
-        std::vector<int> eedge_n(ns);
+        // move the per-element tables to the sorted element order; slots keep their
+        // (edge, original element) order, the order the reference sums CV faces in
+        std::vector<int> eedge_n(ns), slot_n(ns);
         std::vector<uint8_t> orient_n(T);
         for (int e = 0; e < T; ++e) {
             const int o = eperm[e];
             orient_n[e] = orient[o];
             for (int j = 0; j < ned; ++j) eedge_n[(size_t)e * ned + j] = eedge[(size_t)o * ned + j];
         }
-        if (c->eedge.upload(eedge_n.data(), ns) || c->eorient.upload(orient_n.data(), T)) return fail();
+        for (size_t r = 0; r < ns; ++r) {
+            int o = slot[r] / ned, j = slot[r] % ned;
+            slot_n[r] = einv[o] * ned + j;
+        }
+        if (c->eedge.upload(eedge_n.data(), ns) || c->slot.upload(slot_n.data(), ns) ||
+            c->slot_ptr.upload(slot_ptr.data(), slot_ptr.size()) || c->eorient.upload(orient_n.data(), T))
+            return fail();
     }
-    if (c->emask.alloc(T) || c->eq.alloc(c->NE) || c->ea.alloc(c->NE) || c->eflux.alloc(c->NE) ||
+    if (c->emask.alloc(T) || c->qface.alloc((size_t)T * ned) || c->ea.alloc(c->NE) || c->eflux.alloc(c->NE) ||
         c->wf.alloc(c->NE))
         return fail();
     FVP_HIP(hipMemset(c->emask.p, 0, T));
@@ -510,7 +554,7 @@ extern "C" int fvp_create(const fvp_problem *p, int device, fvp_ctx **out) {
         c->ndir = (int)dn.size();
         if (c->dnode.upload(dn.data(), dn.size()) || c->dval.upload(dv.data(), dv.size())) return fail();
     }
-    if (build_tiles(c, c->tile_nodes)) return fail();
+    if (build_tiles(c, c->tile_nodes) || edge_areas(c)) return fail();
     if (c->tmp1.alloc(N) || c->tmp2.alloc(N) || c->tmp3.alloc(N) || c->red.alloc(4096)) return fail();
     c->red_h.resize(4096);
     *out = c;
@@ -528,7 +572,8 @@ extern "C" int fvp_destroy(fvp_ctx *c) {
     c->cap.release(); c->acc.release(); c->bnode.release(); c->bval.release(); c->dnode.release();
     c->dval.release(); c->t_wptr.release(); c->t_whalo.release(); c->t_vptr.release(); c->t_lconn.release();
     c->t_tz.release(); c->t_eptr.release(); c->t_hptr.release(); c->t_hlist.release(); c->t_bptr.release();
-    c->t_dptr.release(); c->eedge.release(); c->eq.release();
+    c->t_dptr.release(); c->t_xoff.release(); c->t_xptr.release(); c->x_iptr.release(); c->x_ilist.release();
+    c->xf.release(); c->eedge.release(); c->slot_ptr.release(); c->slot.release(); c->qface.release();
     c->ea.release(); c->eflux.release(); c->wf.release(); c->tmp1.release(); c->tmp2.release();
     c->tmp3.release(); c->red.release();
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);

@@ -243,8 +243,9 @@ int fvp_integrator_get_stats(const fvp_integrator *I, fvp_integrator_stats *st);
  * consecutive ids (a power of two in [64, 4096], default 512), one workgroup
  * per tile; xcd_map = 1 deals consecutive tiles to one XCD (shared L2 for
  * their halos).  Variants: FVP_RESIDUAL_OWNER evaluates each element once, in
- * the tile of its lowest node, with LDS sums for owned nodes and FP64 atomics
- * for the others, then a node pass finalises (default); FVP_RESIDUAL_WINDOW
+ * the tile of its lowest node, with LDS sums for owned nodes and export slots
+ * for the others, which a node pass gathers while it finalises (default; no
+ * global atomics); FVP_RESIDUAL_WINDOW
  * stages every tile's node window in LDS and evaluates each element once per
  * tile it touches, with no global atomics and a fused finalisation (needs the
  * window to fit the LDS).  fvp_autotune times every variant / tile size / XCD
