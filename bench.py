@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--atol", type=float, default=1e-6)
     ap.add_argument("--precon", default="bjilu0", choices=["bjilu0", "bjilu0_f32", "ilu0", "jacobi"])
     ap.add_argument("--block-rows", type=int, default=0, help="block-Jacobi rows per block (0 = library default)")
+    ap.add_argument("--overlap", type=int, default=0, help="block-Jacobi overlap layers (restricted additive Schwarz)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--launch", default="auto",
                     help="residual launch: 'auto' (autotuner at the initial state) or variant:tile_nodes:xcd_map, "
@@ -107,7 +108,7 @@ def fp32_window(args, prob, ctx, psi0, st0, st1, wall64, psi64):
     from fvmpor_amd.lib import Integrator, PRECON_BJILU0_F32
     tb = st1["t"]
     integ = Integrator(ctx, psi0, rtol=args.rtol, atol=args.atol, max_order=3, precon=PRECON_BJILU0_F32,
-                       block_rows=args.block_rows)
+                       block_rows=args.block_rows, overlap=args.overlap)
     for _ in range(args.warmup):
         integ.step(1.0)
     torch.cuda.synchronize()
@@ -167,7 +168,7 @@ def main():
     pc = {"bjilu0": PRECON_BJILU0, "bjilu0_f32": PRECON_BJILU0_F32, "ilu0": PRECON_ILU0,
           "jacobi": PRECON_JACOBI}[args.precon]
     integ = Integrator(ctx, psi0, rtol=args.rtol, atol=args.atol, max_order=3, precon=pc,
-                       block_rows=args.block_rows)
+                       block_rows=args.block_rows, overlap=args.overlap)
     ncol, _ = ctx.colours()
     if args.launch == "auto":
         launch = ctx.autotune(0.0, psi0, torch.zeros_like(psi0))
@@ -238,7 +239,7 @@ def main():
                         f"BDF(<=3) Newton-GMRES {args.precon}, rtol=atol={args.rtol:g}, timed steps "
                         f"{args.warmup + 1}..{args.warmup + args.steps} from t=0",
             "nodes": N, "elements": T, "edges": NE, "precon": args.precon,
-            "block_rows": args.block_rows or BLOCK_ROWS,
+            "block_rows": args.block_rows or BLOCK_ROWS, "overlap": args.overlap,
             "sim_days_timed": days, "t_start": st0["t"], "t_end": st1["t"],
             "newton_iters": st1["newton_iters"] - st0["newton_iters"],
             "gmres_iters": st1["gmres_iters"] - st0["gmres_iters"],

@@ -410,6 +410,7 @@ extern "C" int fvp_integrator_create(fvp_ctx *c, const fvp_integrator_params *p,
     if (I->p.lin_factor <= 0) I->p.lin_factor = 0.05;
     if (I->p.max_steps <= 0) I->p.max_steps = 100000;
     if (I->p.block_rows > 0 && fvp_set_block_rows(c, I->p.block_rows)) { delete I; return -1; }
+    if (I->p.overlap >= 0 && fvp_set_overlap(c, I->p.overlap)) { delete I; return -1; }
     I->N = c->N;
     const long N = c->N;
     for (int i = 0; i < NHIST; ++i) {

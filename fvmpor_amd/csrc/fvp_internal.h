@@ -78,8 +78,9 @@ struct fvp_ctx {
     int rvariant = FVP_RESIDUAL_OWNER;  // residual kernel variant
     long tile_elems = 0;            // element visits per residual (primary + halo)
     fvp::DevBuf<int> t_eptr, t_hptr, t_hlist, t_bptr, t_dptr;
-    // element-owner variant: per element its first export slot relative to the
-    // tile's (t_xptr), per node the export slots it gathers (CSR), the slots
+    // element-owner variant: per element its first export (element order)
+    // relative to the tile's (t_xptr); per export its slot (x_ilist), slots
+    // grouped by target node (x_iptr: CSR over nodes); the slot values
     fvp::DevBuf<unsigned short> t_xoff;
     fvp::DevBuf<int> t_xptr, x_iptr, x_ilist;
     fvp::DevBuf<double> xf;

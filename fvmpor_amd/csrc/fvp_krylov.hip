@@ -43,8 +43,10 @@ struct fvp_csr {
     DevBuf<int> flag;  // device error flag (zero pivot)
     // block-Jacobi ILU(0): per-block local CSR (columns relative to the block start)
     bool bj = false;
-    int bs = FVP_BLOCK_ROWS;                // rows per block
-    int nblk = 0, b_maxnz = 0, lds_max = 0;
+    int bs = FVP_BLOCK_ROWS;                // owned rows per block
+    int overlap = 0;                        // layers of neighbouring rows added to each block (RAS)
+    int nblk = 0, b_maxnz = 0, b_maxrows = 0, lds_max = 0;
+    DevBuf<int> b_roff, b_rowid, b_own;     // per block: first local row; global row of each local row; owned range
     size_t b_lds = 0;
     DevBuf<int> b_ia, b_src, b_diag;        // row pointers into the block arrays, source entry, diagonal position
     DevBuf<unsigned short> b_ja;            // local column
@@ -361,7 +363,7 @@ __global__ void lincomb(long n, double *__restrict__ out, const double *__restri
 // walk row k's U part and subtract l_ik u_kj from a_ij where (i, j) is in the
 // pattern, found through a per-wavefront column -> position map of row i.
 // Rows of one level run on different wavefronts.
-constexpr int kBjFactorThreads = 512;
+constexpr int kBjFactorThreads = 256;
 
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -371,7 +373,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 template <typename T>
 __global__ void __launch_bounds__(kBjFactorThreads) bj_factor(
-    int n, int bs, int max_nnz, const int *__restrict__ ia, const unsigned short *__restrict__ ja,
+    const int *__restrict__ roff, int max_rows, int max_nnz, const int *__restrict__ ia,
+    const unsigned short *__restrict__ ja,
     const int *__restrict__ diag, const int *__restrict__ src, const double *__restrict__ a,
     const int *__restrict__ blev, const int *__restrict__ lptr, const unsigned short *__restrict__ rows,
     T *__restrict__ lu, int *__restrict__ flag) {
@@ -379,11 +382,11 @@ __global__ void __launch_bounds__(kBjFactorThreads) bj_factor(
     T *val = (T *)smem;
     unsigned short *col = (unsigned short *)(val + max_nnz);
     unsigned short *rp = col + max_nnz;
-    unsigned short *dg = rp + (bs + 1);
-    short *pmap = (short *)(dg + bs);  // per wavefront: position of column j in its current row, -1 if absent
+    unsigned short *dg = rp + (max_rows + 1);
+    short *pmap = (short *)(dg + max_rows);  // per wavefront: position of column j in its current row, -1 if absent
     const int b = blockIdx.x;
-    const int base = b * bs;
-    const int cnt = min(bs, n - base);
+    const int base = roff[b];
+    const int cnt = roff[b + 1] - base;
     const int z0 = ia[base], nzb = ia[base + cnt] - z0;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
     for (int k = threadIdx.x; k < nzb; k += blockDim.x) {
@@ -394,10 +397,10 @@ __global__ void __launch_bounds__(kBjFactorThreads) bj_factor(
         rp[r] = (unsigned short)(ia[base + r] - z0);
         dg[r] = (unsigned short)(diag[base + r] - z0);
     }
-    for (int k = threadIdx.x; k < nwaves * bs; k += blockDim.x) pmap[k] = -1;
+    for (int k = threadIdx.x; k < nwaves * max_rows; k += blockDim.x) pmap[k] = -1;
     if (threadIdx.x == 0) rp[cnt] = (unsigned short)nzb;
     __syncthreads();
-    short *map = pmap + wave * bs;
+    short *map = pmap + wave * max_rows;
     for (int l = blev[b]; l < blev[b + 1]; ++l) {
         for (int t = lptr[l] + wave; t < lptr[l + 1]; t += nwaves) {
             const int r = rows[t];
@@ -501,8 +504,12 @@ __device__ __forceinline__ void bj_sweep(T *v, int base, int l0, int l1, const i
     }
 }
 
+// rowid: global row of each local row (nullptr: block b's rows are the
+// contiguous range starting at roff[b]); own: the block's owned local rows,
+// the only ones written back (restricted additive Schwarz).
 template <typename T>
-__global__ void __launch_bounds__(64) bj_solve(int n, int bs, const int *__restrict__ ia,
+__global__ void __launch_bounds__(64) bj_solve(const int *__restrict__ roff, const int *__restrict__ rowid,
+                                               const int *__restrict__ own, const int *__restrict__ ia,
                                                const unsigned short *__restrict__ ja, const int *__restrict__ diag,
                                                const T *__restrict__ lu, const int *__restrict__ blev_l,
                                                const int *__restrict__ lptr_l, const unsigned short *__restrict__ rows_l,
@@ -512,13 +519,14 @@ __global__ void __launch_bounds__(64) bj_solve(int n, int bs, const int *__restr
     extern __shared__ double smem[];
     T *v = (T *)smem;
     const int b = blockIdx.x;
-    const int base = b * bs;
-    const int cnt = min(bs, n - base);
-    for (int r = threadIdx.x; r < cnt; r += blockDim.x) v[r] = (T)x[base + r];
+    const int base = roff[b];
+    const int cnt = roff[b + 1] - base;
+    for (int r = threadIdx.x; r < cnt; r += blockDim.x) v[r] = (T)x[rowid ? rowid[base + r] : base + r];
     __syncthreads();
     bj_sweep<T, false>(v, base, blev_l[b], blev_l[b + 1], lptr_l, rows_l, ia, diag, ja, lu);
     bj_sweep<T, true>(v, base, blev_u[b], blev_u[b + 1], lptr_u, rows_u, ia, diag, ja, lu);
-    for (int r = threadIdx.x; r < cnt; r += blockDim.x) y[base + r] = (double)v[r];
+    const int olo = own[2 * b], ohi = own[2 * b + 1];
+    for (int r = olo + threadIdx.x; r < ohi; r += blockDim.x) y[rowid ? rowid[base + r] : base + r] = (double)v[r];
 }
 
 static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -717,32 +725,66 @@ static size_t bj_lds(int maxnz, size_t value_bytes, int bs) {
            (size_t)(kBjFactorThreads / 64) * bs * sizeof(short);
 }
 
-// block-local CSR and the per-block level sets of L and U
+// Block-local CSR and the per-block level sets of L and U.  Block b owns rows
+// [b bs, (b+1) bs); with overlap > 0 its local rows are the owned rows plus
+// `overlap` layers of their graph neighbours (sorted by global row, so the
+// local ILU(0) follows the global row order), and only the owned rows of its
+// solve are kept (restricted additive Schwarz).
 static int build_bj(fvp_csr *A) {
     const int n = A->n, BS = A->bs;
     const int nb = (n + BS - 1) / BS;
-    std::vector<int> bia(n + 1, 0), bsrc, bdiag(n);
+    std::vector<int> roff(nb + 1, 0), rowid, own(2 * (size_t)nb), bia(1, 0), bsrc, bdiag;
     std::vector<unsigned short> bja;
-    bsrc.reserve(A->nnz);
-    bja.reserve(A->nnz);
-    for (int i = 0; i < n; ++i) {
-        const int base = (i / BS) * BS, top = std::min(n, base + BS);
-        bdiag[i] = -1;
-        for (int k = A->ia_h[i]; k < A->ia_h[i + 1]; ++k) {
-            int j = A->ja_h[k];
-            if (j < base || j >= top) continue;
-            if (j == i) bdiag[i] = (int)bsrc.size();
-            bsrc.push_back(k);
-            bja.push_back((unsigned short)(j - base));
+    std::vector<int> stamp(n, -1), loc(n, 0), ext, frontier, next;
+    for (int b = 0; b < nb; ++b) {
+        const int lo = b * BS, hi = std::min(n, lo + BS);
+        ext.clear();
+        frontier.clear();
+        for (int i = lo; i < hi; ++i) { stamp[i] = b; ext.push_back(i); frontier.push_back(i); }
+        for (int layer = 0; layer < A->overlap; ++layer) {
+            next.clear();
+            for (int i : frontier)
+                for (int k = A->ia_h[i]; k < A->ia_h[i + 1]; ++k) {
+                    const int j = A->ja_h[k];
+                    if (stamp[j] != b) { stamp[j] = b; ext.push_back(j); next.push_back(j); }
+                }
+            frontier.swap(next);
         }
-        if (bdiag[i] < 0) { set_error("zero diagonal on row " + std::to_string(i)); return FVP_ERR_PIVOT; }
-        bia[i + 1] = (int)bsrc.size();
+        std::sort(ext.begin(), ext.end());
+        const int cnt = (int)ext.size();
+        if (cnt > 65535) { set_error("block-Jacobi ILU(0): a block has more than 65535 rows"); return -1; }
+        for (int r = 0; r < cnt; ++r) loc[ext[r]] = r;
+        const int olo = (int)(std::lower_bound(ext.begin(), ext.end(), lo) - ext.begin());
+        own[2 * (size_t)b] = olo;
+        own[2 * (size_t)b + 1] = olo + (hi - lo);
+        for (int r = 0; r < cnt; ++r) {
+            const int i = ext[r];
+            int d = -1;
+            for (int k = A->ia_h[i]; k < A->ia_h[i + 1]; ++k) {
+                const int j = A->ja_h[k];
+                if (stamp[j] != b) continue;
+                if (j == i) d = (int)bsrc.size();
+                bsrc.push_back(k);
+                bja.push_back((unsigned short)loc[j]);
+            }
+            if (d < 0) { set_error("zero diagonal on row " + std::to_string(i)); return FVP_ERR_PIVOT; }
+            bdiag.push_back(d);
+            bia.push_back((int)bsrc.size());
+            rowid.push_back(i);
+        }
+        roff[b + 1] = roff[b] + cnt;
     }
+    const int total = roff[nb];
     std::vector<int> blev_l(nb + 1, 0), blev_u(nb + 1, 0), lptr_l(1, 0), lptr_u(1, 0);
     std::vector<unsigned short> rows_l, rows_u;
-    std::vector<int> lev(BS);
+    std::vector<int> lev;
+    int maxnz = 0, maxrow = 0, maxrows = 0;
     for (int b = 0; b < nb; ++b) {
-        const int base = b * BS, cnt = std::min(BS, n - base);
+        const int base = roff[b], cnt = roff[b + 1] - base;
+        maxrows = std::max(maxrows, cnt);
+        maxnz = std::max(maxnz, bia[base + cnt] - bia[base]);
+        for (int r = 0; r < cnt; ++r) maxrow = std::max(maxrow, bia[base + r + 1] - bia[base + r]);
+        lev.assign(cnt, 0);
         auto group = [&](bool lower, std::vector<int> &lptr, std::vector<unsigned short> &rows) {
             int nl = 0;
             if (lower) {
@@ -773,22 +815,22 @@ static int build_bj(fvp_csr *A) {
         blev_l[b + 1] = blev_l[b] + group(true, lptr_l, rows_l);
         blev_u[b + 1] = blev_u[b] + group(false, lptr_u, rows_u);
     }
-    if (A->b_ia.upload(bia.data(), n + 1) || A->b_src.upload(bsrc.data(), bsrc.size()) ||
-        A->b_diag.upload(bdiag.data(), n) || A->b_ja.upload(bja.data(), bja.size()) || A->b_lu.alloc(bsrc.size()) ||
+    if (maxrow > 64) { set_error("block-Jacobi ILU(0) supports rows of at most 64 block entries"); return -1; }
+    if (maxnz > 65535) { set_error("block-Jacobi ILU(0): more than 65535 entries in one block"); return -1; }
+    const bool ident = A->overlap == 0;  // rows of block b are [b bs, ...): no row list needed
+    if (A->b_ia.upload(bia.data(), total + 1) || A->b_src.upload(bsrc.data(), bsrc.size()) ||
+        A->b_diag.upload(bdiag.data(), total) || A->b_ja.upload(bja.data(), bja.size()) ||
+        A->b_lu.alloc(bsrc.size()) || A->b_roff.upload(roff.data(), nb + 1) ||
+        A->b_own.upload(own.data(), own.size()) || (!ident && A->b_rowid.upload(rowid.data(), total)) ||
         A->b_lev_l.upload(blev_l.data(), nb + 1) || A->b_lev_u.upload(blev_u.data(), nb + 1) ||
         A->b_lptr_l.upload(lptr_l.data(), lptr_l.size()) || A->b_lptr_u.upload(lptr_u.data(), lptr_u.size()) ||
         A->b_rows_l.upload(rows_l.data(), rows_l.size()) || A->b_rows_u.upload(rows_u.data(), rows_u.size()))
         return -1;
-    int maxnz = 0, maxrow = 0;
-    for (int b = 0; b < nb; ++b) {
-        const int base = b * BS, cnt = std::min(BS, n - base);
-        maxnz = std::max(maxnz, bia[base + cnt] - bia[base]);
-        for (int r = 0; r < cnt; ++r) maxrow = std::max(maxrow, bia[base + r + 1] - bia[base + r]);
-    }
-    if (maxrow > 64) { set_error("block-Jacobi ILU(0) supports rows of at most 64 block entries"); return -1; }
-    if (maxnz > 65535) { set_error("block-Jacobi ILU(0): more than 65535 entries in one block"); return -1; }
+    if (ident) A->b_rowid.release();
+    A->b_lu32.release();
     A->b_maxnz = maxnz;
-    A->b_lds = bj_lds(maxnz, sizeof(double), BS);
+    A->b_maxrows = maxrows;
+    A->b_lds = bj_lds(maxnz, sizeof(double), maxrows);
     if (hipDeviceGetAttribute(&A->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, A->device) != hipSuccess)
         A->lds_max = 64 * 1024;
     // per-kernel attributes (shared by every matrix): allow the device maximum;
@@ -818,7 +860,7 @@ static int precon_build_impl(fvp_csr *A, int type) {
             if (rc) { A->ptype = -1; return rc; }
         }
         const size_t vb = type == FVP_PRECON_BJILU0 ? sizeof(double) : sizeof(float);
-        const size_t need = bj_lds(A->b_maxnz, vb, A->bs);  // the factorisation's staging
+        const size_t need = bj_lds(A->b_maxnz, vb, A->b_maxrows);  // the factorisation's staging
         if (need > (size_t)A->lds_max) {
             set_error("block-Jacobi ILU(0): a block of " + std::to_string(A->bs) + " rows needs " +
                       std::to_string(need) + " B of LDS, the device allows " + std::to_string(A->lds_max));
@@ -826,13 +868,13 @@ static int precon_build_impl(fvp_csr *A, int type) {
             return -1;
         }
         if (type == FVP_PRECON_BJILU0)
-            bj_factor<double><<<A->nblk, kBjFactorThreads, A->b_lds, s>>>(n, A->bs, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p,
+            bj_factor<double><<<A->nblk, kBjFactorThreads, A->b_lds, s>>>(A->b_roff.p, A->b_maxrows, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p,
                                                               A->b_src.p, A->a.p, A->b_lev_l.p, A->b_lptr_l.p,
                                                               A->b_rows_l.p, A->b_lu.p, A->flag.p);
         else {
             if (!A->b_lu32.p && A->b_lu32.alloc(A->b_lu.n)) return -1;
-            bj_factor<float><<<A->nblk, kBjFactorThreads, bj_lds(A->b_maxnz, sizeof(float), A->bs), s>>>(
-                n, A->bs, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p, A->b_lev_l.p, A->b_lptr_l.p,
+            bj_factor<float><<<A->nblk, kBjFactorThreads, bj_lds(A->b_maxnz, sizeof(float), A->b_maxrows), s>>>(
+                A->b_roff.p, A->b_maxrows, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p, A->b_lev_l.p, A->b_lptr_l.p,
                 A->b_rows_l.p, A->b_lu32.p, A->flag.p);
         }
     } else if (type == FVP_PRECON_JACOBI) {
@@ -878,13 +920,13 @@ static int precon_apply_impl(fvp_csr *A, const double *x, double *y) {
         vmul<<<nblk(n, 256), 256, 0, s>>>(n, x, A->jac.p, y);
         break;
     case FVP_PRECON_BJILU0:
-        bj_solve<double><<<A->nblk, 64, sizeof(double) * A->bs, s>>>(
-            n, A->bs, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_lu.p, A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p,
+        bj_solve<double><<<A->nblk, 64, sizeof(double) * A->b_maxrows, s>>>(
+            A->b_roff.p, A->b_rowid.p, A->b_own.p, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_lu.p, A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p,
             A->b_lev_u.p, A->b_lptr_u.p, A->b_rows_u.p, x, y);
         break;
     case FVP_PRECON_BJILU0_F32:
-        bj_solve<float><<<A->nblk, 64, sizeof(float) * A->bs, s>>>(
-            n, A->bs, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_lu32.p, A->b_lev_l.p, A->b_lptr_l.p,
+        bj_solve<float><<<A->nblk, 64, sizeof(float) * A->b_maxrows, s>>>(
+            A->b_roff.p, A->b_rowid.p, A->b_own.p, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_lu32.p, A->b_lev_l.p, A->b_lptr_l.p,
             A->b_rows_l.p, A->b_lev_u.p, A->b_lptr_u.p, A->b_rows_u.p, x, y);
         break;
     case FVP_PRECON_ILU0: {
@@ -952,6 +994,7 @@ extern "C" int fvp_csr_destroy(fvp_csr *A) {
     if (A->hstat) (void)hipHostFree(A->hstat);
     for (hipEvent_t e : A->ev) (void)hipEventDestroy(e);
     A->b_ia.release(); A->b_src.release(); A->b_diag.release(); A->b_ja.release(); A->b_lu.release(); A->b_lu32.release();
+    A->b_roff.release(); A->b_rowid.release(); A->b_own.release();
     A->b_lev_l.release(); A->b_lev_u.release(); A->b_lptr_l.release(); A->b_lptr_u.release();
     A->b_rows_l.release(); A->b_rows_u.release();
     if (A->own_stream) (void)hipStreamDestroy(A->own_stream);
@@ -1005,6 +1048,21 @@ extern "C" int fvp_precon_block_rows(fvp_csr *A, int rows) {
     return set_block_rows(A, rows);
 }
 
+static int set_overlap(fvp_csr *A, int layers) {
+    if (layers < 0 || layers > 2) { set_error("overlap must be 0, 1 or 2 layers"); return -1; }
+    if (layers != A->overlap) {
+        A->overlap = layers;
+        A->bj = false;
+        if (A->ptype == FVP_PRECON_BJILU0 || A->ptype == FVP_PRECON_BJILU0_F32) A->ptype = -1;
+    }
+    return 0;
+}
+
+extern "C" int fvp_precon_overlap(fvp_csr *A, int layers) {
+    if (!A) { set_error("null matrix"); return -1; }
+    return set_overlap(A, layers);
+}
+
 extern "C" int fvp_precon_build(fvp_csr *A, int type) {
     if (!A) { set_error("null matrix"); return -1; }
     FVP_HIP(hipSetDevice(A->device));
@@ -1026,7 +1084,10 @@ extern "C" int fvp_precon_values(fvp_csr *A, double *values_host) {
         FVP_HIP(hipMemcpy(values_host, A->lu.p, sizeof(double) * A->nnz, hipMemcpyDeviceToHost));
     else if (A->ptype == FVP_PRECON_JACOBI)
         FVP_HIP(hipMemcpy(values_host, A->jac.p, sizeof(double) * A->n, hipMemcpyDeviceToHost));
-    else if (A->ptype == FVP_PRECON_BJILU0 || A->ptype == FVP_PRECON_BJILU0_F32) {
+    else if ((A->ptype == FVP_PRECON_BJILU0 || A->ptype == FVP_PRECON_BJILU0_F32) && A->overlap > 0) {
+        set_error("overlapping blocks share entries: their factors have no single-pattern form");
+        return -1;
+    } else if (A->ptype == FVP_PRECON_BJILU0 || A->ptype == FVP_PRECON_BJILU0_F32) {
         // scattered back to the full pattern, 0 off the blocks
         std::vector<double> lu(A->b_lu.n);
         std::vector<int> src(A->b_src.n);
@@ -1245,6 +1306,13 @@ extern "C" int fvp_set_block_rows(fvp_ctx *c, int rows) {
     FVP_HIP(hipSetDevice(c->device));
     if (jac_init(c)) return -1;
     return set_block_rows(c->jd->J, rows);
+}
+
+extern "C" int fvp_set_overlap(fvp_ctx *c, int layers) {
+    if (!c) { set_error("null argument"); return -1; }
+    FVP_HIP(hipSetDevice(c->device));
+    if (jac_init(c)) return -1;
+    return set_overlap(c->jd->J, layers);
 }
 
 extern "C" int fvp_colours(fvp_ctx *c, int *n_colours, int *colour_host) {

@@ -49,27 +49,35 @@ __device__ __forceinline__ void vg_eval(double h, const double *zp, double &Sw, 
     Sw = s; dSw = d; krw = k;
 }
 
+// One node per thread; the node's zones (its mask bits) are visited in a
+// per-lane loop with the zone parameters in LDS, so a wavefront pays for the
+// largest zone count of its nodes rather than for every zone any of them has.
 template <bool JV>
 __global__ void __launch_bounds__(256) vg_kernel(int N, ZoneTable zt, double rho0, const double *__restrict__ psi,
                                                  const uint8_t *__restrict__ zmask, const double *__restrict__ zw,
                                                  double *__restrict__ kr, double *__restrict__ cap,
                                                  const double *__restrict__ v, const double *__restrict__ sig) {
+    __shared__ double zp[kMaxZones * 10];
+    if (threadIdx.x < 10 * zt.nz) zp[threadIdx.x] = zt.p[threadIdx.x / 10][threadIdx.x % 10];
+    __syncthreads();
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     double h = psi[i];
     if (JV) h = psi[i] + sig[0] * v[i];
     unsigned m = zmask[i];
     double phi = 0., Sw = 0., dSw = 0., Ss = 0.;
-    for (int z = 0; z < zt.nz; ++z) {
-        if (!(m & (1u << z))) continue;
-        double w = zw[(size_t)z * N + i];
+    while (m) {
+        const int z = __ffs(m) - 1;
+        m &= m - 1;
+        const double *p = zp + 10 * z;
+        const double w = zw[(size_t)z * N + i];
         double s, d, k;
-        vg_eval(h, zt.p[z], s, d, k);
+        vg_eval(h, p, s, d, k);
         kr[(size_t)z * N + i] = k;
-        phi += zt.p[z][Z_PHI] * w;
+        phi += p[Z_PHI] * w;
         Sw += s * w;
         dSw += d * w;
-        Ss += zt.p[z][Z_SS] * w;
+        Ss += p[Z_SS] * w;
     }
     cap[i] = (phi * dSw + Ss * Sw) * rho0;
 }
@@ -207,17 +215,18 @@ __global__ void __launch_bounds__(256) elem_kernel(int T, int N, ZoneTable zt, d
 // exactly one primary tile, so each element is evaluated once), gathering node
 // data through the caches.  Contributions to owned nodes accumulate in LDS
 // (ds_add_f64) and are stored, with the tile's boundary fluxes, to acc; those
-// to nodes of other tiles are stored to export slots (the element's first
-// slot: tile base + xoff[e], then one per foreign node in local order), which
-// the finalisation gathers per node.  No global atomics.
+// to nodes of other tiles are exports (the element's first: tile base +
+// xoff[e], then one per foreign node in local order), each stored to its slot
+// xdst[export]; slots are grouped by target node, so the finalisation reads
+// each node's imports contiguously.  No global atomics.
 template <int DIM, bool JV>
-__global__ void __launch_bounds__(256) owner_kernel(
+__global__ void __launch_bounds__(1024) owner_kernel(
     int N, int ntiles, int tile_nodes, int per_xcd, ZoneTable zt, double rho0, const int4 *__restrict__ conn,
     const uint8_t *__restrict__ ezone, const double *__restrict__ kmul, const uint8_t *__restrict__ emask,
     const double *__restrict__ xyz, const double *__restrict__ psi, const double *__restrict__ kr,
     const int *__restrict__ eptr, const int *__restrict__ bptr, const int *__restrict__ bnode,
     const double *__restrict__ bval, const unsigned short *__restrict__ xoff, const int *__restrict__ xptr,
-    double *__restrict__ xf, double *__restrict__ acc_g, JvArgs jv) {
+    const int *__restrict__ xdst, double *__restrict__ xf, double *__restrict__ acc_g, JvArgs jv) {
     extern __shared__ double sm[];
     const int tile = per_xcd ? (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3) : blockIdx.x;
     if (tile >= ntiles) return;  // uniform for the whole workgroup
@@ -227,7 +236,7 @@ __global__ void __launch_bounds__(256) owner_kernel(
     __shared__ double zk[3 * kMaxZones];
     if (threadIdx.x < 3 * zt.nz) zk[threadIdx.x] = zt.p[threadIdx.x / 3][ZK_X + threadIdx.x % 3];
     __syncthreads();
-    double *xt = xf + xptr[tile];
+    const int *xd = xdst + xptr[tile];
     for (int e = eptr[tile] + threadIdx.x; e < eptr[tile + 1]; e += blockDim.x) {
         int n[4];
         double f[4];
@@ -237,7 +246,7 @@ __global__ void __launch_bounds__(256) owner_kernel(
 #pragma unroll
         for (int q = 0; q < DIM + 1; ++q) {
             if (n[q] >= lo && n[q] < hi) atomicAdd(&sm[n[q] - lo], f[q]);
-            else xt[x++] = f[q];
+            else xf[xd[x++]] = f[q];
         }
     }
     __syncthreads();
@@ -251,13 +260,13 @@ __global__ void __launch_bounds__(256) owner_kernel(
 // difference quotient).
 template <bool JV>
 __global__ void owner_finalize(int N, const double *__restrict__ acc, const int *__restrict__ iptr,
-                               const int *__restrict__ ilist, const double *__restrict__ xf,
+                               const double *__restrict__ xf,
                                const double *__restrict__ invV, const double *__restrict__ psip,
                                const double *__restrict__ cap, double *__restrict__ F, JvArgs jv) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     double a = acc[i];
-    for (int k = iptr[i]; k < iptr[i + 1]; ++k) a += xf[ilist[k]];
+    for (int k = iptr[i]; k < iptr[i + 1]; ++k) a += xf[k];
     if (JV) {
         const double sig = jv.sig[0];
         const double yp = psip[i] + jv.cj * (sig * jv.v[i]);
@@ -559,13 +568,15 @@ static void launch_tile(fvp_ctx *c, const double *psi, const double *psip, doubl
     if (c->rvariant == FVP_RESIDUAL_OWNER) {
         const int N = c->N;
         prof_mark(c, 0, 0);
-        owner_kernel<DIM, JV><<<c->xcd_map ? per * 8 : c->ntiles, 256, sizeof(double) * c->tile_nodes, s>>>(
+        // one workgroup per tile, wide enough that large tiles still fill the CUs
+        const int threads = std::min(1024, std::max(256, c->tile_nodes / 2));
+        owner_kernel<DIM, JV><<<c->xcd_map ? per * 8 : c->ntiles, threads, sizeof(double) * c->tile_nodes, s>>>(
             N, c->ntiles, c->tile_nodes, c->xcd_map ? per : 0, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
             c->emask.p, c->xyz.p, psi, c->kr.p, c->t_eptr.p, c->t_bptr.p, c->bnode.p, c->bval.p, c->t_xoff.p,
-            c->t_xptr.p, c->xf.p, c->acc.p, jv);
+            c->t_xptr.p, c->x_ilist.p, c->xf.p, c->acc.p, jv);
         prof_mark(c, 0, 1);
-        owner_finalize<JV><<<blocks(N, 256), 256, 0, s>>>(N, c->acc.p, c->x_iptr.p, c->x_ilist.p, c->xf.p, c->invV.p,
-                                                          psip, c->cap.p, F, jv);
+        owner_finalize<JV><<<blocks(N, 256), 256, 0, s>>>(N, c->acc.p, c->x_iptr.p, c->xf.p, c->invV.p, psip,
+                                                          c->cap.p, F, jv);
         if (c->ndir)
             dirichlet_kernel<JV><<<blocks(c->ndir, 256), 256, 0, s>>>(c->ndir, c->dnode.p, c->dval.p, psi, F, jv);
         return;

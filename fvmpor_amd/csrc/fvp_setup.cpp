@@ -224,14 +224,16 @@ int build_tiles(fvp_ctx *c, int TILE) {
                 }
                 xptr[t + 1] = xptr[t] + run;
             }
-            std::vector<int> iptr(N + 1, 0), ilist(target.size());
+            // slots are stored grouped by target node (exporting element order within
+            // a node), so the finalisation reads each node's imports contiguously
+            std::vector<int> iptr(N + 1, 0), xdst(target.size());
             for (int tn : target) iptr[tn + 1]++;
             for (int i = 0; i < N; ++i) iptr[i + 1] += iptr[i];
             std::vector<int> fill(iptr.begin(), iptr.end() - 1);
-            for (size_t k = 0; k < target.size(); ++k) ilist[fill[target[k]]++] = (int)k;
+            for (size_t k = 0; k < target.size(); ++k) xdst[k] = fill[target[k]]++;
             c->nexport = (long)target.size();
             if (c->t_xoff.upload(xoff.data(), T) || c->t_xptr.upload(xptr.data(), nt + 1) ||
-                c->x_iptr.upload(iptr.data(), N + 1) || c->x_ilist.upload(ilist.data(), ilist.size()) ||
+                c->x_iptr.upload(iptr.data(), N + 1) || c->x_ilist.upload(xdst.data(), xdst.size()) ||
                 c->xf.alloc(std::max<size_t>(1, target.size())))
                 return -1;
         } else {

@@ -50,6 +50,8 @@ SIGNATURES = {
     "fvp_precon_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fvp_precon_block_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fvp_set_block_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "fvp_set_overlap": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "fvp_precon_overlap": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fvp_set_residual_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "fvp_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, _c_int_p,
                                     _c_int_p, _c_int_p, _c_dbl_p]),
@@ -268,6 +270,9 @@ class CsrMatrix:
     def precon_build(self, ptype) -> int:
         return _check(lib().fvp_precon_build(self.h, int(ptype)), "fvp_precon_build")
 
+    def precon_overlap(self, layers):
+        _check(lib().fvp_precon_overlap(self.h, int(layers)), "fvp_precon_overlap")
+
     def precon_block_rows(self, rows):
         _check(lib().fvp_precon_block_rows(self.h, int(rows)), "fvp_precon_block_rows")
 
@@ -300,6 +305,10 @@ def _ctx_jacobian(self) -> CsrMatrix:
     h = ctypes.c_void_p()
     _check(lib().fvp_jacobian(self.h, ctypes.byref(h)), "fvp_jacobian")
     return CsrMatrix(None, None, handle=h)
+
+
+def _ctx_set_overlap(self, layers):
+    _check(lib().fvp_set_overlap(self.h, int(layers)), "fvp_set_overlap")
 
 
 def _ctx_set_block_rows(self, rows):
@@ -351,6 +360,7 @@ Context.precon_solve = _ctx_precon_solve
 Context.jacobian = _ctx_jacobian
 Context.colours = _ctx_colours
 Context.set_block_rows = _ctx_set_block_rows
+Context.set_overlap = _ctx_set_overlap
 Context.set_residual_launch = _ctx_set_residual_launch
 Context.autotune = _ctx_autotune
 Context.newton_gmres = _ctx_newton_gmres
@@ -361,7 +371,7 @@ class IntegratorParams(ctypes.Structure):
                 ("h0", ctypes.c_double), ("h_max", ctypes.c_double), ("precon_type", ctypes.c_int),
                 ("dim_krylov", ctypes.c_int), ("max_restarts", ctypes.c_int), ("lin_factor", ctypes.c_double),
                 ("newton_tol", ctypes.c_double), ("max_newton", ctypes.c_int), ("max_steps", ctypes.c_int),
-                ("block_rows", ctypes.c_int)]
+                ("block_rows", ctypes.c_int), ("overlap", ctypes.c_int)]
 
 
 class IntegratorStats(ctypes.Structure):
@@ -379,10 +389,10 @@ class Integrator:
 
     def __init__(self, ctx: Context, psi0, t0=0.0, rtol=1e-8, atol=1e-8, max_order=3, h0=0.0, h_max=0.0,
                  precon=PRECON_ILU0, dim_krylov=20, max_restarts=5, lin_factor=0.05, newton_tol=0.33,
-                 max_newton=4, max_steps=1000000, block_rows=0):
+                 max_newton=4, max_steps=1000000, block_rows=0, overlap=-1):
         self.ctx = ctx
         self.params = IntegratorParams(rtol, atol, max_order, h0, h_max, precon, dim_krylov, max_restarts,
-                                       lin_factor, newton_tol, max_newton, max_steps, block_rows)
+                                       lin_factor, newton_tol, max_newton, max_steps, block_rows, overlap)
         h = ctypes.c_void_p()
         _check(lib().fvp_integrator_create(ctx.h, ctypes.byref(self.params), float(t0), _ptr(psi0),
                                            ctypes.byref(h)), "fvp_integrator_create")

@@ -137,9 +137,17 @@ int fvp_precon_build(fvp_csr *A, int type);
  * default FVP_BLOCK_ROWS; takes effect at the next build.  The build fails
  * (< 0) when one block's factor does not fit the workgroup's LDS. */
 int fvp_precon_block_rows(fvp_csr *A, int rows);
+/* Overlap of the block-Jacobi blocks: each block adds `layers` (0..2) layers of
+ * graph-neighbour rows to its owned rows, factors ILU(0) on the extended block
+ * (rows in global order) and keeps only its owned rows of the solve
+ * (restricted additive Schwarz); 0 (default) is plain block-Jacobi.  The
+ * 1-GPU form of decomp's overlapping subdomain preconditioners; takes effect
+ * at the next build. */
+int fvp_precon_overlap(fvp_csr *A, int layers);
 /* y = M^{-1} x (precon_apply, precon.c:112-149) */
 int fvp_precon_apply(fvp_csr *A, const double *x, double *y);
-/* host copy of the factored values (ILU(0): L unit-lower, U with inverted diagonal) */
+/* host copy of the factored values (ILU(0): L unit-lower, U with inverted
+ * diagonal; not available for overlapping block-Jacobi blocks) */
 int fvp_precon_values(fvp_csr *A, double *values_host);
 
 /* GMRES parameters and statistics (Tgmres, gmres.h:15-32) */
@@ -176,8 +184,10 @@ int fvp_precon_setup(fvp_ctx *ctx, double t, double c_j, double h, const double 
 int fvp_precon_solve(fvp_ctx *ctx, const double *r, double *z);
 /* the context's Jacobian as a CSR handle (pattern = node adjacency plus diagonal) */
 int fvp_jacobian(fvp_ctx *ctx, fvp_csr **A);
-/* rows per block of the context's block-Jacobi preconditioner (see fvp_precon_block_rows) */
+/* rows per block / overlap layers of the context's block-Jacobi preconditioner
+ * (see fvp_precon_block_rows, fvp_precon_overlap) */
 int fvp_set_block_rows(fvp_ctx *ctx, int rows);
+int fvp_set_overlap(fvp_ctx *ctx, int layers);
 /* number of colours of the distance-2 colouring (preconditioner_dss.cpp:72-94) */
 int fvp_colours(fvp_ctx *ctx, int *n_colours, int *colour_host);
 /* Solve J x = b, J = dF/dpsi + c_j dF/dpsi' applied matrix-free (fvp_jv) and
@@ -207,6 +217,7 @@ typedef struct {
     int max_newton;       /* IDA maxcor, 4 */
     int max_steps;        /* per advance call */
     int block_rows;       /* block-Jacobi block size, 0 = FVP_BLOCK_ROWS */
+    int overlap;          /* block-Jacobi overlap layers (0..2); < 0 keeps the context's setting */
 } fvp_integrator_params;
 
 typedef struct {
