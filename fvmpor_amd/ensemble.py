@@ -164,3 +164,58 @@ def run_batch(problems, t_end=0.5, rtol=1e-6, atol=1e-6, precon=None, block_rows
     psi = out.cpu().numpy()
     heads = [psi[offsets[i]:offsets[i + 1]] for i in range(len(problems))]
     return heads, integ.stats(), wall
+
+
+def _build(d):
+    return make_instance(d)
+
+
+def main(argv=None):
+    """Run a Monte Carlo ensemble to t_end on cuda:0 and print one JSON line:
+    python -m fvmpor_amd.ensemble [--n 256] [--t-end 0.5] [--workers 16]."""
+    import argparse
+    import json
+    import sys
+    from concurrent.futures import ProcessPoolExecutor
+
+    import torch
+
+    from .lib import PRECON_BJILU0_F32
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--t-end", type=float, default=0.5)
+    ap.add_argument("--n-min", type=int, default=5000)
+    ap.add_argument("--n-max", type=int, default=60000)
+    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--rtol", type=float, default=1e-6)
+    args = ap.parse_args(argv)
+    draws = draw_instances(args.n, n_min=args.n_min, n_max=args.n_max)
+    t0 = time.perf_counter()
+    with ProcessPoolExecutor(max_workers=args.workers) as pool:
+        probs = list(pool.map(_build, draws, chunksize=4))
+    print(f"built {len(probs)} meshes in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    out = {"instances": len(probs), "t_end": args.t_end, "batches": []}
+    wall_total = 0.0
+    for dim in (2, 3):
+        batch = [p for p in probs if p.mesh.dim == dim]
+        if not batch:
+            continue
+        heads, st, wall = run_batch(batch, t_end=args.t_end, rtol=args.rtol, atol=args.rtol,
+                                    precon=PRECON_BJILU0_F32)
+        torch.cuda.synchronize()
+        nodes = sum(p.mesh.n_nodes for p in batch)
+        wall_total += wall
+        out["batches"].append({"dim": dim, "instances": len(batch), "nodes": nodes, "wall_s": wall,
+                               "steps": st["steps"], "newton_iters": st["newton_iters"],
+                               "gmres_iters": st["gmres_iters"], "precon_setups": st["precon_setups"],
+                               "finite": bool(all(np.isfinite(h).all() for h in heads))})
+        print(f"{dim}D batch: {len(batch)} instances, {nodes} nodes, {wall:.2f}s, {st['steps']} steps",
+              file=sys.stderr, flush=True)
+    out["wall_s"] = wall_total
+    out["instance_days_per_s"] = len(probs) * args.t_end / wall_total
+    out["node_days_per_s"] = sum(p.mesh.n_nodes for p in probs) * args.t_end / wall_total
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
