@@ -371,7 +371,11 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename T>
+// STAGED: the block's values, columns, row pointers and diagonal offsets are
+// staged in LDS (when they fit); otherwise the factorisation works in place on
+// the block's values in memory (one block's values stay in its CU's caches) and
+// LDS holds only the column maps.
+template <typename T, bool STAGED>
 __global__ void __launch_bounds__(kBjFactorThreads) bj_factor(
     const int *__restrict__ roff, int max_rows, int max_nnz, const int *__restrict__ ia,
     const unsigned short *__restrict__ ja,
@@ -379,38 +383,53 @@ __global__ void __launch_bounds__(kBjFactorThreads) bj_factor(
     const int *__restrict__ blev, const int *__restrict__ lptr, const unsigned short *__restrict__ rows,
     T *__restrict__ lu, int *__restrict__ flag) {
     extern __shared__ double smem[];
-    T *val = (T *)smem;
-    unsigned short *col = (unsigned short *)(val + max_nnz);
-    unsigned short *rp = col + max_nnz;
-    unsigned short *dg = rp + (max_rows + 1);
-    short *pmap = (short *)(dg + max_rows);  // per wavefront: position of column j in its current row, -1 if absent
     const int b = blockIdx.x;
     const int base = roff[b];
     const int cnt = roff[b + 1] - base;
     const int z0 = ia[base], nzb = ia[base + cnt] - z0;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-    for (int k = threadIdx.x; k < nzb; k += blockDim.x) {
-        val[k] = (T)a[src[z0 + k]];
-        col[k] = ja[z0 + k];
+    T *val;
+    const unsigned short *col;
+    short *pmap;  // per wavefront: position of column j in its current row, -1 if absent
+    unsigned short *rps = nullptr, *dgs = nullptr;
+    if (STAGED) {
+        val = (T *)smem;
+        unsigned short *cs = (unsigned short *)(val + max_nnz);
+        rps = cs + max_nnz;
+        dgs = rps + (max_rows + 1);
+        pmap = (short *)(dgs + max_rows);
+        for (int k = threadIdx.x; k < nzb; k += blockDim.x) {
+            val[k] = (T)a[src[z0 + k]];
+            cs[k] = ja[z0 + k];
+        }
+        for (int r = threadIdx.x; r < cnt; r += blockDim.x) {
+            rps[r] = (unsigned short)(ia[base + r] - z0);
+            dgs[r] = (unsigned short)(diag[base + r] - z0);
+        }
+        if (threadIdx.x == 0) rps[cnt] = (unsigned short)nzb;
+        col = cs;
+    } else {
+        val = lu + z0;
+        col = ja + z0;
+        pmap = (short *)smem;
+        for (int k = threadIdx.x; k < nzb; k += blockDim.x) val[k] = (T)a[src[z0 + k]];
+        __threadfence_block();
     }
-    for (int r = threadIdx.x; r < cnt; r += blockDim.x) {
-        rp[r] = (unsigned short)(ia[base + r] - z0);
-        dg[r] = (unsigned short)(diag[base + r] - z0);
-    }
+    auto rp = [&](int r) -> int { return STAGED ? (int)rps[r] : ia[base + r] - z0; };
+    auto dg = [&](int r) -> int { return STAGED ? (int)dgs[r] : diag[base + r] - z0; };
     for (int k = threadIdx.x; k < nwaves * max_rows; k += blockDim.x) pmap[k] = -1;
-    if (threadIdx.x == 0) rp[cnt] = (unsigned short)nzb;
     __syncthreads();
     short *map = pmap + wave * max_rows;
     for (int l = blev[b]; l < blev[b + 1]; ++l) {
         for (int t = lptr[l] + wave; t < lptr[l + 1]; t += nwaves) {
             const int r = rows[t];
-            const int s0 = rp[r], len = rp[r + 1] - s0, d = dg[r] - s0;
+            const int s0 = rp(r), len = rp(r + 1) - s0, d = dg(r) - s0;
             if (lane < len) map[col[s0 + lane]] = (short)lane;
             wave_lds_sync();
             for (int kk = 0; kk < d; ++kk) {
                 const int k = col[s0 + kk];
-                const T lik = val[s0 + kk] * val[dg[k]];  // u_kk is stored inverted
-                const int ks = dg[k] + 1, ke = rp[k + 1];
+                const T lik = val[s0 + kk] * val[dg(k)];  // u_kk is stored inverted
+                const int ks = dg(k) + 1, ke = rp(k + 1);
                 for (int m = ks + lane; m < ke; m += 64) {
                     const int p = map[col[m]];
                     if (p > kk) val[s0 + p] -= lik * val[m];
@@ -427,9 +446,11 @@ __global__ void __launch_bounds__(kBjFactorThreads) bj_factor(
             if (lane < len) map[col[s0 + lane]] = -1;
             wave_lds_sync();
         }
+        if (!STAGED) __threadfence_block();
         __syncthreads();
     }
-    for (int k = threadIdx.x; k < nzb; k += blockDim.x) lu[z0 + k] = val[k];
+    if (STAGED)
+        for (int k = threadIdx.x; k < nzb; k += blockDim.x) lu[z0 + k] = val[k];
 }
 
 // Solve: one wavefront per block.  Only the block's solution lives in LDS (so
@@ -835,8 +856,9 @@ static int build_bj(fvp_csr *A) {
         A->lds_max = 64 * 1024;
     // per-kernel attributes (shared by every matrix): allow the device maximum;
     // each build checks its own block against it
-    for (const void *f : {(const void *)bj_factor<double>, (const void *)bj_solve<double>,
-                          (const void *)bj_factor<float>, (const void *)bj_solve<float>})
+    for (const void *f : {(const void *)bj_factor<double, true>, (const void *)bj_factor<double, false>,
+                          (const void *)bj_solve<double>, (const void *)bj_factor<float, true>,
+                          (const void *)bj_factor<float, false>, (const void *)bj_solve<float>})
         (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, A->lds_max);
     A->nblk = nb;
     A->bj = true;
@@ -860,22 +882,33 @@ static int precon_build_impl(fvp_csr *A, int type) {
             if (rc) { A->ptype = -1; return rc; }
         }
         const size_t vb = type == FVP_PRECON_BJILU0 ? sizeof(double) : sizeof(float);
-        const size_t need = bj_lds(A->b_maxnz, vb, A->b_maxrows);  // the factorisation's staging
-        if (need > (size_t)A->lds_max) {
-            set_error("block-Jacobi ILU(0): a block of " + std::to_string(A->bs) + " rows needs " +
-                      std::to_string(need) + " B of LDS, the device allows " + std::to_string(A->lds_max));
+        const size_t staged = bj_lds(A->b_maxnz, vb, A->b_maxrows);
+        const size_t maps = (size_t)(kBjFactorThreads / 64) * A->b_maxrows * sizeof(short);
+        if (maps > (size_t)A->lds_max) {
+            set_error("block-Jacobi ILU(0): blocks of " + std::to_string(A->b_maxrows) + " rows exceed the LDS");
             A->ptype = -1;
             return -1;
         }
-        if (type == FVP_PRECON_BJILU0)
-            bj_factor<double><<<A->nblk, kBjFactorThreads, A->b_lds, s>>>(A->b_roff.p, A->b_maxrows, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p,
-                                                              A->b_src.p, A->a.p, A->b_lev_l.p, A->b_lptr_l.p,
-                                                              A->b_rows_l.p, A->b_lu.p, A->flag.p);
-        else {
+        const bool stage = staged <= (size_t)A->lds_max;
+        if (type == FVP_PRECON_BJILU0) {
+            if (stage)
+                bj_factor<double, true><<<A->nblk, kBjFactorThreads, staged, s>>>(
+                    A->b_roff.p, A->b_maxrows, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p,
+                    A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p, A->b_lu.p, A->flag.p);
+            else
+                bj_factor<double, false><<<A->nblk, kBjFactorThreads, maps, s>>>(
+                    A->b_roff.p, A->b_maxrows, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p,
+                    A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p, A->b_lu.p, A->flag.p);
+        } else {
             if (!A->b_lu32.p && A->b_lu32.alloc(A->b_lu.n)) return -1;
-            bj_factor<float><<<A->nblk, kBjFactorThreads, bj_lds(A->b_maxnz, sizeof(float), A->b_maxrows), s>>>(
-                A->b_roff.p, A->b_maxrows, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p, A->b_lev_l.p, A->b_lptr_l.p,
-                A->b_rows_l.p, A->b_lu32.p, A->flag.p);
+            if (stage)
+                bj_factor<float, true><<<A->nblk, kBjFactorThreads, staged, s>>>(
+                    A->b_roff.p, A->b_maxrows, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p,
+                    A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p, A->b_lu32.p, A->flag.p);
+            else
+                bj_factor<float, false><<<A->nblk, kBjFactorThreads, maps, s>>>(
+                    A->b_roff.p, A->b_maxrows, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p,
+                    A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p, A->b_lu32.p, A->flag.p);
         }
     } else if (type == FVP_PRECON_JACOBI) {
         if (A->jac.n != (size_t)n && A->jac.alloc(n)) return -1;

@@ -228,6 +228,8 @@ static void preprocess(OrcProblem *p, double t, const double *h) {
     for (int z = 0; z < nz; ++z) {
         const double *zp = p->zp + z * ZP_COUNT;
         double phi = zp[ZP_PHI]; /* porosity() with alpha == 0 (fvmpor.h:269-272) */
+        /* a node appears once per zone: the zone's nodes update disjoint entries */
+#pragma omp parallel for schedule(static)
         for (int k = p->zn_ptr[z]; k < p->zn_ptr[z + 1]; ++k) {
             int i = p->zn_node[k];
             double wgt = p->zn_w[k];
@@ -243,6 +245,7 @@ static void preprocess(OrcProblem *p, double t, const double *h) {
         }
     }
     /* k_r at faces from the face element's zone, front then back (fvmpor.h:919-924) */
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < NFI; ++f) {
         const CvFace *c = &m->cf[f];
         int z = p->ezone[c->elem];
@@ -253,9 +256,11 @@ static void preprocess(OrcProblem *p, double t, const double *h) {
         p->krw[f] = v;
     }
     /* process_derivative_coefficients (fvmpor.h:930-945) + specific storage */
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < N; ++i) p->ahh[i] = (p->phi_v[i] * p->dSw_v[i] + p->Ss_v[i] * p->Sw_v[i]) * p->rho0;
 
     /* process_fluxes (fvmpor.h:681-759) */
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < NFI; ++f) {
         const CvFace *c = &m->cf[f];
         int z = p->ezone[c->elem];
@@ -305,6 +310,7 @@ void orc_residual(OrcProblem *p, double t, const double *h, const double *hp, do
     int N = m->N;
     preprocess(p, t, h);
     csr_mv(N, m->cv_ia, m->cv_ja, m->cv_w, p->M, res);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < N; ++i) res[i] -= hp[i] * p->ahh[i];
     for (int k = 0; k < p->ndir; ++k) {
         int i = p->dir_node[k];
