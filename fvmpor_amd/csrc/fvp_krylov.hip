@@ -521,16 +521,54 @@ __device__ __forceinline__ void bj_sweep(T *v, int base, int l0, int l1, const i
             bj_load_row<T, UPPER>(w, tt, p0, cnt, base, rows, ia, diag, ja, lu);
             bj_row_update<T, UPPER>(w, v, base, diag, ja, lu);
         }
-        __syncthreads();
+        // one wavefront: its LDS accesses are ordered, so a wavefront-scope
+        // barrier suffices and the next level's loads stay in flight across it
+        wave_lds_sync();
+    }
+}
+
+// Staged sweep: the block's factor, row pointers, diagonal offsets and level
+// row lists are in LDS; a thread loads its row's first kBjRow entries and their
+// solution values in parallel, so a level costs a few LDS round trips.
+template <typename T, bool UPPER>
+__device__ __forceinline__ void bj_sweep_lds(T *v, const T *val, const unsigned short *col,
+                                             const unsigned short *rp, const unsigned short *dg,
+                                             const unsigned short *rws, int l0, int l1,
+                                             const int *__restrict__ lptr) {
+    const int p00 = l0 < l1 ? lptr[l0] : 0;
+    for (int l = l0; l < l1; ++l) {
+        const int p0 = lptr[l] - p00, cnt = lptr[l + 1] - lptr[l];
+        for (int t = threadIdx.x; t < cnt; t += 64) {
+            const int r = rws[p0 + t];
+            const int k0 = UPPER ? dg[r] + 1 : rp[r], k1 = UPPER ? rp[r + 1] : dg[r];
+            T a[kBjRow];
+            int c[kBjRow];
+#pragma unroll
+            for (int u = 0; u < kBjRow; ++u) {
+                const bool ok = k0 + u < k1;
+                a[u] = ok ? val[k0 + u] : T(0);
+                c[u] = ok ? (int)col[k0 + u] : r;
+            }
+            T s = v[r];
+#pragma unroll
+            for (int u = 0; u < kBjRow; ++u) s -= a[u] * v[c[u]];
+            for (int k = k0 + kBjRow; k < k1; ++k) s -= val[k] * v[col[k]];
+            if (UPPER) s *= val[dg[r]];  // inverted diagonal
+            v[r] = s;
+        }
+        wave_lds_sync();
     }
 }
 
 // rowid: global row of each local row (nullptr: block b's rows are the
 // contiguous range starting at roff[b]); own: the block's owned local rows,
-// the only ones written back (restricted additive Schwarz).
-template <typename T>
+// the only ones written back (restricted additive Schwarz).  STAGED: the block
+// is staged in LDS (max_nnz, max_rows size the arrays); otherwise only the
+// solution is, and the factor streams from memory one level ahead.
+template <typename T, bool STAGED>
 __global__ void __launch_bounds__(64) bj_solve(const int *__restrict__ roff, const int *__restrict__ rowid,
-                                               const int *__restrict__ own, const int *__restrict__ ia,
+                                               const int *__restrict__ own, int max_nnz, int max_rows,
+                                               const int *__restrict__ ia,
                                                const unsigned short *__restrict__ ja, const int *__restrict__ diag,
                                                const T *__restrict__ lu, const int *__restrict__ blev_l,
                                                const int *__restrict__ lptr_l, const unsigned short *__restrict__ rows_l,
@@ -538,16 +576,47 @@ __global__ void __launch_bounds__(64) bj_solve(const int *__restrict__ roff, con
                                                const unsigned short *__restrict__ rows_u, const double *__restrict__ x,
                                                double *__restrict__ y) {
     extern __shared__ double smem[];
-    T *v = (T *)smem;
     const int b = blockIdx.x;
     const int base = roff[b];
     const int cnt = roff[b + 1] - base;
-    for (int r = threadIdx.x; r < cnt; r += blockDim.x) v[r] = (T)x[rowid ? rowid[base + r] : base + r];
-    __syncthreads();
-    bj_sweep<T, false>(v, base, blev_l[b], blev_l[b + 1], lptr_l, rows_l, ia, diag, ja, lu);
-    bj_sweep<T, true>(v, base, blev_u[b], blev_u[b + 1], lptr_u, rows_u, ia, diag, ja, lu);
+    T *v = (T *)smem;
+    if (STAGED) {
+        T *val = v + max_rows;
+        unsigned short *col = (unsigned short *)(val + max_nnz);
+        unsigned short *rp = col + max_nnz;
+        unsigned short *dg = rp + (max_rows + 1);
+        unsigned short *rl = dg + max_rows;
+        unsigned short *ru = rl + max_rows;
+        const int z0 = ia[base], nzb = ia[base + cnt] - z0;
+        const int pl = lptr_l[blev_l[b]], pu = lptr_u[blev_u[b]];
+        for (int k = threadIdx.x; k < nzb; k += 64) {
+            val[k] = lu[z0 + k];
+            col[k] = ja[z0 + k];
+        }
+        for (int r = threadIdx.x; r < cnt; r += 64) {
+            rp[r] = (unsigned short)(ia[base + r] - z0);
+            dg[r] = (unsigned short)(diag[base + r] - z0);
+            rl[r] = rows_l[pl + r];
+            ru[r] = rows_u[pu + r];
+            v[r] = (T)x[rowid ? rowid[base + r] : base + r];
+        }
+        if (threadIdx.x == 0) rp[cnt] = (unsigned short)nzb;
+        wave_lds_sync();
+        bj_sweep_lds<T, false>(v, val, col, rp, dg, rl, blev_l[b], blev_l[b + 1], lptr_l);
+        bj_sweep_lds<T, true>(v, val, col, rp, dg, ru, blev_u[b], blev_u[b + 1], lptr_u);
+    } else {
+        for (int r = threadIdx.x; r < cnt; r += 64) v[r] = (T)x[rowid ? rowid[base + r] : base + r];
+        wave_lds_sync();
+        bj_sweep<T, false>(v, base, blev_l[b], blev_l[b + 1], lptr_l, rows_l, ia, diag, ja, lu);
+        bj_sweep<T, true>(v, base, blev_u[b], blev_u[b + 1], lptr_u, rows_u, ia, diag, ja, lu);
+    }
     const int olo = own[2 * b], ohi = own[2 * b + 1];
-    for (int r = olo + threadIdx.x; r < ohi; r += blockDim.x) y[rowid ? rowid[base + r] : base + r] = (double)v[r];
+    for (int r = olo + threadIdx.x; r < ohi; r += 64) y[rowid ? rowid[base + r] : base + r] = (double)v[r];
+}
+
+// LDS of the staged solve
+static size_t bj_solve_lds(int maxnz, int maxrows, size_t value_bytes) {
+    return (size_t)maxnz * (value_bytes + 2) + (size_t)maxrows * (value_bytes + 2 * 4) + 2 + 8;
 }
 
 static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -857,8 +926,9 @@ static int build_bj(fvp_csr *A) {
     // per-kernel attributes (shared by every matrix): allow the device maximum;
     // each build checks its own block against it
     for (const void *f : {(const void *)bj_factor<double, true>, (const void *)bj_factor<double, false>,
-                          (const void *)bj_solve<double>, (const void *)bj_factor<float, true>,
-                          (const void *)bj_factor<float, false>, (const void *)bj_solve<float>})
+                          (const void *)bj_solve<double, true>, (const void *)bj_solve<double, false>,
+                          (const void *)bj_factor<float, true>, (const void *)bj_factor<float, false>,
+                          (const void *)bj_solve<float, true>, (const void *)bj_solve<float, false>})
         (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, A->lds_max);
     A->nblk = nb;
     A->bj = true;
@@ -953,15 +1023,26 @@ static int precon_apply_impl(fvp_csr *A, const double *x, double *y) {
         vmul<<<nblk(n, 256), 256, 0, s>>>(n, x, A->jac.p, y);
         break;
     case FVP_PRECON_BJILU0:
-        bj_solve<double><<<A->nblk, 64, sizeof(double) * A->b_maxrows, s>>>(
-            A->b_roff.p, A->b_rowid.p, A->b_own.p, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_lu.p, A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p,
-            A->b_lev_u.p, A->b_lptr_u.p, A->b_rows_u.p, x, y);
+    case FVP_PRECON_BJILU0_F32: {
+        const bool f64 = A->ptype == FVP_PRECON_BJILU0;
+        const size_t vb = f64 ? sizeof(double) : sizeof(float);
+        const size_t staged = bj_solve_lds(A->b_maxnz, A->b_maxrows, vb);
+        const bool stage = staged <= (size_t)A->lds_max;
+        const size_t lds = stage ? staged : vb * A->b_maxrows;
+#define FVP_BJ_SOLVE(T, S, LU)                                                                                    \
+    bj_solve<T, S><<<A->nblk, 64, lds, s>>>(A->b_roff.p, A->b_rowid.p, A->b_own.p, A->b_maxnz, A->b_maxrows,      \
+                                             A->b_ia.p, A->b_ja.p, A->b_diag.p, LU, A->b_lev_l.p, A->b_lptr_l.p,  \
+                                             A->b_rows_l.p, A->b_lev_u.p, A->b_lptr_u.p, A->b_rows_u.p, x, y)
+        if (f64) {
+            if (stage) FVP_BJ_SOLVE(double, true, A->b_lu.p);
+            else FVP_BJ_SOLVE(double, false, A->b_lu.p);
+        } else {
+            if (stage) FVP_BJ_SOLVE(float, true, A->b_lu32.p);
+            else FVP_BJ_SOLVE(float, false, A->b_lu32.p);
+        }
+#undef FVP_BJ_SOLVE
         break;
-    case FVP_PRECON_BJILU0_F32:
-        bj_solve<float><<<A->nblk, 64, sizeof(float) * A->b_maxrows, s>>>(
-            A->b_roff.p, A->b_rowid.p, A->b_own.p, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_lu32.p, A->b_lev_l.p, A->b_lptr_l.p,
-            A->b_rows_l.p, A->b_lev_u.p, A->b_lptr_u.p, A->b_rows_u.p, x, y);
-        break;
+    }
     case FVP_PRECON_ILU0: {
         const int L = 8;
         for (size_t l = 0; l + 1 < A->lptr_h.size(); ++l) {
