@@ -82,7 +82,7 @@ struct fvp_ctx {
     // relative to the tile's (t_xptr); per export its slot (x_ilist), slots
     // grouped by target node (x_iptr: CSR over nodes); the slot values
     fvp::DevBuf<unsigned short> t_xoff;
-    fvp::DevBuf<int> t_xptr, x_iptr, x_ilist;
+    fvp::DevBuf<int> t_xptr, x_iptr, x_ilist, x_tgt;  // x_tgt: target node of each slot
     fvp::DevBuf<double> xf;
     long nexport = 0;
     fvp::DevBuf<int> t_wptr, t_whalo, t_vptr, t_tz;  // window halo nodes, visits, zones per tile

@@ -45,6 +45,7 @@ struct fvp_csr {
     bool bj = false;
     int bs = FVP_BLOCK_ROWS;                // owned rows per block
     int overlap = 0;                        // layers of neighbouring rows added to each block (RAS)
+    bool bj_stage = true;                   // stage blocks in LDS for the solve when they fit
     int nblk = 0, b_maxnz = 0, b_maxrows = 0, lds_max = 0;
     DevBuf<int> b_roff, b_rowid, b_own;     // per block: first local row; global row of each local row; owned range
     size_t b_lds = 0;
@@ -365,6 +366,10 @@ __global__ void lincomb(long n, double *__restrict__ out, const double *__restri
 // Rows of one level run on different wavefronts.
 constexpr int kBjFactorThreads = 256;
 
+// compiler-only ordering of one wavefront's LDS accesses (the LDS executes a
+// wavefront's accesses in issue order; no wait for outstanding memory loads)
+__device__ __forceinline__ void wave_lds_order() { asm volatile("" ::: "memory"); }
+
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -521,9 +526,10 @@ __device__ __forceinline__ void bj_sweep(T *v, int base, int l0, int l1, const i
             bj_load_row<T, UPPER>(w, tt, p0, cnt, base, rows, ia, diag, ja, lu);
             bj_row_update<T, UPPER>(w, v, base, diag, ja, lu);
         }
-        // one wavefront: its LDS accesses are ordered, so a wavefront-scope
-        // barrier suffices and the next level's loads stay in flight across it
-        wave_lds_sync();
+        // one wavefront: the LDS executes its accesses in order, so only the
+        // compiler must keep them in program order; the next level's loads
+        // stay in flight
+        wave_lds_order();
     }
 }
 
@@ -556,25 +562,27 @@ __device__ __forceinline__ void bj_sweep_lds(T *v, const T *val, const unsigned 
             if (UPPER) s *= val[dg[r]];  // inverted diagonal
             v[r] = s;
         }
-        wave_lds_sync();
+        wave_lds_order();
     }
 }
 
 // rowid: global row of each local row (nullptr: block b's rows are the
 // contiguous range starting at roff[b]); own: the block's owned local rows,
-// the only ones written back (restricted additive Schwarz).  STAGED: the block
-// is staged in LDS (max_nnz, max_rows size the arrays); otherwise only the
-// solution is, and the factor streams from memory one level ahead.
+// the only ones written back (restricted additive Schwarz).
+// STAGED (workgroup of 4 wavefronts): all four stage the block (factor, row
+// pointers, diagonal offsets, level row lists, right-hand side) in LDS, then
+// the first wavefront walks the levels alone.  Otherwise (one wavefront) only
+// the solution is in LDS and the factor streams from memory a level ahead.
 template <typename T, bool STAGED>
-__global__ void __launch_bounds__(64) bj_solve(const int *__restrict__ roff, const int *__restrict__ rowid,
-                                               const int *__restrict__ own, int max_nnz, int max_rows,
-                                               const int *__restrict__ ia,
-                                               const unsigned short *__restrict__ ja, const int *__restrict__ diag,
-                                               const T *__restrict__ lu, const int *__restrict__ blev_l,
-                                               const int *__restrict__ lptr_l, const unsigned short *__restrict__ rows_l,
-                                               const int *__restrict__ blev_u, const int *__restrict__ lptr_u,
-                                               const unsigned short *__restrict__ rows_u, const double *__restrict__ x,
-                                               double *__restrict__ y) {
+__global__ void __launch_bounds__(256) bj_solve(const int *__restrict__ roff, const int *__restrict__ rowid,
+                                                const int *__restrict__ own, int max_nnz, int max_rows,
+                                                const int *__restrict__ ia,
+                                                const unsigned short *__restrict__ ja, const int *__restrict__ diag,
+                                                const T *__restrict__ lu, const int *__restrict__ blev_l,
+                                                const int *__restrict__ lptr_l, const unsigned short *__restrict__ rows_l,
+                                                const int *__restrict__ blev_u, const int *__restrict__ lptr_u,
+                                                const unsigned short *__restrict__ rows_u, const double *__restrict__ x,
+                                                double *__restrict__ y) {
     extern __shared__ double smem[];
     const int b = blockIdx.x;
     const int base = roff[b];
@@ -589,11 +597,13 @@ __global__ void __launch_bounds__(64) bj_solve(const int *__restrict__ roff, con
         unsigned short *ru = rl + max_rows;
         const int z0 = ia[base], nzb = ia[base + cnt] - z0;
         const int pl = lptr_l[blev_l[b]], pu = lptr_u[blev_u[b]];
-        for (int k = threadIdx.x; k < nzb; k += 64) {
+#pragma unroll 4
+        for (int k = threadIdx.x; k < nzb; k += blockDim.x) {
             val[k] = lu[z0 + k];
             col[k] = ja[z0 + k];
         }
-        for (int r = threadIdx.x; r < cnt; r += 64) {
+#pragma unroll 2
+        for (int r = threadIdx.x; r < cnt; r += blockDim.x) {
             rp[r] = (unsigned short)(ia[base + r] - z0);
             dg[r] = (unsigned short)(diag[base + r] - z0);
             rl[r] = rows_l[pl + r];
@@ -601,12 +611,13 @@ __global__ void __launch_bounds__(64) bj_solve(const int *__restrict__ roff, con
             v[r] = (T)x[rowid ? rowid[base + r] : base + r];
         }
         if (threadIdx.x == 0) rp[cnt] = (unsigned short)nzb;
-        wave_lds_sync();
+        __syncthreads();
+        if (threadIdx.x >= 64) return;  // the level walk is one wavefront's
         bj_sweep_lds<T, false>(v, val, col, rp, dg, rl, blev_l[b], blev_l[b + 1], lptr_l);
         bj_sweep_lds<T, true>(v, val, col, rp, dg, ru, blev_u[b], blev_u[b + 1], lptr_u);
     } else {
         for (int r = threadIdx.x; r < cnt; r += 64) v[r] = (T)x[rowid ? rowid[base + r] : base + r];
-        wave_lds_sync();
+        wave_lds_order();
         bj_sweep<T, false>(v, base, blev_l[b], blev_l[b + 1], lptr_l, rows_l, ia, diag, ja, lu);
         bj_sweep<T, true>(v, base, blev_u[b], blev_u[b + 1], lptr_u, rows_u, ia, diag, ja, lu);
     }
@@ -1027,12 +1038,11 @@ static int precon_apply_impl(fvp_csr *A, const double *x, double *y) {
         const bool f64 = A->ptype == FVP_PRECON_BJILU0;
         const size_t vb = f64 ? sizeof(double) : sizeof(float);
         const size_t staged = bj_solve_lds(A->b_maxnz, A->b_maxrows, vb);
-        const bool stage = staged <= (size_t)A->lds_max;
-        const size_t lds = stage ? staged : vb * A->b_maxrows;
+        const bool stage = A->bj_stage && staged <= (size_t)A->lds_max;
 #define FVP_BJ_SOLVE(T, S, LU)                                                                                    \
-    bj_solve<T, S><<<A->nblk, 64, lds, s>>>(A->b_roff.p, A->b_rowid.p, A->b_own.p, A->b_maxnz, A->b_maxrows,      \
-                                             A->b_ia.p, A->b_ja.p, A->b_diag.p, LU, A->b_lev_l.p, A->b_lptr_l.p,  \
-                                             A->b_rows_l.p, A->b_lev_u.p, A->b_lptr_u.p, A->b_rows_u.p, x, y)
+    bj_solve<T, S><<<A->nblk, S ? 256 : 64, S ? staged : vb * A->b_maxrows, s>>>(                                 \
+        A->b_roff.p, A->b_rowid.p, A->b_own.p, A->b_maxnz, A->b_maxrows, A->b_ia.p, A->b_ja.p, A->b_diag.p, LU,   \
+        A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p, A->b_lev_u.p, A->b_lptr_u.p, A->b_rows_u.p, x, y)
         if (f64) {
             if (stage) FVP_BJ_SOLVE(double, true, A->b_lu.p);
             else FVP_BJ_SOLVE(double, false, A->b_lu.p);

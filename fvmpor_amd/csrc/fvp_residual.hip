@@ -99,27 +99,45 @@ __device__ constexpr int edge_b(int j) { return j < 3 ? (j + 1) % 3 : 3; }
 // face fluxes (back -> front) are stored per (element, local edge) in qface
 // for the edge-flux sign test.
 // zk: per-zone K_xx, K_yy, K_zz in LDS.  JV: the state is psi + sig v.
+// The per-element record: nodes, zone, conductivity multiplier, upwind and
+// orientation bits (loaded ahead of use by the owner kernel's pipeline).
+struct ElemRec {
+    int4 c4;
+    int z;
+    double km;
+    unsigned mask, orient;
+};
+
+template <int MODE>
+__device__ __forceinline__ ElemRec load_elem(int e, const int4 *__restrict__ conn, const uint8_t *__restrict__ ezone,
+                                             const double *__restrict__ kmul, const uint8_t *__restrict__ emask,
+                                             const uint8_t *__restrict__ eorient) {
+    ElemRec r;
+    r.c4 = conn[e];
+    r.z = ezone[e];
+    r.km = kmul[e];
+    r.mask = MODE == 0 ? emask[e] : 0u;
+    r.orient = MODE != 0 ? eorient[e] : 0u;
+    return r;
+}
+
 template <int DIM, int MODE, bool JV = false>
-__device__ __forceinline__ void element_fluxes(int e, int N, const double *zk, double rho0,
-                                               const int4 *__restrict__ conn, const uint8_t *__restrict__ ezone,
-                                               const double *__restrict__ kmul, const uint8_t *__restrict__ emask,
-                                               const uint8_t *__restrict__ eorient,
-                                               const double *__restrict__ xyz, const double *__restrict__ psi,
-                                               const double *__restrict__ v, double sig,
-                                               const double *__restrict__ kr, double *__restrict__ qface,
-                                               int n[4], double f[4]) {
+__device__ __forceinline__ void element_fluxes_rec(int e, const ElemRec &rec, int N, const double *zk, double rho0,
+                                                   const double *__restrict__ xyz, const double *__restrict__ psi,
+                                                   const double *__restrict__ v, double sig,
+                                                   const double *__restrict__ kr, double *__restrict__ qface,
+                                                   int n[4], double f[4]) {
     constexpr int NPE = DIM + 1;
     constexpr int NED = DIM == 2 ? 3 : 6;
-    int4 c4 = conn[e];
-    n[0] = c4.x; n[1] = c4.y; n[2] = c4.z; n[3] = c4.w;
+    n[0] = rec.c4.x; n[1] = rec.c4.y; n[2] = rec.c4.z; n[3] = rec.c4.w;
     double px[NPE], py[NPE], pz[NPE], h[NPE];
 #pragma unroll
     for (int k = 0; k < NPE; ++k) {
         load_pt(xyz, n[k], px[k], py[k], pz[k]);
         h[k] = JV ? psi[n[k]] + sig * v[n[k]] : psi[n[k]];
     }
-    const int z = ezone[e];
-    const double km = kmul[e];
+    const int z = rec.z;
+    const double km = rec.km;
     double G[NPE][3];  // adjugate rows: grad N_k * det
     double det;
     if (DIM == 3) {
@@ -160,8 +178,8 @@ __device__ __forceinline__ void element_fluxes(int e, int N, const double *zk, d
     const double sc = (det > 0 ? 1.0 : -1.0) / (DIM == 3 ? 24.0 : 6.0);
 #pragma unroll
     for (int k = 0; k < 4; ++k) f[k] = 0.;
-    const unsigned mask = MODE == 0 ? emask[e] : 0u;
-    const unsigned orient = MODE != 0 ? eorient[e] : 0u;
+    const unsigned mask = rec.mask;
+    const unsigned orient = rec.orient;
 #pragma unroll
     for (int j = 0; j < NED; ++j) {
         const int a = edge_a(j);
@@ -183,6 +201,19 @@ __device__ __forceinline__ void element_fluxes(int e, int N, const double *zk, d
         f[a] -= M;
         f[b] += M;
     }
+}
+
+template <int DIM, int MODE, bool JV = false>
+__device__ __forceinline__ void element_fluxes(int e, int N, const double *zk, double rho0,
+                                               const int4 *__restrict__ conn, const uint8_t *__restrict__ ezone,
+                                               const double *__restrict__ kmul, const uint8_t *__restrict__ emask,
+                                               const uint8_t *__restrict__ eorient,
+                                               const double *__restrict__ xyz, const double *__restrict__ psi,
+                                               const double *__restrict__ v, double sig,
+                                               const double *__restrict__ kr, double *__restrict__ qface,
+                                               int n[4], double f[4]) {
+    const ElemRec rec = load_elem<MODE>(e, conn, ezone, kmul, emask, eorient);
+    element_fluxes_rec<DIM, MODE, JV>(e, rec, N, zk, rho0, xyz, psi, v, sig, kr, qface, n, f);
 }
 
 // Atomic scatter variant: the averaged-weight residual (MODE 1, before the
@@ -237,11 +268,17 @@ __global__ void __launch_bounds__(1024) owner_kernel(
     if (threadIdx.x < 3 * zt.nz) zk[threadIdx.x] = zt.p[threadIdx.x / 3][ZK_X + threadIdx.x % 3];
     __syncthreads();
     const int *xd = xdst + xptr[tile];
-    for (int e = eptr[tile] + threadIdx.x; e < eptr[tile + 1]; e += blockDim.x) {
+    const int e1 = eptr[tile + 1];
+    // the next element's record is loaded while the current one is evaluated
+    int e = eptr[tile] + threadIdx.x;
+    ElemRec nxt;
+    if (e < e1) nxt = load_elem<0>(e, conn, ezone, kmul, emask, nullptr);
+    for (; e < e1; e += blockDim.x) {
+        const ElemRec cur = nxt;
+        if (e + (int)blockDim.x < e1) nxt = load_elem<0>(e + blockDim.x, conn, ezone, kmul, emask, nullptr);
         int n[4];
         double f[4];
-        element_fluxes<DIM, 0, JV>(e, N, zk, rho0, conn, ezone, kmul, emask, nullptr, xyz, psi, jv.v, sig, kr,
-                                   nullptr, n, f);
+        element_fluxes_rec<DIM, 0, JV>(e, cur, N, zk, rho0, xyz, psi, jv.v, sig, kr, nullptr, n, f);
         int x = xoff[e];
 #pragma unroll
         for (int q = 0; q < DIM + 1; ++q) {
@@ -255,25 +292,43 @@ __global__ void __launch_bounds__(1024) owner_kernel(
     for (int i = threadIdx.x; i < nown; i += blockDim.x) acc_g[lo + i] = sm[i];
 }
 
-// Finalisation of the element-owner residual: the node's own sum plus the
-// contributions exported to it, then F = acc / V - psi' ahh (JV: the
-// difference quotient).
+// Finalisation of the element-owner residual, one workgroup per tile: the
+// tile's imports are a contiguous slot range (slots are grouped by target
+// node), read coalesced and added to the nodes' own sums in LDS, then
+// F = acc / V - psi' ahh (JV: the difference quotient) and the Dirichlet rows.
 template <bool JV>
-__global__ void owner_finalize(int N, const double *__restrict__ acc, const int *__restrict__ iptr,
-                               const double *__restrict__ xf,
-                               const double *__restrict__ invV, const double *__restrict__ psip,
-                               const double *__restrict__ cap, double *__restrict__ F, JvArgs jv) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    double a = acc[i];
-    for (int k = iptr[i]; k < iptr[i + 1]; ++k) a += xf[k];
-    if (JV) {
-        const double sig = jv.sig[0];
-        const double yp = psip[i] + jv.cj * (sig * jv.v[i]);
-        const double F1 = a * invV[i] - yp * cap[i];
-        F[i] = (F1 - jv.F0[i]) / sig;
-    } else
-        F[i] = a * invV[i] - psip[i] * cap[i];
+__global__ void __launch_bounds__(256) owner_finalize(
+    int N, int ntiles, int tile_nodes, const double *__restrict__ acc, const int *__restrict__ iptr,
+    const int *__restrict__ xtgt, const double *__restrict__ xf, const double *__restrict__ invV,
+    const double *__restrict__ psi, const double *__restrict__ psip, const double *__restrict__ cap,
+    const int *__restrict__ dptr, const int *__restrict__ dnode, const double *__restrict__ dval,
+    double *__restrict__ F, JvArgs jv) {
+    extern __shared__ double sm[];
+    const int tile = blockIdx.x;
+    const int lo = tile * tile_nodes, hi = min(N, lo + tile_nodes), nown = hi - lo;
+    for (int i = threadIdx.x; i < nown; i += blockDim.x) sm[i] = acc[lo + i];
+    __syncthreads();
+    for (int k = iptr[lo] + threadIdx.x; k < iptr[hi]; k += blockDim.x) atomicAdd(&sm[xtgt[k] - lo], xf[k]);
+    __syncthreads();
+    const double sig = JV ? jv.sig[0] : 0.;
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const double a = sm[i - lo];
+        if (JV) {
+            const double yp = psip[i] + jv.cj * (sig * jv.v[i]);
+            const double F1 = a * invV[i] - yp * cap[i];
+            F[i] = (F1 - jv.F0[i]) / sig;
+        } else
+            F[i] = a * invV[i] - psip[i] * cap[i];
+    }
+    __syncthreads();
+    for (int k = dptr[tile] + threadIdx.x; k < dptr[tile + 1]; k += blockDim.x) {
+        const int i = dnode[k];
+        if (JV) {
+            const double y = psi[i] + sig * jv.v[i];
+            F[i] = ((dval[k] - y) - jv.F0[i]) / sig;
+        } else
+            F[i] = dval[k] - psi[i];
+    }
 }
 
 // LDS-staged owner-computes residual.  A workgroup owns tile_nodes nodes; it
@@ -569,16 +624,15 @@ static void launch_tile(fvp_ctx *c, const double *psi, const double *psip, doubl
         const int N = c->N;
         prof_mark(c, 0, 0);
         // one workgroup per tile, wide enough that large tiles still fill the CUs
-        const int threads = std::min(1024, std::max(256, c->tile_nodes / 2));
+        const int threads = c->tile_nodes >= 1024 ? 512 : 256;
         owner_kernel<DIM, JV><<<c->xcd_map ? per * 8 : c->ntiles, threads, sizeof(double) * c->tile_nodes, s>>>(
             N, c->ntiles, c->tile_nodes, c->xcd_map ? per : 0, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
             c->emask.p, c->xyz.p, psi, c->kr.p, c->t_eptr.p, c->t_bptr.p, c->bnode.p, c->bval.p, c->t_xoff.p,
             c->t_xptr.p, c->x_ilist.p, c->xf.p, c->acc.p, jv);
         prof_mark(c, 0, 1);
-        owner_finalize<JV><<<blocks(N, 256), 256, 0, s>>>(N, c->acc.p, c->x_iptr.p, c->xf.p, c->invV.p, psip,
-                                                          c->cap.p, F, jv);
-        if (c->ndir)
-            dirichlet_kernel<JV><<<blocks(c->ndir, 256), 256, 0, s>>>(c->ndir, c->dnode.p, c->dval.p, psi, F, jv);
+        owner_finalize<JV><<<c->ntiles, 256, sizeof(double) * c->tile_nodes, s>>>(
+            N, c->ntiles, c->tile_nodes, c->acc.p, c->x_iptr.p, c->x_tgt.p, c->xf.p, c->invV.p, psi, psip, c->cap.p,
+            c->t_dptr.p, c->dnode.p, c->dval.p, F, jv);
         return;
     }
     prof_mark(c, 0, 0);

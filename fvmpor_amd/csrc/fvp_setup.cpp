@@ -226,18 +226,22 @@ int build_tiles(fvp_ctx *c, int TILE) {
             }
             // slots are stored grouped by target node (exporting element order within
             // a node), so the finalisation reads each node's imports contiguously
-            std::vector<int> iptr(N + 1, 0), xdst(target.size());
+            std::vector<int> iptr(N + 1, 0), xdst(target.size()), xtgt(target.size());
             for (int tn : target) iptr[tn + 1]++;
             for (int i = 0; i < N; ++i) iptr[i + 1] += iptr[i];
             std::vector<int> fill(iptr.begin(), iptr.end() - 1);
-            for (size_t k = 0; k < target.size(); ++k) xdst[k] = fill[target[k]]++;
+            for (size_t k = 0; k < target.size(); ++k) {
+                xdst[k] = fill[target[k]]++;
+                xtgt[xdst[k]] = target[k];
+            }
             c->nexport = (long)target.size();
             if (c->t_xoff.upload(xoff.data(), T) || c->t_xptr.upload(xptr.data(), nt + 1) ||
                 c->x_iptr.upload(iptr.data(), N + 1) || c->x_ilist.upload(xdst.data(), xdst.size()) ||
-                c->xf.alloc(std::max<size_t>(1, target.size())))
+                c->x_tgt.upload(xtgt.data(), xtgt.size()) || c->xf.alloc(std::max<size_t>(1, target.size())))
                 return -1;
         } else {
             c->t_xoff.release(); c->t_xptr.release(); c->x_iptr.release(); c->x_ilist.release(); c->xf.release();
+            c->x_tgt.release();
             c->nexport = 0;
         }
     }
@@ -575,6 +579,7 @@ extern "C" int fvp_destroy(fvp_ctx *c) {
     c->dval.release(); c->t_wptr.release(); c->t_whalo.release(); c->t_vptr.release(); c->t_lconn.release();
     c->t_tz.release(); c->t_eptr.release(); c->t_hptr.release(); c->t_hlist.release(); c->t_bptr.release();
     c->t_dptr.release(); c->t_xoff.release(); c->t_xptr.release(); c->x_iptr.release(); c->x_ilist.release();
+    c->x_tgt.release();
     c->xf.release(); c->eedge.release(); c->slot_ptr.release(); c->slot.release(); c->qface.release();
     c->ea.release(); c->eflux.release(); c->wf.release(); c->tmp1.release(); c->tmp2.release();
     c->tmp3.release(); c->red.release();
