@@ -45,7 +45,6 @@ struct fvp_csr {
     bool bj = false;
     int bs = FVP_BLOCK_ROWS;                // owned rows per block
     int overlap = 0;                        // layers of neighbouring rows added to each block (RAS)
-    bool bj_stage = true;                   // stage blocks in LDS for the solve when they fit
     int nblk = 0, b_maxnz = 0, b_maxrows = 0, lds_max = 0;
     DevBuf<int> b_roff, b_rowid, b_own;     // per block: first local row; global row of each local row; owned range
     size_t b_lds = 0;
@@ -53,6 +52,14 @@ struct fvp_csr {
     DevBuf<unsigned short> b_ja;            // local column
     DevBuf<double> b_lu;
     DevBuf<float> b_lu32;                   // FP32 factor (FVP_PRECON_BJILU0_F32)
+    // solve layout: one slot per (row, triangle) in level order (the order of
+    // b_rows_l / b_rows_u); per slot the first kBjRow entries of the row's L
+    // (U) part as padded columns and values, the row's first entry and count
+    DevBuf<unsigned short> sl_col, su_col;
+    DevBuf<double> sl_val, su_val, su_dinv;
+    DevBuf<float> sl_val32, su_val32, su_dinv32;
+    DevBuf<int> sl_k0, su_k0;
+    DevBuf<unsigned short> sl_kn, su_kn;
     DevBuf<int> b_lev_l, b_lev_u;           // per block: level pointer range [b_lev[b], b_lev[b+1]) into *_lptr
     DevBuf<int> b_lptr_l, b_lptr_u;         // level -> rows range in *_rows
     DevBuf<unsigned short> b_rows_l, b_rows_u;  // local rows grouped by level
@@ -364,7 +371,7 @@ __global__ void lincomb(long n, double *__restrict__ out, const double *__restri
 // walk row k's U part and subtract l_ik u_kj from a_ij where (i, j) is in the
 // pattern, found through a per-wavefront column -> position map of row i.
 // Rows of one level run on different wavefronts.
-constexpr int kBjFactorThreads = 256;
+constexpr int kBjFactorThreads = 512;
 
 // compiler-only ordering of one wavefront's LDS accesses (the LDS executes a
 // wavefront's accesses in issue order; no wait for outstanding memory loads)
@@ -460,174 +467,138 @@ __global__ void __launch_bounds__(kBjFactorThreads) bj_factor(
 
 // Solve: one wavefront per block.  Only the block's solution lives in LDS (so
 // that every block of the matrix is resident at once and the CU interleaves
-// their dependency chains); the factor streams from memory.  Levels are walked
-// with a thread per row; the next level's row (index, bounds and first kBjRow
-// entries) is loaded before the current level's barrier so that only the LDS
-// reads of the solution and the FMA chain sit on the critical path.  The
-// entries of a row are subtracted in column order, as the sequential solve
-// does.  T is the factor's arithmetic type (double, or float for the FP32
-// preconditioner); x and y stay FP64.
+// their dependency chains); the factor streams from a level-ordered copy (one
+// slot per row and triangle, its first kBjRow entries padded), whose addresses
+// are known in advance, so the slots of the next levels are loaded ahead and
+// only the LDS reads of the solution and the FMA chain sit on the critical
+// path.  The entries of a row are subtracted in column order, as the
+// sequential solve does.  T is the factor's arithmetic type (double, or float
+// for the FP32 preconditioner); x and y stay FP64.
 constexpr int kBjRow = 8;
 
+// After a factorisation: copy the first kBjRow entries of every slot's row part
+// (zero beyond the row's count) and, for U, the row's inverted diagonal into
+// the level-ordered solve layout.
 template <typename T>
-struct BjRow {
-    int r = -1, k0 = 0, k1 = 0;
+__global__ void bj_pack(long S, const int *__restrict__ k0, const unsigned short *__restrict__ kn,
+                        const T *__restrict__ lu, T *__restrict__ val, T *__restrict__ dinv) {
+    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const int a = k0[s], n = kn[s];
+#pragma unroll
+    for (int u = 0; u < kBjRow; ++u) val[s * kBjRow + u] = u < n ? lu[a + u] : T(0);
+    if (dinv) dinv[s] = lu[a - 1];  // U part starts right after the diagonal
+}
+
+template <typename T>
+struct BjSlot {
+    int r, k0, kn;
     T a[kBjRow];
-    int c[kBjRow];
+    unsigned short c[kBjRow];
+    T d;
 };
 
 template <typename T, bool UPPER>
-__device__ __forceinline__ void bj_load_row(BjRow<T> &w, int t, int p0, int cnt, int base,
-                                            const unsigned short *__restrict__ rows, const int *__restrict__ ia,
-                                            const int *__restrict__ diag, const unsigned short *__restrict__ ja,
-                                            const T *__restrict__ lu) {
-    w.r = -1;
-    if (t >= cnt) return;
-    w.r = rows[p0 + t];
-    const int gi = base + w.r;
-    w.k0 = UPPER ? diag[gi] + 1 : ia[gi];
-    w.k1 = UPPER ? ia[gi + 1] : diag[gi];
+__device__ __forceinline__ void bj_slot_load(BjSlot<T> &w, long s, const unsigned short *__restrict__ rows,
+                                             const unsigned short *__restrict__ col, const T *__restrict__ val,
+                                             const int *__restrict__ k0, const unsigned short *__restrict__ kn,
+                                             const T *__restrict__ dinv) {
+    w.r = rows[s];
+    w.k0 = k0[s];
+    w.kn = kn[s];
 #pragma unroll
     for (int u = 0; u < kBjRow; ++u) {
-        const int k = w.k0 + u;
-        const bool ok = k < w.k1;
-        w.a[u] = ok ? lu[k] : T(0);
-        w.c[u] = ok ? (int)ja[k] : w.r;  // padding: 0 * v[r]
+        w.a[u] = val[s * kBjRow + u];
+        w.c[u] = col[s * kBjRow + u];
     }
+    w.d = UPPER ? dinv[s] : T(0);
 }
 
 template <typename T, bool UPPER>
-__device__ __forceinline__ void bj_row_update(const BjRow<T> &w, T *v, int base, const int *__restrict__ diag,
-                                              const unsigned short *__restrict__ ja, const T *__restrict__ lu) {
+__device__ __forceinline__ void bj_slot_update(const BjSlot<T> &w, T *v, const unsigned short *__restrict__ ja,
+                                               const T *__restrict__ lu) {
     T s = v[w.r];
 #pragma unroll
     for (int u = 0; u < kBjRow; ++u) s -= w.a[u] * v[w.c[u]];
-    for (int k = w.k0 + kBjRow; k < w.k1; ++k) s -= lu[k] * v[ja[k]];
-    if (UPPER) s *= lu[diag[base + w.r]];  // inverted diagonal
+    for (int k = w.k0 + kBjRow; k < w.k0 + w.kn; ++k) s -= lu[k] * v[ja[k]];
+    if (UPPER) s *= w.d;  // inverted diagonal
     v[w.r] = s;
 }
 
+// One triangle of one block: thread t takes the t-th slot of each level; the
+// slots of the next kBjAhead levels are loaded ahead (their addresses do not
+// depend on the solution), so the level chain waits only on LDS.
+constexpr int kBjAhead = 4;
+
 template <typename T, bool UPPER>
-__device__ __forceinline__ void bj_sweep(T *v, int base, int l0, int l1, const int *__restrict__ lptr,
-                                         const unsigned short *__restrict__ rows, const int *__restrict__ ia,
-                                         const int *__restrict__ diag, const unsigned short *__restrict__ ja,
+__device__ __forceinline__ void bj_sweep(T *v, int l0, int l1, const int *__restrict__ lptr,
+                                         const unsigned short *__restrict__ rows,
+                                         const unsigned short *__restrict__ col, const T *__restrict__ val,
+                                         const int *__restrict__ k0, const unsigned short *__restrict__ kn,
+                                         const T *__restrict__ dinv, const unsigned short *__restrict__ ja,
                                          const T *__restrict__ lu) {
     const int t = threadIdx.x;
-    BjRow<T> nxt;
-    if (l0 < l1) bj_load_row<T, UPPER>(nxt, t, lptr[l0], lptr[l0 + 1] - lptr[l0], base, rows, ia, diag, ja, lu);
-    for (int l = l0; l < l1; ++l) {
-        const BjRow<T> cur = nxt;
-        const int p0 = lptr[l], cnt = lptr[l + 1] - p0;
-        if (l + 1 < l1)
-            bj_load_row<T, UPPER>(nxt, t, lptr[l + 1], lptr[l + 2] - lptr[l + 1], base, rows, ia, diag, ja, lu);
-        if (cur.r >= 0) bj_row_update<T, UPPER>(cur, v, base, diag, ja, lu);
-        for (int tt = t + 64; tt < cnt; tt += 64) {  // levels wider than a wavefront
-            BjRow<T> w;
-            bj_load_row<T, UPPER>(w, tt, p0, cnt, base, rows, ia, diag, ja, lu);
-            bj_row_update<T, UPPER>(w, v, base, diag, ja, lu);
-        }
-        // one wavefront: the LDS executes its accesses in order, so only the
-        // compiler must keep them in program order; the next level's loads
-        // stay in flight
-        wave_lds_order();
+    BjSlot<T> ring[kBjAhead];
+    bool have[kBjAhead];
+#pragma unroll
+    for (int d = 0; d < kBjAhead; ++d) {
+        const int l = l0 + d;
+        have[d] = l < l1 && t < lptr[l + 1] - lptr[l];
+        if (have[d]) bj_slot_load<T, UPPER>(ring[d], lptr[l] + t, rows, col, val, k0, kn, dinv);
     }
-}
-
-// Staged sweep: the block's factor, row pointers, diagonal offsets and level
-// row lists are in LDS; a thread loads its row's first kBjRow entries and their
-// solution values in parallel, so a level costs a few LDS round trips.
-template <typename T, bool UPPER>
-__device__ __forceinline__ void bj_sweep_lds(T *v, const T *val, const unsigned short *col,
-                                             const unsigned short *rp, const unsigned short *dg,
-                                             const unsigned short *rws, int l0, int l1,
-                                             const int *__restrict__ lptr) {
-    const int p00 = l0 < l1 ? lptr[l0] : 0;
-    for (int l = l0; l < l1; ++l) {
-        const int p0 = lptr[l] - p00, cnt = lptr[l + 1] - lptr[l];
-        for (int t = threadIdx.x; t < cnt; t += 64) {
-            const int r = rws[p0 + t];
-            const int k0 = UPPER ? dg[r] + 1 : rp[r], k1 = UPPER ? rp[r + 1] : dg[r];
-            T a[kBjRow];
-            int c[kBjRow];
+    for (int lb = l0; lb < l1; lb += kBjAhead) {
 #pragma unroll
-            for (int u = 0; u < kBjRow; ++u) {
-                const bool ok = k0 + u < k1;
-                a[u] = ok ? val[k0 + u] : T(0);
-                c[u] = ok ? (int)col[k0 + u] : r;
+        for (int d = 0; d < kBjAhead; ++d) {
+            const int l = lb + d;
+            if (l < l1) {
+                const int p0 = lptr[l], cnt = lptr[l + 1] - p0;
+                if (have[d]) bj_slot_update<T, UPPER>(ring[d], v, ja, lu);
+                for (int tt = t + 64; tt < cnt; tt += 64) {  // levels wider than a wavefront
+                    BjSlot<T> w;
+                    bj_slot_load<T, UPPER>(w, p0 + tt, rows, col, val, k0, kn, dinv);
+                    bj_slot_update<T, UPPER>(w, v, ja, lu);
+                }
+                const int ln = l + kBjAhead;  // refill this ring slot
+                have[d] = ln < l1 && t < lptr[ln + 1] - lptr[ln];
+                if (have[d]) bj_slot_load<T, UPPER>(ring[d], lptr[ln] + t, rows, col, val, k0, kn, dinv);
+                // one wavefront: the LDS executes its accesses in order, so only
+                // the compiler must keep them in program order
+                wave_lds_order();
             }
-            T s = v[r];
-#pragma unroll
-            for (int u = 0; u < kBjRow; ++u) s -= a[u] * v[c[u]];
-            for (int k = k0 + kBjRow; k < k1; ++k) s -= val[k] * v[col[k]];
-            if (UPPER) s *= val[dg[r]];  // inverted diagonal
-            v[r] = s;
         }
-        wave_lds_order();
     }
 }
 
 // rowid: global row of each local row (nullptr: block b's rows are the
 // contiguous range starting at roff[b]); own: the block's owned local rows,
-// the only ones written back (restricted additive Schwarz).
-// STAGED (workgroup of 4 wavefronts): all four stage the block (factor, row
-// pointers, diagonal offsets, level row lists, right-hand side) in LDS, then
-// the first wavefront walks the levels alone.  Otherwise (one wavefront) only
-// the solution is in LDS and the factor streams from memory a level ahead.
-template <typename T, bool STAGED>
-__global__ void __launch_bounds__(256) bj_solve(const int *__restrict__ roff, const int *__restrict__ rowid,
-                                                const int *__restrict__ own, int max_nnz, int max_rows,
-                                                const int *__restrict__ ia,
-                                                const unsigned short *__restrict__ ja, const int *__restrict__ diag,
-                                                const T *__restrict__ lu, const int *__restrict__ blev_l,
-                                                const int *__restrict__ lptr_l, const unsigned short *__restrict__ rows_l,
-                                                const int *__restrict__ blev_u, const int *__restrict__ lptr_u,
-                                                const unsigned short *__restrict__ rows_u, const double *__restrict__ x,
-                                                double *__restrict__ y) {
+// the only ones written back (restricted additive Schwarz).  One wavefront per
+// block with only the solution in LDS, so that every block of the matrix is
+// resident at once.  (Staging the whole block in LDS was measured slower: it
+// limits the resident blocks, and one wavefront stages too slowly.)
+template <typename T>
+__global__ void __launch_bounds__(64) bj_solve(const int *__restrict__ roff, const int *__restrict__ rowid,
+                                               const int *__restrict__ own, const unsigned short *__restrict__ ja,
+                                               const T *__restrict__ lu, const int *__restrict__ blev_l,
+                                               const int *__restrict__ lptr_l, const unsigned short *__restrict__ rows_l,
+                                               const unsigned short *__restrict__ lcol, const T *__restrict__ lval,
+                                               const int *__restrict__ lk0, const unsigned short *__restrict__ lkn,
+                                               const int *__restrict__ blev_u, const int *__restrict__ lptr_u,
+                                               const unsigned short *__restrict__ rows_u,
+                                               const unsigned short *__restrict__ ucol, const T *__restrict__ uval,
+                                               const int *__restrict__ uk0, const unsigned short *__restrict__ ukn,
+                                               const T *__restrict__ udinv, const double *__restrict__ x,
+                                               double *__restrict__ y) {
     extern __shared__ double smem[];
     const int b = blockIdx.x;
     const int base = roff[b];
     const int cnt = roff[b + 1] - base;
     T *v = (T *)smem;
-    if (STAGED) {
-        T *val = v + max_rows;
-        unsigned short *col = (unsigned short *)(val + max_nnz);
-        unsigned short *rp = col + max_nnz;
-        unsigned short *dg = rp + (max_rows + 1);
-        unsigned short *rl = dg + max_rows;
-        unsigned short *ru = rl + max_rows;
-        const int z0 = ia[base], nzb = ia[base + cnt] - z0;
-        const int pl = lptr_l[blev_l[b]], pu = lptr_u[blev_u[b]];
-#pragma unroll 4
-        for (int k = threadIdx.x; k < nzb; k += blockDim.x) {
-            val[k] = lu[z0 + k];
-            col[k] = ja[z0 + k];
-        }
-#pragma unroll 2
-        for (int r = threadIdx.x; r < cnt; r += blockDim.x) {
-            rp[r] = (unsigned short)(ia[base + r] - z0);
-            dg[r] = (unsigned short)(diag[base + r] - z0);
-            rl[r] = rows_l[pl + r];
-            ru[r] = rows_u[pu + r];
-            v[r] = (T)x[rowid ? rowid[base + r] : base + r];
-        }
-        if (threadIdx.x == 0) rp[cnt] = (unsigned short)nzb;
-        __syncthreads();
-        if (threadIdx.x >= 64) return;  // the level walk is one wavefront's
-        bj_sweep_lds<T, false>(v, val, col, rp, dg, rl, blev_l[b], blev_l[b + 1], lptr_l);
-        bj_sweep_lds<T, true>(v, val, col, rp, dg, ru, blev_u[b], blev_u[b + 1], lptr_u);
-    } else {
-        for (int r = threadIdx.x; r < cnt; r += 64) v[r] = (T)x[rowid ? rowid[base + r] : base + r];
-        wave_lds_order();
-        bj_sweep<T, false>(v, base, blev_l[b], blev_l[b + 1], lptr_l, rows_l, ia, diag, ja, lu);
-        bj_sweep<T, true>(v, base, blev_u[b], blev_u[b + 1], lptr_u, rows_u, ia, diag, ja, lu);
-    }
+    for (int r = threadIdx.x; r < cnt; r += 64) v[r] = (T)x[rowid ? rowid[base + r] : base + r];
+    wave_lds_order();
+    bj_sweep<T, false>(v, blev_l[b], blev_l[b + 1], lptr_l, rows_l, lcol, lval, lk0, lkn, nullptr, ja, lu);
+    bj_sweep<T, true>(v, blev_u[b], blev_u[b + 1], lptr_u, rows_u, ucol, uval, uk0, ukn, udinv, ja, lu);
     const int olo = own[2 * b], ohi = own[2 * b + 1];
     for (int r = olo + threadIdx.x; r < ohi; r += 64) y[rowid ? rowid[base + r] : base + r] = (double)v[r];
-}
-
-// LDS of the staged solve
-static size_t bj_solve_lds(int maxnz, int maxrows, size_t value_bytes) {
-    return (size_t)maxnz * (value_bytes + 2) + (size_t)maxrows * (value_bytes + 2 * 4) + 2 + 8;
 }
 
 static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -929,6 +900,35 @@ static int build_bj(fvp_csr *A) {
         return -1;
     if (ident) A->b_rowid.release();
     A->b_lu32.release();
+    {
+        // slot tables of the solve (static part: columns padded with the row itself, first entry, count)
+        const size_t S = (size_t)total;
+        std::vector<unsigned short> lc(S * kBjRow), uc(S * kBjRow), lkn(S), ukn(S);
+        std::vector<int> lk0(S), uk0(S);
+        for (int b = 0; b < nb; ++b) {
+            const int base = roff[b], cnt = roff[b + 1] - base;
+            const int pl = lptr_l[blev_l[b]], pu = lptr_u[blev_u[b]];
+            for (int q = 0; q < cnt; ++q) {
+                for (int tri = 0; tri < 2; ++tri) {
+                    const size_t sl = (size_t)(tri ? pu : pl) + q;
+                    const int r = tri ? rows_u[sl] : rows_l[sl];
+                    const int i = base + r;
+                    const int k0 = tri ? bdiag[i] + 1 : bia[i], k1 = tri ? bia[i + 1] : bdiag[i];
+                    if (k1 - k0 > 65535) { set_error("block row too long"); return -1; }
+                    (tri ? uk0 : lk0)[sl] = k0;
+                    (tri ? ukn : lkn)[sl] = (unsigned short)(k1 - k0);
+                    for (int u = 0; u < kBjRow; ++u)
+                        (tri ? uc : lc)[sl * kBjRow + u] = (unsigned short)(k0 + u < k1 ? bja[k0 + u] : r);
+                }
+            }
+        }
+        if (A->sl_col.upload(lc.data(), lc.size()) || A->su_col.upload(uc.data(), uc.size()) ||
+            A->sl_k0.upload(lk0.data(), S) || A->su_k0.upload(uk0.data(), S) || A->sl_kn.upload(lkn.data(), S) ||
+            A->su_kn.upload(ukn.data(), S))
+            return -1;
+        A->sl_val.release(); A->su_val.release(); A->su_dinv.release();
+        A->sl_val32.release(); A->su_val32.release(); A->su_dinv32.release();
+    }
     A->b_maxnz = maxnz;
     A->b_maxrows = maxrows;
     A->b_lds = bj_lds(maxnz, sizeof(double), maxrows);
@@ -937,9 +937,8 @@ static int build_bj(fvp_csr *A) {
     // per-kernel attributes (shared by every matrix): allow the device maximum;
     // each build checks its own block against it
     for (const void *f : {(const void *)bj_factor<double, true>, (const void *)bj_factor<double, false>,
-                          (const void *)bj_solve<double, true>, (const void *)bj_solve<double, false>,
-                          (const void *)bj_factor<float, true>, (const void *)bj_factor<float, false>,
-                          (const void *)bj_solve<float, true>, (const void *)bj_solve<float, false>})
+                          (const void *)bj_solve<double>, (const void *)bj_factor<float, true>,
+                          (const void *)bj_factor<float, false>, (const void *)bj_solve<float>})
         (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, A->lds_max);
     A->nblk = nb;
     A->bj = true;
@@ -991,6 +990,23 @@ static int precon_build_impl(fvp_csr *A, int type) {
                     A->b_roff.p, A->b_maxrows, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p,
                     A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p, A->b_lu32.p, A->flag.p);
         }
+        // level-ordered solve layout of the new factor
+        const long S = (long)A->sl_k0.n;
+        if (type == FVP_PRECON_BJILU0) {
+            if ((!A->sl_val.p && (A->sl_val.alloc(S * kBjRow) || A->su_val.alloc(S * kBjRow) ||
+                                  A->su_dinv.alloc(S))))
+                return -1;
+            bj_pack<double><<<nblk(S, 256), 256, 0, s>>>(S, A->sl_k0.p, A->sl_kn.p, A->b_lu.p, A->sl_val.p, nullptr);
+            bj_pack<double><<<nblk(S, 256), 256, 0, s>>>(S, A->su_k0.p, A->su_kn.p, A->b_lu.p, A->su_val.p,
+                                                         A->su_dinv.p);
+        } else {
+            if ((!A->sl_val32.p && (A->sl_val32.alloc(S * kBjRow) || A->su_val32.alloc(S * kBjRow) ||
+                                    A->su_dinv32.alloc(S))))
+                return -1;
+            bj_pack<float><<<nblk(S, 256), 256, 0, s>>>(S, A->sl_k0.p, A->sl_kn.p, A->b_lu32.p, A->sl_val32.p, nullptr);
+            bj_pack<float><<<nblk(S, 256), 256, 0, s>>>(S, A->su_k0.p, A->su_kn.p, A->b_lu32.p, A->su_val32.p,
+                                                        A->su_dinv32.p);
+        }
     } else if (type == FVP_PRECON_JACOBI) {
         if (A->jac.n != (size_t)n && A->jac.alloc(n)) return -1;
         jacobi_build<<<nblk(n, 256), 256, 0, s>>>(n, A->ia.p, A->ja.p, A->a.p, A->jac.p, A->flag.p);
@@ -1034,25 +1050,17 @@ static int precon_apply_impl(fvp_csr *A, const double *x, double *y) {
         vmul<<<nblk(n, 256), 256, 0, s>>>(n, x, A->jac.p, y);
         break;
     case FVP_PRECON_BJILU0:
-    case FVP_PRECON_BJILU0_F32: {
-        const bool f64 = A->ptype == FVP_PRECON_BJILU0;
-        const size_t vb = f64 ? sizeof(double) : sizeof(float);
-        const size_t staged = bj_solve_lds(A->b_maxnz, A->b_maxrows, vb);
-        const bool stage = A->bj_stage && staged <= (size_t)A->lds_max;
-#define FVP_BJ_SOLVE(T, S, LU)                                                                                    \
-    bj_solve<T, S><<<A->nblk, S ? 256 : 64, S ? staged : vb * A->b_maxrows, s>>>(                                 \
-        A->b_roff.p, A->b_rowid.p, A->b_own.p, A->b_maxnz, A->b_maxrows, A->b_ia.p, A->b_ja.p, A->b_diag.p, LU,   \
-        A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p, A->b_lev_u.p, A->b_lptr_u.p, A->b_rows_u.p, x, y)
-        if (f64) {
-            if (stage) FVP_BJ_SOLVE(double, true, A->b_lu.p);
-            else FVP_BJ_SOLVE(double, false, A->b_lu.p);
-        } else {
-            if (stage) FVP_BJ_SOLVE(float, true, A->b_lu32.p);
-            else FVP_BJ_SOLVE(float, false, A->b_lu32.p);
-        }
-#undef FVP_BJ_SOLVE
+        bj_solve<double><<<A->nblk, 64, sizeof(double) * A->b_maxrows, s>>>(
+            A->b_roff.p, A->b_rowid.p, A->b_own.p, A->b_ja.p, A->b_lu.p, A->b_lev_l.p, A->b_lptr_l.p,
+            A->b_rows_l.p, A->sl_col.p, A->sl_val.p, A->sl_k0.p, A->sl_kn.p, A->b_lev_u.p, A->b_lptr_u.p,
+            A->b_rows_u.p, A->su_col.p, A->su_val.p, A->su_k0.p, A->su_kn.p, A->su_dinv.p, x, y);
         break;
-    }
+    case FVP_PRECON_BJILU0_F32:
+        bj_solve<float><<<A->nblk, 64, sizeof(float) * A->b_maxrows, s>>>(
+            A->b_roff.p, A->b_rowid.p, A->b_own.p, A->b_ja.p, A->b_lu32.p, A->b_lev_l.p, A->b_lptr_l.p,
+            A->b_rows_l.p, A->sl_col.p, A->sl_val32.p, A->sl_k0.p, A->sl_kn.p, A->b_lev_u.p, A->b_lptr_u.p,
+            A->b_rows_u.p, A->su_col.p, A->su_val32.p, A->su_k0.p, A->su_kn.p, A->su_dinv32.p, x, y);
+        break;
     case FVP_PRECON_ILU0: {
         const int L = 8;
         for (size_t l = 0; l + 1 < A->lptr_h.size(); ++l) {
@@ -1119,6 +1127,9 @@ extern "C" int fvp_csr_destroy(fvp_csr *A) {
     for (hipEvent_t e : A->ev) (void)hipEventDestroy(e);
     A->b_ia.release(); A->b_src.release(); A->b_diag.release(); A->b_ja.release(); A->b_lu.release(); A->b_lu32.release();
     A->b_roff.release(); A->b_rowid.release(); A->b_own.release();
+    A->sl_col.release(); A->su_col.release(); A->sl_val.release(); A->su_val.release(); A->su_dinv.release();
+    A->sl_val32.release(); A->su_val32.release(); A->su_dinv32.release(); A->sl_k0.release(); A->su_k0.release();
+    A->sl_kn.release(); A->su_kn.release();
     A->b_lev_l.release(); A->b_lev_u.release(); A->b_lptr_l.release(); A->b_lptr_u.release();
     A->b_rows_l.release(); A->b_rows_u.release();
     if (A->own_stream) (void)hipStreamDestroy(A->own_stream);
