@@ -51,6 +51,13 @@ struct fvp_integrator {
     int nseg = 0, seg_min = 0;
     DevBuf<int> seg;
     DevBuf<double> seg_sum;
+    // learned initial iterate (network, its features / output, the candidate
+    // state) and training-pair recording into caller buffers
+    fvp_mpnn *mp = nullptr;
+    DevBuf<float> stat, feat, ylearn;  // static feature columns, features, network output
+    DevBuf<double> y2, ypn2, G2;
+    float *rec_feat = nullptr, *rec_targ = nullptr;
+    double learn_trust = 2., learn_band = 0.5;
 };
 
 namespace fvp {
@@ -102,6 +109,60 @@ __global__ void zero_rows(int nd, const int *__restrict__ rows, double *__restri
 }
 
 static inline int nb(long n) { return (int)((n + 255) / 256); }
+
+// node features of the learned iterate: columns 0..4 from the step, 5..15 static.
+// Column 4 is the Newton correction of the capacity-diagonal approximation of
+// the iteration matrix (dF/dpsi' = -ahh dominates for small h), in error-weight
+// units: ewt G / (c_j ahh), 0 where ahh vanishes.
+__global__ void learn_features(long n, const double *__restrict__ yn, const double *__restrict__ ypred,
+                               const double *__restrict__ ewt, const double *__restrict__ G,
+                               const double *__restrict__ ahh, double cj, float lh, float kk,
+                               const float *__restrict__ stat, float *__restrict__ x) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float *xi = x + i * 16;
+    xi[0] = (float)yn[i];
+    xi[1] = (float)asinh((ypred[i] - yn[i]) * ewt[i]);
+    xi[2] = lh;
+    xi[3] = kk;
+    const double d = cj * ahh[i];
+    xi[4] = d > 0. ? (float)asinh(ewt[i] * G[i] / d) : 0.f;
+    for (int c = 5; c < 16; ++c) xi[c] = stat ? stat[i * 16 + c] : 0.f;
+}
+
+// learned candidate y2 = ypred + c / ewt from the network output dy (error-weight
+// units), with c limited by the capacity-diagonal Newton step
+// jac = ewt G / (c_j ahh) of the node (0 where ahh = 0):
+//   |c| <= trust |jac| and c of the sign of jac -- a node whose equation the
+//   predictor already satisfies is not moved;
+//   c = 0 where ypred > -band (the near-saturated band, where the van
+//   Genuchten-Mualem conductivity's derivative is unbounded and a stray
+//   correction dominates the next residual and the linear solve)
+__global__ void learn_guess(long n, const double *__restrict__ ypred, const float *__restrict__ dy,
+                            const double *__restrict__ ewt, const double *__restrict__ G,
+                            const double *__restrict__ ahh, double cj, double trust, double band,
+                            double *__restrict__ y2) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double d = cj * ahh[i];
+    const double jac = d > 0. ? ewt[i] * G[i] / d : 0.;
+    const double lim = trust * std::fabs(jac);
+    double c = std::fmin(std::fmax((double)dy[i], -lim), lim);
+    if (c * jac < 0. || ypred[i] > -band) c = 0.;
+    y2[i] = ypred[i] + c / ewt[i];
+}
+
+// training target ewt * (y - ypred)
+__global__ void learn_target(long n, const double *__restrict__ y, const double *__restrict__ ypred,
+                             const double *__restrict__ ewt, float *__restrict__ t) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) t[i] = (float)((y[i] - ypred[i]) * ewt[i]);
+}
+
+__global__ void zero_rows_f(int nd, const int *__restrict__ rows, float *__restrict__ v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nd) v[rows[i]] = 0.f;
+}
 
 // sum over each instance's nodes of (v ewt)^2: one workgroup per instance
 __global__ void __launch_bounds__(256) seg_sq(const int *__restrict__ seg, const double *__restrict__ v,
@@ -231,6 +292,23 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
     };
     deriv();
     if (fvp::residual_impl(c, t, I->y.p, I->ypn.p, I->G.p, c->averaging)) return -1;
+    if (I->mp || I->rec_feat) {
+        float *x = I->rec_feat ? I->rec_feat : I->feat.p;
+        learn_features<<<nb(N), 256, 0, s>>>(N, H(I, 0), I->ypred.p, I->ewt.p, I->G.p, c->cap.p, cj,
+                                             (float)std::log10(h), k / 3.f, I->stat.p, x);
+        FVP_HIP(hipGetLastError());
+        if (I->mp) {
+            // learned candidate and its first Newton residual
+            if (fvp_mpnn_forward(I->mp, x, I->ylearn.p)) return -1;
+            if (c->ndir) zero_rows_f<<<(c->ndir + 255) / 256, 256, 0, s>>>(c->ndir, c->dnode.p, I->ylearn.p);
+            learn_guess<<<nb(N), 256, 0, s>>>(N, I->ypred.p, I->ylearn.p, I->ewt.p, I->G.p, c->cap.p, cj,
+                                              I->learn_trust, I->learn_band, I->y2.p);
+            double c2[2] = {cj, 1.};
+            const double *x2[2] = {I->y2.p, I->beta.p};
+            lincomb(I, 2, c2, x2, I->ypn2.p);
+            if (fvp::residual_impl(c, t, I->y2.p, I->ypn2.p, I->G2.p, c->averaging)) return -1;
+        }
+    }
     // preconditioner setup when stale (IDA: first step, after a failure, c_j moved by > 25%)
     double cjratio = I->cj_setup > 0 ? cj / I->cj_setup : 0.;
     if (I->p.precon_type != FVP_PRECON_NONE &&
@@ -244,6 +322,28 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
         I->need_setup = false;
         I->steps_since_setup = 0;
         I->ss = 20.;
+    }
+    if (I->mp) {
+        // judge the learned candidate against the predictor by the size of the
+        // Newton step each would start with, the preconditioned residual
+        // P G (P ~ J^-1) in the error-weighted RMS norm; without a
+        // preconditioner by the weighted residual itself
+        double g0, g1;
+        if (I->p.precon_type != FVP_PRECON_NONE) {
+            if (fvp_precon_solve(c, I->G.p, I->b.p) || fvp_precon_solve(c, I->G2.p, I->delta.p)) return -1;
+            g0 = wrms(I, I->b.p);
+            g1 = wrms(I, I->delta.p);
+        } else {
+            g0 = wrms(I, I->G.p);
+            g1 = wrms(I, I->G2.p);
+        }
+        ++I->st.learned_tried;
+        if (g1 < g0) {
+            ++I->st.learned_used;
+            std::swap(I->y, I->y2);
+            std::swap(I->ypn, I->ypn2);
+            std::swap(I->G, I->G2);
+        }
     }
     fvp_csr *W = fvp::jacobian_handle(c);
     if (!W) return -1;
@@ -306,6 +406,7 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
     }
     if (!conv) return 2;
     deriv();
+    if (I->rec_targ) learn_target<<<nb(N), 256, 0, s>>>(N, I->y.p, I->ypred.p, I->ewt.p, I->rec_targ);
     // local error estimate from the predictor-corrector difference
     {
         double c2[2] = {1., -1.};
@@ -448,6 +549,8 @@ extern "C" int fvp_integrator_destroy(fvp_integrator *I) {
     I->G.release(); I->b.release(); I->delta.release(); I->beta.release(); I->tmp.release(); I->part.release();
     I->scal.release();
     I->seg.release(); I->seg_sum.release();
+    I->stat.release(); I->feat.release(); I->ylearn.release(); I->y2.release(); I->ypn2.release();
+    I->G2.release();
     delete I;
     return 0;
 }
@@ -500,6 +603,51 @@ extern "C" int fvp_integrator_set_segments(fvp_integrator *I, int nseg, const in
     if (I->seg.upload(offsets, nseg + 1) || I->seg_sum.alloc(nseg)) return -1;
     I->nseg = nseg;
     I->seg_min = mn;
+    return 0;
+}
+
+// static feature columns (device N x 16) into the integrator's copy; null = zeros
+static int set_static(fvp_integrator *I, const float *static_features) {
+    const long N = I->N;
+    if (!I->stat.p && I->stat.alloc(N * 16)) return -1;
+    hipStream_t s = I->c->stream;
+    if (static_features)
+        FVP_HIP(hipMemcpyAsync(I->stat.p, static_features, sizeof(float) * N * 16, hipMemcpyDeviceToDevice, s));
+    else
+        FVP_HIP(hipMemsetAsync(I->stat.p, 0, sizeof(float) * N * 16, s));
+    return 0;
+}
+
+extern "C" int fvp_integrator_set_learned(fvp_integrator *I, fvp_mpnn *m, const float *static_features, double trust,
+                                          double band) {
+    if (!I) { set_error("null integrator"); return -1; }
+    if (m && (!(trust >= 0.) || !(band >= 0.))) { set_error("trust and band must be >= 0"); return -1; }
+    FVP_HIP(hipSetDevice(I->c->device));
+    if (!m) {
+        I->mp = nullptr;
+        I->feat.release(); I->ylearn.release(); I->y2.release(); I->ypn2.release(); I->G2.release();
+        return 0;
+    }
+    if (fvp::mpnn_nodes(m) != I->N) { set_error("network node count differs from the problem's"); return -1; }
+    const long N = I->N;
+    if (set_static(I, static_features) || (!I->feat.p && I->feat.alloc(N * 16)) || I->ylearn.alloc(N) ||
+        I->y2.alloc(N) || I->ypn2.alloc(N) || I->G2.alloc(N))
+        return -1;
+    if (fvp_mpnn_set_stream(m, (void *)I->c->stream)) return -1;
+    I->mp = m;
+    I->learn_trust = trust;
+    I->learn_band = band;
+    return 0;
+}
+
+extern "C" int fvp_integrator_set_recording(fvp_integrator *I, const float *static_features, float *features,
+                                            float *targets) {
+    if (!I) { set_error("null integrator"); return -1; }
+    if ((features == nullptr) != (targets == nullptr)) { set_error("give both recording buffers or neither"); return -1; }
+    FVP_HIP(hipSetDevice(I->c->device));
+    if (features && set_static(I, static_features)) return -1;
+    I->rec_feat = features;
+    I->rec_targ = targets;
     return 0;
 }
 

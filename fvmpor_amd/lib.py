@@ -79,6 +79,22 @@ SIGNATURES = {
     "fvp_integrator_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "fvp_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fvp_profile_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_dbl_p, ctypes.POINTER(ctypes.c_long)]),
+    "fvp_mpnn_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, ctypes.c_int,
+                                       ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_void_p)]),
+    "fvp_mpnn_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "fvp_mpnn_num_params": (ctypes.c_long, [ctypes.c_void_p]),
+    "fvp_mpnn_set_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "fvp_mpnn_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "fvp_mpnn_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "fvp_mpnn_get_grads": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "fvp_mpnn_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fvp_mpnn_loss_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _c_dbl_p]),
+    "fvp_mpnn_adam": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
+    "fvp_mpnn_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "fvp_integrator_set_learned": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                  ctypes.c_double, ctypes.c_double]),
+    "fvp_integrator_set_recording": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                    ctypes.c_void_p]),
 }
 
 PRECON_NONE, PRECON_JACOBI, PRECON_ILU0, PRECON_BJILU0, PRECON_BJILU0_F32 = 0, 2, 5, 7, 8
@@ -378,7 +394,7 @@ class IntegratorStats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_long), ("error_fails", ctypes.c_long), ("newton_fails", ctypes.c_long),
                 ("residuals", ctypes.c_long), ("newton_iters", ctypes.c_long), ("gmres_iters", ctypes.c_long),
                 ("precon_setups", ctypes.c_long), ("t", ctypes.c_double), ("h_last", ctypes.c_double),
-                ("order_last", ctypes.c_int)]
+                ("order_last", ctypes.c_int), ("learned_tried", ctypes.c_long), ("learned_used", ctypes.c_long)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -411,6 +427,24 @@ class Integrator:
         _check(lib().fvp_integrator_set_segments(self.h, len(off) - 1, off.ctypes.data_as(_c_int_p)),
                "fvp_integrator_set_segments")
 
+    def set_learned(self, model, static_features=None, trust=2.0, band=0.5):
+        """Newton initial iterate from a Mpnn (None switches it off); static
+        feature columns 5..15 from a (N, 16) float32 device tensor."""
+        self._learned = (model, static_features)
+        _check(lib().fvp_integrator_set_learned(self.h, model.h if model is not None else None,
+                                                _ptr(static_features) if static_features is not None else None,
+                                                float(trust), float(band)),
+               "fvp_integrator_set_learned")
+
+    def set_recording(self, features=None, targets=None, static_features=None):
+        """Write each accepted step's (features, target) pair into device
+        tensors (N, 16) and (N,) float32; None switches recording off."""
+        self._recording = (features, targets, static_features)
+        _check(lib().fvp_integrator_set_recording(
+            self.h, _ptr(static_features) if static_features is not None else None,
+            _ptr(features) if features is not None else None, _ptr(targets) if targets is not None else None),
+            "fvp_integrator_set_recording")
+
     def step(self, t_stop) -> int:
         return _check(lib().fvp_integrator_step(self.h, float(t_stop)), "fvp_integrator_step")
 
@@ -430,3 +464,97 @@ class Integrator:
         st = IntegratorStats()
         _check(lib().fvp_integrator_get_stats(self.h, ctypes.byref(st)), "fvp_integrator_get_stats")
         return st.as_dict()
+
+
+MPNN_FEATURES = 16
+MPNN_HIDDEN = 64
+
+
+class Mpnn:
+    """Message-passing network on a node graph (fvp_mpnn_*): BF16 matrix-core
+    forward, backward and Adam on the device.  ``ia``/``ja``: CSR adjacency
+    without the diagonal (symmetric)."""
+
+    def __init__(self, ia, ja, layers=3, seed=0, device=0):
+        self._ia = np.ascontiguousarray(ia, dtype=np.int32)
+        self._ja = np.ascontiguousarray(ja, dtype=np.int32)
+        self.N = len(self._ia) - 1
+        self.layers = int(layers)
+        h = ctypes.c_void_p()
+        _check(lib().fvp_mpnn_create(int(device), self.N, self._ia.ctypes.data_as(_c_int_p),
+                                     self._ja.ctypes.data_as(_c_int_p), self.layers, int(seed), ctypes.byref(h)),
+               "fvp_mpnn_create")
+        self.h = h
+        self.n_params = int(lib().fvp_mpnn_num_params(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fvp_mpnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_stream(self, stream_ptr: int):
+        _check(lib().fvp_mpnn_set_stream(self.h, ctypes.c_void_p(stream_ptr)), "fvp_mpnn_set_stream")
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.n_params, dtype=np.float32)
+        _check(lib().fvp_mpnn_get_params(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+               "fvp_mpnn_get_params")
+        return out
+
+    def set_params(self, flat):
+        flat = np.ascontiguousarray(flat, dtype=np.float32)
+        if flat.shape != (self.n_params,):
+            raise ValueError(f"expected {self.n_params} parameters, got {flat.shape}")
+        _check(lib().fvp_mpnn_set_params(self.h, flat.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+               "fvp_mpnn_set_params")
+
+    def get_grads(self) -> np.ndarray:
+        out = np.empty(self.n_params, dtype=np.float32)
+        _check(lib().fvp_mpnn_get_grads(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+               "fvp_mpnn_get_grads")
+        return out
+
+    def forward(self, x, y):
+        """x: (N, 16) float32 device tensor, y: (N,) float32 device tensor."""
+        _check(lib().fvp_mpnn_forward(self.h, _ptr(x), _ptr(y)), "fvp_mpnn_forward")
+
+    def loss_grad(self, x, t) -> float:
+        loss = ctypes.c_double()
+        _check(lib().fvp_mpnn_loss_grad(self.h, _ptr(x), _ptr(t), ctypes.byref(loss)), "fvp_mpnn_loss_grad")
+        return loss.value
+
+    def adam(self, lr: float):
+        _check(lib().fvp_mpnn_adam(self.h, float(lr)), "fvp_mpnn_adam")
+
+    def synchronize(self):
+        _check(lib().fvp_mpnn_synchronize(self.h), "fvp_mpnn_synchronize")
+
+
+def mpnn_pack(p: dict, layers: int) -> np.ndarray:
+    """Flat parameter array (fvp_mpnn_set_params order) from named arrays
+    We (16 x 64), be, W1_l, W2_l, b_l, Wd (64 x 1), bd."""
+    parts = [p["We"], p["be"]]
+    for l in range(layers):
+        parts += [p[f"W1_{l}"], p[f"W2_{l}"], p[f"b_{l}"]]
+    parts += [p["Wd"], p["bd"]]
+    return np.concatenate([np.asarray(a, dtype=np.float32).ravel() for a in parts])
+
+
+def mpnn_unpack(flat, layers: int) -> dict:
+    """Inverse of mpnn_pack."""
+    F, H = MPNN_FEATURES, MPNN_HIDDEN
+    shapes = [("We", (F, H)), ("be", (H,))]
+    for l in range(layers):
+        shapes += [(f"W1_{l}", (H, H)), (f"W2_{l}", (H, H)), (f"b_{l}", (H,))]
+    shapes += [("Wd", (H, 1)), ("bd", (1,))]
+    out, o = {}, 0
+    for name, shp in shapes:
+        n = int(np.prod(shp))
+        out[name] = np.asarray(flat[o:o + n], dtype=np.float64).reshape(shp)
+        o += n
+    if o != len(flat):
+        raise ValueError("parameter count mismatch")
+    return out

@@ -226,6 +226,8 @@ typedef struct {
     long newton_iters, gmres_iters, precon_setups;
     double t, h_last;
     int order_last;
+    long learned_tried;   /* steps that evaluated a learned initial iterate */
+    long learned_used;    /* ... whose first Newton step estimate beat the predictor's */
 } fvp_integrator_stats;
 
 /* psi0: device, n_nodes.  Consistent psi'(t0) = F/ahh on non-Dirichlet nodes with
@@ -276,6 +278,63 @@ int fvp_autotune(fvp_ctx *ctx, double t, const double *psi, const double *psip, 
  * element kernel, 1 = whole residual, 2 = preconditioner solve. */
 int fvp_profile(fvp_ctx *ctx, int enable);
 int fvp_profile_get(fvp_ctx *ctx, int kind, double *total_ms, long *count);
+
+/* ---------------------------------------------------------------------------
+ * Learned initial iterate: a message-passing network on the node-adjacency
+ * graph of the mesh (CSR ia/ja without the diagonal, host arrays), 16 input
+ * features per node, `layers` residual layers of width 64, one output per
+ * node.  BF16 weights and activations with FP32 accumulation on the matrix
+ * cores; training by the mean squared error against per-node targets and Adam
+ * (beta 0.9 / 0.999, eps 1e-8) on FP32 master weights.  Parameters are one flat
+ * FP32 array: We (16 x 64, row major), be (64), per layer W1 (64 x 64), W2
+ * (64 x 64), b (64), then Wd (64), bd (1).  x (N x 16), y and t (N) are FP32
+ * device arrays.  All work runs on the model's stream (its own, or the one
+ * given to fvp_mpnn_set_stream).  No counterpart in the reference: the
+ * reference starts Newton from the BDF predictor only (fvmpor_ODE.cpp:120-200). */
+typedef struct fvp_mpnn fvp_mpnn;
+int fvp_mpnn_create(int device, int n_nodes, const int *ia, const int *ja, int layers, unsigned long long seed,
+                    fvp_mpnn **out);
+int fvp_mpnn_destroy(fvp_mpnn *m);
+long fvp_mpnn_num_params(const fvp_mpnn *m);
+int fvp_mpnn_set_stream(fvp_mpnn *m, void *stream);
+int fvp_mpnn_get_params(fvp_mpnn *m, float *host);
+int fvp_mpnn_set_params(fvp_mpnn *m, const float *host);
+int fvp_mpnn_get_grads(fvp_mpnn *m, float *host);
+int fvp_mpnn_forward(fvp_mpnn *m, const float *x, float *y);
+int fvp_mpnn_loss_grad(fvp_mpnn *m, const float *x, const float *t, double *loss);
+int fvp_mpnn_adam(fvp_mpnn *m, float lr);
+int fvp_mpnn_synchronize(fvp_mpnn *m);
+
+/* Learned Newton initial iterate for the integrator.  Every step attempt
+ * builds the node features x (N x 16, FP32) as
+ *   x[:, 0] = psi_n, x[:, 1] = asinh(ewt * (psi_pred - psi_n)),
+ *   x[:, 2] = log10(h), x[:, 3] = BDF order / 3,
+ *   x[:, 4] = asinh(jac), jac = ewt * G / (c_j ahh): the Newton step of the
+ *             iteration matrix's capacity diagonal (G the residual at the
+ *             predictor, ahh the capacity; 0 where ahh = 0),
+ *   x[:, 5:16] = the caller's static columns (static_features, device N x 16,
+ *                its columns 0..4 ignored; null = zeros),
+ * runs the network to y (the Newton correction in error-weight units) and
+ * proposes psi_pred + c / ewt with c = y limited to |c| <= trust |jac|, of the
+ * sign of jac, and 0 where psi_pred > -band (the near-saturated band, where
+ * the van Genuchten-Mualem conductivity has an unbounded derivative) and on
+ * Dirichlet rows.  Judge and fallback: the residual is evaluated at the
+ * proposal and Newton starts from whichever of predictor and proposal has the
+ * smaller first Newton step estimate, the preconditioned residual P G in the
+ * error-weighted RMS norm (the weighted residual without a preconditioner);
+ * the local error estimate still measures the corrector against the
+ * predictor.  m = null switches it off.  The network must be built on this
+ * context's node count; it runs on the context's stream from here on.
+ * Defaults of the bindings: trust 2, band 0.5 (head units). */
+int fvp_integrator_set_learned(fvp_integrator *I, fvp_mpnn *m, const float *static_features, double trust,
+                               double band);
+/* Training-pair recording: on every step attempt the integrator writes the
+ * features above (static columns from static_features, null = zeros) into
+ * features (device N x 16) and, once Newton has converged, the target
+ * ewt * (psi_corrected - psi_pred) into targets (device N, FP32); after
+ * fvp_integrator_step returns they hold the accepted step's pair.  Null
+ * features / targets switch recording off. */
+int fvp_integrator_set_recording(fvp_integrator *I, const float *static_features, float *features, float *targets);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,71 @@
+"""The learned Newton initial iterate in the integrator: training-pair
+recording, the fallback (a network that proposes nothing never moves the
+Newton start), and what a network trained on fresh-seed boxes gains on a
+held-out box against the plain predictor."""
+import numpy as np
+import pytest
+
+from conftest import device
+
+pytestmark = pytest.mark.gpu
+
+
+def _box(nz, seed, sigma=1.0, rain=0.02):
+    from fvmpor_amd.problems import box3d_problem
+    return box3d_problem(nx=4 * nz, ny=4 * nz, nz=nz, seed_mesh=seed, seed_field=seed + 1, seed_layers=seed + 2,
+                         sigma_lnK=sigma, rain=rain)
+
+
+def test_recording_writes_the_step_features_and_targets(slice2d):
+    from fvmpor_amd.learn import record_pairs, static_features
+    device()
+    pairs, st = record_pairs(slice2d, 0.05)
+    assert len(pairs) == st["steps"] > 3
+    stat = static_features(slice2d)
+    for x, t in pairs:
+        x, t = x.cpu().numpy(), t.cpu().numpy()
+        assert np.array_equal(x[:, 5:], stat[:, 5:])
+        assert np.all(np.isfinite(x)) and np.all(np.isfinite(t))
+        assert np.all(x[:, 2] == x[0, 2]) and np.all(x[:, 3] == x[0, 3])
+        assert np.isclose(x[0, 3] * 3, round(x[0, 3] * 3)) and 1 <= round(x[0, 3] * 3) <= 3
+    # the last pair belongs to the last accepted step
+    assert np.isclose(10 ** float(pairs[-1][0][0, 2]), st["h_last"], rtol=1e-5)
+    # converged corrections are small against the error weights (the error test bounds them)
+    assert max(float(np.abs(t.cpu().numpy()).max()) for _, t in pairs[2:]) < 1e3
+
+
+def test_a_network_that_proposes_nothing_keeps_the_plain_start(box3d_small):
+    from fvmpor_amd.learn import run
+    from fvmpor_amd.lib import MPNN_HIDDEN
+    device()
+    n_params = 16 * MPNN_HIDDEN + MPNN_HIDDEN + 3 * (2 * MPNN_HIDDEN * MPNN_HIDDEN + MPNN_HIDDEN) + MPNN_HIDDEN + 1
+    hp, sp, _ = run(box3d_small, 0.05)
+    hz, sz, _ = run(box3d_small, 0.05, model_params=np.zeros(n_params, dtype=np.float32))
+    assert sz["learned_tried"] >= sz["steps"] and sz["learned_used"] == 0
+    for k in ("steps", "newton_iters", "error_fails"):
+        assert sz[k] == sp[k], k
+    assert np.abs(hz - hp).max() <= 1e-10 * np.abs(hp).max()
+
+
+@pytest.mark.timeout(600)
+def test_trained_network_gain_on_a_held_out_box():
+    from fvmpor_amd.learn import run, train_on
+    device()
+    train = [_box(10, 100 + 10 * i, sigma=s, rain=r) for i, (s, r) in enumerate([(0.8, 0.015), (1.0, 0.02),
+                                                                                (1.3, 0.03)])]
+    params, hist, npairs, _ = train_on(train, 0.25, epochs=30, lr=1e-3)
+    test = _box(10, 7777, sigma=1.1, rain=0.025)
+    hp, sp, wp = run(test, 0.25)
+    hl, sl, wl = run(test, 0.25, model_params=params)
+    rel = np.abs(hl - hp).max() / np.abs(hp).max()
+    print(f"trained on {npairs} pairs: loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    print(f"plain:   steps {sp['steps']}, newton {sp['newton_iters']}, gmres {sp['gmres_iters']}, {wp:.2f} s")
+    print(f"learned: steps {sl['steps']}, newton {sl['newton_iters']}, gmres {sl['gmres_iters']}, {wl:.2f} s, "
+          f"start taken from the network in {sl['learned_used']}/{sl['learned_tried']} attempts")
+    print(f"final heads: max rel diff {rel:.2e}")
+    assert hist[-1] < 0.1 * hist[0]
+    # the judge accepts the network's start in most steps, the linear work does not grow,
+    # and both runs end within the integration tolerance of each other
+    assert sl["learned_used"] >= 0.5 * sl["learned_tried"]
+    assert sl["gmres_iters"] <= 1.05 * sp["gmres_iters"]
+    assert rel <= 1e-5
