@@ -118,16 +118,22 @@ __global__ void learn_features(long n, const double *__restrict__ yn, const doub
                                const double *__restrict__ ewt, const double *__restrict__ G,
                                const double *__restrict__ ahh, double cj, float lh, float kk,
                                const float *__restrict__ stat, float *__restrict__ x) {
-    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    // four lanes per node, one 16-byte column group each
+    const long g = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long i = g >> 2;
+    const int q = (int)(g & 3);
     if (i >= n) return;
-    float *xi = x + i * 16;
-    xi[0] = (float)yn[i];
-    xi[1] = (float)asinh((ypred[i] - yn[i]) * ewt[i]);
-    xi[2] = lh;
-    xi[3] = kk;
-    const double d = cj * ahh[i];
-    xi[4] = d > 0. ? (float)asinh(ewt[i] * G[i] / d) : 0.f;
-    for (int c = 5; c < 16; ++c) xi[c] = stat ? stat[i * 16 + c] : 0.f;
+    float4 o = stat ? reinterpret_cast<const float4 *>(stat)[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (q == 0) {
+        o.x = (float)yn[i];
+        o.y = (float)asinh((ypred[i] - yn[i]) * ewt[i]);
+        o.z = lh;
+        o.w = kk;
+    } else if (q == 1) {
+        const double d = cj * ahh[i];
+        o.x = d > 0. ? (float)asinh(ewt[i] * G[i] / d) : 0.f;
+    }
+    reinterpret_cast<float4 *>(x)[g] = o;
 }
 
 // learned candidate y2 = ypred + c / ewt from the network output dy (error-weight
@@ -294,7 +300,7 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
     if (fvp::residual_impl(c, t, I->y.p, I->ypn.p, I->G.p, c->averaging)) return -1;
     if (I->mp || I->rec_feat) {
         float *x = I->rec_feat ? I->rec_feat : I->feat.p;
-        learn_features<<<nb(N), 256, 0, s>>>(N, H(I, 0), I->ypred.p, I->ewt.p, I->G.p, c->cap.p, cj,
+        learn_features<<<nb(4 * N), 256, 0, s>>>(N, H(I, 0), I->ypred.p, I->ewt.p, I->G.p, c->cap.p, cj,
                                              (float)std::log10(h), k / 3.f, I->stat.p, x);
         FVP_HIP(hipGetLastError());
         if (I->mp) {

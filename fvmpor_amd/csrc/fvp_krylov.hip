@@ -474,7 +474,7 @@ __global__ void __launch_bounds__(kBjFactorThreads) bj_factor(
 // path.  The entries of a row are subtracted in column order, as the
 // sequential solve does.  T is the factor's arithmetic type (double, or float
 // for the FP32 preconditioner); x and y stay FP64.
-constexpr int kBjRow = 8;
+constexpr int kBjRow = 12;
 
 // After a factorisation: copy the first kBjRow entries of every slot's row part
 // (zero beyond the row's count) and, for U, the row's inverted diagonal into

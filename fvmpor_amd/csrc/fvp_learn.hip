@@ -169,13 +169,15 @@ __global__ void __launch_bounds__(256) mp_wgrad(int N, int rows_per_block, const
         part[(size_t)blockIdx.x * K * kH + idx] = red[0][idx] + red[1][idx] + red[2][idx] + red[3][idx];
 }
 
-// out[i] = sum over blocks of part[b * n + i]  (+ into out when accumulate)
-__global__ void sum_parts(int nparts, int n, const float *__restrict__ part, float *__restrict__ out) {
+// out[i] += sum over the blocks b of chunk blockIdx.y of part[b * n + i]
+// (out zeroed beforehand; one FP32 atomic per output and chunk)
+__global__ void sum_parts(int nparts, int n, int per, const float *__restrict__ part, float *__restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int b0 = blockIdx.y * per, b1 = min(nparts, b0 + per);
     float s = 0.f;
-    for (int b = 0; b < nparts; ++b) s += part[(size_t)b * n + i];
-    out[i] = s;
+    for (int b = b0; b < b1; ++b) s += part[(size_t)b * n + i];
+    atomicAdd(&out[i], s);
 }
 
 // column sums of D (N x 64, BF16) into block partials part[b * 64 + n]
@@ -192,40 +194,65 @@ __global__ void __launch_bounds__(256) mp_colsum(int N, int rows_per_block, cons
 }
 
 // neighbour mean: out_i = bf16((1/deg_i) sum_j H_j); transposed (scale_src):
-// outf_i (+)= sum_j G_j / deg_j (FP32 out).  16 threads per node, 4 features each.
+// outf_i (+)= sum_j G_j / deg_j (FP32 out).  8 lanes per node, 8 features
+// (one 16-byte load per neighbour) each.
 template <bool TRANSPOSE>
 __global__ void __launch_bounds__(256) mp_aggregate(int N, const int *__restrict__ ia, const int *__restrict__ ja,
                                                     const bf16_t *__restrict__ H, bf16_t *__restrict__ outb,
                                                     float *__restrict__ outf) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = gid >> 4, f = (gid & 15) * 4;
+    const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int i = (int)(gid >> 3), f = (int)(gid & 7) * 8;
     if (i >= N) return;
-    float s[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int k = ia[i]; k < ia[i + 1]; ++k) {
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const int k1 = ia[i + 1];
+    for (int k = ia[i]; k < k1; ++k) {
         const int j = ja[k];
-        const bf16_t *h = H + (size_t)j * kH + f;
+        const uint4 hv = *reinterpret_cast<const uint4 *>(H + (size_t)j * kH + f);
         const float w = TRANSPOSE ? 1.f / (float)max(1, ia[j + 1] - ia[j]) : 1.f;
+        const unsigned u[4] = {hv.x, hv.y, hv.z, hv.w};
 #pragma unroll
-        for (int q = 0; q < 4; ++q) s[q] += w * bf2f(h[q]);
+        for (int q = 0; q < 4; ++q) {
+            s[2 * q] += w * __uint_as_float(u[q] << 16);
+            s[2 * q + 1] += w * __uint_as_float(u[q] & 0xFFFF0000u);
+        }
     }
     if (TRANSPOSE) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) outf[(size_t)i * kH + f + q] += s[q];
+        float4 *o = reinterpret_cast<float4 *>(outf + (size_t)i * kH + f);
+        float4 a = o[0], b = o[1];
+        a.x += s[0]; a.y += s[1]; a.z += s[2]; a.w += s[3];
+        b.x += s[4]; b.y += s[5]; b.z += s[6]; b.w += s[7];
+        o[0] = a;
+        o[1] = b;
     } else {
-        const float inv = 1.f / (float)max(1, ia[i + 1] - ia[i]);
+        const float inv = 1.f / (float)max(1, k1 - ia[i]);
+        unsigned u[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) outb[(size_t)i * kH + f + q] = f2bf(s[q] * inv);
+        for (int q = 0; q < 4; ++q)
+            u[q] = (unsigned)f2bf(s[2 * q] * inv) | ((unsigned)f2bf(s[2 * q + 1] * inv) << 16);
+        *reinterpret_cast<uint4 *>(outb + (size_t)i * kH + f) = make_uint4(u[0], u[1], u[2], u[3]);
     }
 }
 
-// y_i = H_i . Wd + bd  (FP32)
-__global__ void mp_decode(int N, const bf16_t *__restrict__ H, const bf16_t *__restrict__ wd, const float *__restrict__ bd,
-                          float *__restrict__ y) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
+// y_i = H_i . Wd + bd  (FP32); 8 lanes per node, 8 features each, summed
+// across the 8 lanes
+__global__ void __launch_bounds__(256) mp_decode(int N, const bf16_t *__restrict__ H, const bf16_t *__restrict__ wd,
+                                                 const float *__restrict__ bd, float *__restrict__ y) {
+    const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int i = (int)(gid >> 3), f = (int)(gid & 7) * 8;
     float s = 0.f;
-    for (int f = 0; f < kH; ++f) s += bf2f(H[(size_t)i * kH + f]) * bf2f(wd[f]);
-    y[i] = s + bd[0];
+    if (i < N) {
+        const uint4 hv = *reinterpret_cast<const uint4 *>(H + (size_t)i * kH + f);
+        const unsigned u[4] = {hv.x, hv.y, hv.z, hv.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            s += __uint_as_float(u[q] << 16) * bf2f(wd[f + 2 * q]);
+            s += __uint_as_float(u[q] & 0xFFFF0000u) * bf2f(wd[f + 2 * q + 1]);
+        }
+    }
+    s += __shfl_xor(s, 1, 8);
+    s += __shfl_xor(s, 2, 8);
+    s += __shfl_xor(s, 4, 8);
+    if (i < N && (gid & 7) == 0) y[i] = s + bd[0];
 }
 
 // loss partials and dy = 2 (y - t) / N
@@ -365,28 +392,34 @@ static int forward_impl(fvp_mpnn *m) {
     for (int l = 0; l < m->L; ++l) {
         const bf16_t *Hl = m->H.p + l * NH;
         bf16_t *Al = m->A.p + l * NH;
-        mp_aggregate<false><<<nb((long)N * 16, 256), 256, 0, s>>>(N, m->ia.p, m->ja.p, Hl, Al, nullptr);
+        mp_aggregate<false><<<nb((long)N * 8, 256), 256, 0, s>>>(N, m->ia.p, m->ja.p, Hl, Al, nullptr);
         mp_gemm<EPI_RESID><<<g, 256, 0, s>>>(N, Hl, kH, pb + m->o_W1[l], Al, kH, pb + m->o_W2[l], m->p.p + m->o_b[l],
                                              Hl, m->Z.p + l * NH, m->H.p + (l + 1) * NH, nullptr);
     }
-    mp_decode<<<nb(N, 256), 256, 0, s>>>(N, m->H.p + m->L * NH, pb + m->o_Wd, m->p.p + m->o_bd, m->y.p);
+    mp_decode<<<nb((long)N * 8, 256), 256, 0, s>>>(N, m->H.p + m->L * NH, pb + m->o_Wd, m->p.p + m->o_bd, m->y.p);
     FVP_HIP(hipGetLastError());
     return 0;
 }
 
+constexpr int kRowsPerBlock = 512;
+
+// partial sums of nparts blocks reduced into out (zeroed by the caller)
+static void reduce_parts(fvp_mpnn *m, int nparts, int n, float *out) {
+    const int chunks = std::min(16, nparts), per = (nparts + chunks - 1) / chunks;
+    sum_parts<<<dim3(nb(n, 256), chunks), 256, 0, m->stream>>>(nparts, n, per, m->part.p, out);
+}
+
 static int wgrad(fvp_mpnn *m, const bf16_t *X, int K, const bf16_t *D, float *out) {
-    const int rows = 2048;
-    const int nbk = nb(m->N, rows);
-    mp_wgrad<<<nbk, 256, 0, m->stream>>>(m->N, rows, X, K, D, m->part.p);
-    sum_parts<<<nb(K * kH, 256), 256, 0, m->stream>>>(nbk, K * kH, m->part.p, out);
+    const int nbk = nb(m->N, kRowsPerBlock);
+    mp_wgrad<<<nbk, 256, 0, m->stream>>>(m->N, kRowsPerBlock, X, K, D, m->part.p);
+    reduce_parts(m, nbk, K * kH, out);
     return 0;
 }
 
 static int colsum(fvp_mpnn *m, const bf16_t *D, float *out) {
-    const int rows = 2048;
-    const int nbk = nb(m->N, rows);
-    mp_colsum<<<nbk, 256, 0, m->stream>>>(m->N, rows, D, m->part.p);
-    sum_parts<<<1, kH, 0, m->stream>>>(nbk, kH, m->part.p, out);
+    const int nbk = nb(m->N, kRowsPerBlock);
+    mp_colsum<<<nbk, 256, 0, m->stream>>>(m->N, kRowsPerBlock, D, m->part.p);
+    reduce_parts(m, nbk, kH, out);
     return 0;
 }
 
@@ -396,10 +429,11 @@ static int backward_impl(fvp_mpnn *m) {
     const size_t NH = (size_t)N * kH;
     const int g = nb(N, 128);
     float *G = m->g.p;
+    FVP_HIP(hipMemsetAsync(G, 0, sizeof(float) * m->np, s));  // every gradient below accumulates
     // decoder
     const int nbd = std::min(1024, nb(N, 256));
     mp_decode_bwd<<<nbd, 256, 0, s>>>(N, m->H.p + L * NH, m->pb.p + m->o_Wd, m->dy.p, m->dH.p, m->part.p);
-    sum_parts<<<1, kH + 1, 0, s>>>(nbd, kH + 1, m->part.p, G + m->o_Wd);  // Wd (64) then bd (1): adjacent
+    reduce_parts(m, nbd, kH + 1, G + m->o_Wd);  // Wd (64) then bd (1): adjacent
     for (int l = L - 1; l >= 0; --l) {
         const bf16_t *Hin = m->H.p + l * NH, *Al = m->A.p + l * NH, *Zl = m->Z.p + l * NH;
         mp_relu_bwd<<<nb((long)NH, 256), 256, 0, s>>>((long)NH, m->dH.p, Zl, m->dZ.p);
@@ -411,7 +445,7 @@ static int backward_impl(fvp_mpnn *m) {
                                             nullptr, nullptr, nullptr, m->dA.p, nullptr);
         mp_gemm<EPI_ADD><<<g, 256, 0, s>>>(N, m->dZ.p, kH, m->wt1.p + (size_t)l * kH * kH, nullptr, 0, nullptr,
                                            nullptr, nullptr, nullptr, nullptr, m->dH.p);
-        mp_aggregate<true><<<nb((long)N * 16, 256), 256, 0, s>>>(N, m->ia.p, m->ja.p, m->dA.p, nullptr, m->dH.p);
+        mp_aggregate<true><<<nb((long)N * 8, 256), 256, 0, s>>>(N, m->ia.p, m->ja.p, m->dA.p, nullptr, m->dH.p);
     }
     // encoder: relu mask from H0 > 0
     mp_relu_bwd<<<nb((long)NH, 256), 256, 0, s>>>((long)NH, m->dH.p, m->H.p, m->dZ.p);
@@ -461,7 +495,7 @@ extern "C" int fvp_mpnn_create(int device, int n_nodes, const int *ia, const int
             p0[m->o_W2[l] + k] = (float)(gauss() * std::sqrt(1.0 / kH));
         }
     for (long k = 0; k < kH; ++k) p0[m->o_Wd + k] = (float)(gauss() * std::sqrt(1.0 / kH));
-    const int nparts = std::max(nb(n_nodes, 2048), 1024) * (kF > kH ? kF : kH) * kH;
+    const int nparts = std::max(nb(n_nodes, kRowsPerBlock), 1024) * (kF > kH ? kF : kH) * kH;
     if (m->ia.upload(ia, n_nodes + 1) || m->ja.upload(ja, ia[n_nodes]) || m->p.upload(p0.data(), m->np) ||
         m->m.alloc(m->np) || m->v.alloc(m->np) || m->g.alloc(m->np) || m->pb.alloc(m->np) ||
         m->wt1.alloc((size_t)layers * kH * kH) || m->wt2.alloc((size_t)layers * kH * kH) ||

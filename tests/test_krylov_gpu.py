@@ -95,17 +95,20 @@ def test_gmres_known_answer(mats, which, ptype, reorth):
     from fvmpor_amd.lib import CsrMatrix
     ia, ja, a = mats[which]
     n = len(ia) - 1
+    b = O.csr_gemv(ia, ja, a, np.ones(n))
+    xo, sto = O.gmres_csr(ia, ja, a, b, precon=ptype, m=10, max_restarts=100, tol=1e-12)
+    assert sto["converged"], sto
     A = CsrMatrix(ia, ja)
     A.set_values(_dev(torch, a))
     assert A.precon_build(ptype) == 0
-    b = O.csr_gemv(ia, ja, a, np.ones(n))
     x = torch.zeros(n, dtype=torch.float64, device=device())
+    torch.cuda.synchronize()
     rc, st = A.gmres(_dev(torch, b), x, m=30, max_restarts=40, tol=1e-12, reorth=reorth)
     assert rc == 0 and st["converged"], st
     assert np.abs(x.cpu().numpy() - 1.0).max() < 1e-9
-    xo, sto = O.gmres_csr(ia, ja, a, b, precon=ptype, m=10, max_restarts=100, tol=1e-12)
-    assert sto["converged"]
     np.testing.assert_allclose(x.cpu().numpy(), xo, atol=1e-9)
+    # the host inputs are untouched by the device solve
+    assert np.array_equal(b, O.csr_gemv(ia, ja, a, np.ones(n)))
 
 
 @pytest.mark.parametrize("which", ["slice2d", "box3d_small"])
