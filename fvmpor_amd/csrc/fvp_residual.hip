@@ -121,21 +121,45 @@ __device__ __forceinline__ ElemRec load_elem(int e, const int4 *__restrict__ con
     return r;
 }
 
-template <int DIM, int MODE, bool JV = false>
-__device__ __forceinline__ void element_fluxes_rec(int e, const ElemRec &rec, int N, const double *zk, double rho0,
-                                                   const double *__restrict__ xyz, const double *__restrict__ psi,
-                                                   const double *__restrict__ v, double sig,
-                                                   const double *__restrict__ kr, double *__restrict__ qface,
-                                                   int n[4], double f[4]) {
+// t ? x : y on the bit patterns: keeps the choice between two register
+// values a choice of values (a plain conditional between two elements of a
+// register array is turned into an indexed load from the stack)
+__device__ __forceinline__ double bit_select(unsigned t, double x, double y) {
+    const unsigned long long m = 0ULL - (unsigned long long)(t & 1u);
+    return __longlong_as_double((__double_as_longlong(x) & (long long)m) | (__double_as_longlong(y) & (long long)~m));
+}
+
+// One element's node values: coordinates, head (J v: psi + sig v) and the
+// relative permeability of the element's zone at each node.
+template <int DIM>
+struct NodeVals {
+    double px[DIM + 1], py[DIM + 1], pz[DIM + 1], h[DIM + 1], k[DIM + 1];
+};
+
+template <int DIM, bool JV>
+__device__ __forceinline__ void load_nodes(const ElemRec &rec, int N, const double *__restrict__ xyz,
+                                           const double *__restrict__ psi, const double *__restrict__ v, double sig,
+                                           const double *__restrict__ kr, NodeVals<DIM> &nv) {
+    const int n[4] = {rec.c4.x, rec.c4.y, rec.c4.z, rec.c4.w};
+    const double *krz = kr + (size_t)rec.z * N;
+#pragma unroll
+    for (int k = 0; k < DIM + 1; ++k) {
+        load_pt(xyz, n[k], nv.px[k], nv.py[k], nv.pz[k]);
+        nv.h[k] = JV ? psi[n[k]] + sig * v[n[k]] : psi[n[k]];
+        nv.k[k] = krz[n[k]];
+    }
+}
+
+// The element's face fluxes from its record and node values; f[k] = net mass
+// flow into node k (n[k] its node ids).
+template <int DIM, int MODE>
+__device__ __forceinline__ void element_fluxes_vals(int e, const ElemRec &rec, const NodeVals<DIM> &nv,
+                                                    const double *zk, double rho0, double *__restrict__ qface,
+                                                    int n[4], double f[4]) {
     constexpr int NPE = DIM + 1;
     constexpr int NED = DIM == 2 ? 3 : 6;
     n[0] = rec.c4.x; n[1] = rec.c4.y; n[2] = rec.c4.z; n[3] = rec.c4.w;
-    double px[NPE], py[NPE], pz[NPE], h[NPE];
-#pragma unroll
-    for (int k = 0; k < NPE; ++k) {
-        load_pt(xyz, n[k], px[k], py[k], pz[k]);
-        h[k] = JV ? psi[n[k]] + sig * v[n[k]] : psi[n[k]];
-    }
+    const double *px = nv.px, *py = nv.py, *pz = nv.pz, *h = nv.h;
     const int z = rec.z;
     const double km = rec.km;
     double G[NPE][3];  // adjugate rows: grad N_k * det
@@ -187,12 +211,11 @@ __device__ __forceinline__ void element_fluxes_rec(int e, const ElemRec &rec, in
         double flux = sc * (g[b] - g[a]);  // flow a -> b
         double krf;
         if (MODE == 0) {
-            int up = (mask >> j) & 1u ? n[a] : n[b];
-            krf = kr[(size_t)z * N + up];
+            krf = bit_select((mask >> j) & 1u, nv.k[a], nv.k[b]);
         } else {
-            const bool a_front = (orient >> j) & 1u;
-            double kf = kr[(size_t)z * N + (a_front ? n[a] : n[b])];
-            double kb = kr[(size_t)z * N + (a_front ? n[b] : n[a])];
+            const unsigned a_front = (orient >> j) & 1u;
+            double kf = bit_select(a_front, nv.k[a], nv.k[b]);
+            double kb = bit_select(a_front, nv.k[b], nv.k[a]);
             krf = kf * 0.5;
             krf += kb * 0.5;
             if (MODE == 2) qface[(size_t)e * NED + j] = krf * (a_front ? -flux : flux);  // back -> front
@@ -201,6 +224,17 @@ __device__ __forceinline__ void element_fluxes_rec(int e, const ElemRec &rec, in
         f[a] -= M;
         f[b] += M;
     }
+}
+
+template <int DIM, int MODE, bool JV = false>
+__device__ __forceinline__ void element_fluxes_rec(int e, const ElemRec &rec, int N, const double *zk, double rho0,
+                                                   const double *__restrict__ xyz, const double *__restrict__ psi,
+                                                   const double *__restrict__ v, double sig,
+                                                   const double *__restrict__ kr, double *__restrict__ qface,
+                                                   int n[4], double f[4]) {
+    NodeVals<DIM> nv;
+    load_nodes<DIM, JV>(rec, N, xyz, psi, v, sig, kr, nv);
+    element_fluxes_vals<DIM, MODE>(e, rec, nv, zk, rho0, qface, n, f);
 }
 
 template <int DIM, int MODE, bool JV = false>
@@ -241,6 +275,17 @@ __global__ void __launch_bounds__(256) elem_kernel(int T, int N, ZoneTable zt, d
     }
 }
 
+// Export slots of an element evaluated in tile [lo, hi): -1 for owned nodes,
+// else the next slot of the element's run starting at xd[x] (foreign nodes in
+// local order).
+template <int DIM>
+__device__ __forceinline__ void owner_exports(const ElemRec &r, int lo, int hi, const int *__restrict__ xd, int x,
+                                              int d[4]) {
+    const int nn[4] = {r.c4.x, r.c4.y, r.c4.z, r.c4.w};
+#pragma unroll
+    for (int q = 0; q < DIM + 1; ++q) d[q] = (nn[q] >= lo && nn[q] < hi) ? -1 : xd[x++];
+}
+
 // Element-owner residual: a workgroup owns the nodes of one tile and evaluates
 // the tile's primary elements (lowest node in the tile; every element has
 // exactly one primary tile, so each element is evaluated once), gathering node
@@ -251,7 +296,7 @@ __global__ void __launch_bounds__(256) elem_kernel(int T, int N, ZoneTable zt, d
 // xdst[export]; slots are grouped by target node, so the finalisation reads
 // each node's imports contiguously.  No global atomics.
 template <int DIM, bool JV>
-__global__ void __launch_bounds__(1024) owner_kernel(
+__global__ void __launch_bounds__(512) owner_kernel(
     int N, int ntiles, int tile_nodes, int per_xcd, ZoneTable zt, double rho0, const int4 *__restrict__ conn,
     const uint8_t *__restrict__ ezone, const double *__restrict__ kmul, const uint8_t *__restrict__ emask,
     const double *__restrict__ xyz, const double *__restrict__ psi, const double *__restrict__ kr,
@@ -269,21 +314,45 @@ __global__ void __launch_bounds__(1024) owner_kernel(
     __syncthreads();
     const int *xd = xdst + xptr[tile];
     const int e1 = eptr[tile + 1];
-    // the next element's record is loaded while the current one is evaluated
+    const int bs = blockDim.x;
+    // Two-stage software pipeline over the thread's elements: while element e
+    // is evaluated, the node values and export slots of element e + bs and the
+    // record of element e + 2 bs are in flight, so the gathers' latency hides
+    // behind the flux arithmetic.  Look-ahead indices past the tile's range are
+    // clamped to its last element (loaded, never used).
     int e = eptr[tile] + threadIdx.x;
-    ElemRec nxt;
-    if (e < e1) nxt = load_elem<0>(e, conn, ezone, kmul, emask, nullptr);
-    for (; e < e1; e += blockDim.x) {
-        const ElemRec cur = nxt;
-        if (e + (int)blockDim.x < e1) nxt = load_elem<0>(e + blockDim.x, conn, ezone, kmul, emask, nullptr);
-        int n[4];
-        double f[4];
-        element_fluxes_rec<DIM, 0, JV>(e, cur, N, zk, rho0, xyz, psi, jv.v, sig, kr, nullptr, n, f);
-        int x = xoff[e];
+    if (e < e1) {
+        const int last = e1 - 1;
+        ElemRec cur = load_elem<0>(e, conn, ezone, kmul, emask, nullptr);
+        NodeVals<DIM> vc;
+        load_nodes<DIM, JV>(cur, N, xyz, psi, jv.v, sig, kr, vc);
+        int dc[4];
+        owner_exports<DIM>(cur, lo, hi, xd, xoff[e], dc);
+        int en = min(e + bs, last);
+        ElemRec nxt = load_elem<0>(en, conn, ezone, kmul, emask, nullptr);
+        int xn = xoff[en];
+        for (; e < e1; e += bs) {
+            NodeVals<DIM> vn;
+            int dn[4];
+            load_nodes<DIM, JV>(nxt, N, xyz, psi, jv.v, sig, kr, vn);
+            owner_exports<DIM>(nxt, lo, hi, xd, xn, dn);
+            const int e2 = min(e + 2 * bs, last);
+            const ElemRec nx2 = load_elem<0>(e2, conn, ezone, kmul, emask, nullptr);
+            const int x2 = xoff[e2];
+            int n[4];
+            double f[4];
+            element_fluxes_vals<DIM, 0>(e, cur, vc, zk, rho0, nullptr, n, f);
 #pragma unroll
-        for (int q = 0; q < DIM + 1; ++q) {
-            if (n[q] >= lo && n[q] < hi) atomicAdd(&sm[n[q] - lo], f[q]);
-            else xf[xd[x++]] = f[q];
+            for (int q = 0; q < DIM + 1; ++q) {
+                if (dc[q] < 0) atomicAdd(&sm[n[q] - lo], f[q]);
+                else xf[dc[q]] = f[q];
+            }
+            cur = nxt;
+            vc = vn;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) dc[q] = dn[q];
+            nxt = nx2;
+            xn = x2;
         }
     }
     __syncthreads();
