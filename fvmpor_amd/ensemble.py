@@ -140,7 +140,8 @@ def union_problem(problems):
     return prob, np.array(offsets, dtype=np.int64)
 
 
-def run_batch(problems, t_end=0.5, rtol=1e-6, atol=1e-6, precon=None, block_rows=0, stream=None):
+def run_batch(problems, t_end=0.5, rtol=1e-6, atol=1e-6, precon=None, block_rows=0, stream=None, progress=1,
+              log=None):
     """Advance one batch (instances of one dimension) to t_end on cuda:0.
     Returns per-instance final heads, the integrator statistics and the wall
     time of the advance (set-up excluded)."""
@@ -158,7 +159,11 @@ def run_batch(problems, t_end=0.5, rtol=1e-6, atol=1e-6, precon=None, block_rows
     out = torch.empty_like(psi0)
     torch.cuda.synchronize()
     w0 = time.perf_counter()
-    integ.advance(t_end, out)
+    for q in range(1, progress + 1):  # advance in chunks so that long batches report progress
+        integ.advance(t_end * q / progress, out)
+        if log is not None and progress > 1:
+            st = integ.stats()
+            log(f"  t={st['t']:.4g}: {st['steps']} steps, {time.perf_counter() - w0:.1f}s")
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
     psi = out.cpu().numpy()
@@ -201,7 +206,8 @@ def main(argv=None):
         if not batch:
             continue
         heads, st, wall = run_batch(batch, t_end=args.t_end, rtol=args.rtol, atol=args.rtol,
-                                    precon=PRECON_BJILU0_F32)
+                                    precon=PRECON_BJILU0_F32, progress=10,
+                                    log=lambda m: print(m, file=sys.stderr, flush=True))
         torch.cuda.synchronize()
         nodes = sum(p.mesh.n_nodes for p in batch)
         wall_total += wall

@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfvp.so")
+# FVP_LIBRARY selects another build of the same library (e.g. a variant being timed)
+LIB_PATH = os.environ.get("FVP_LIBRARY") or os.path.join(_HERE, "libfvp.so")
 _LIB = None
 
 ZONE_FIELDS = 8
