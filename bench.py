@@ -111,6 +111,22 @@ def cpu_baseline(args, gpu_window_days, gpu_window_residuals, n_nodes_gpu):
     }
 
 
+def measured_traffic(n_nodes, n_elements, launch):
+    """HBM bytes per launch of the element kernel measured with rocprofv3
+    FETCH_SIZE / WRITE_SIZE counter runs (scaled by a plain copy of known size)
+    for this workload and launch shape, from profiles/; None when no
+    measurement matches."""
+    path = os.path.join(ROOT, "profiles", "owner_kernel_box3d_1M_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        m = json.load(f)
+    if (m.get("nodes"), m.get("elements")) != (n_nodes, n_elements) or m.get("launch") != [
+            launch["variant"], launch["tile_nodes"], launch["xcd_map"]]:
+        return None
+    return m
+
+
 def run_window(args, prob, ctx, psi0, precon, t_stop=None, steps=None, learned=None, profile=False):
     """One integrator from t = 0 on the context: W untimed warm-up steps, then
     either exactly `steps` timed steps or steps up to t_stop.  Returns the
@@ -259,6 +275,7 @@ def main():
     kname = "owner_kernel<3> (residual element fluxes)" if owner else "tile_lds_kernel<3> (residual)"
     ek_avg_s = ek_ms / max(ek_n, 1) * 1e-3
     achieved = kbytes / ek_avg_s / 1e9
+    traffic = measured_traffic(N, T, launch)
     rs_avg_s = rs_ms / max(rs_n, 1) * 1e-3
     log(f"timed {args.steps} steps: {days:.4e} days in {wall:.2f}s; element kernel {ek_avg_s*1e6:.1f} us x {ek_n}, "
         f"residual {rs_ms/max(rs_n,1)*1e3:.1f} us x {rs_n}, precon solve {pc_ms/max(pc_n,1)*1e3:.1f} us x {pc_n}")
@@ -303,7 +320,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
+            "traffic_source": "profiles/owner_kernel_box3d_1M_traffic.json" if traffic else None,
             "bytes_per_launch": kbytes,
             "avg_launch_us": ek_avg_s * 1e6,
             "launches": ek_n,

@@ -130,6 +130,8 @@ void jac_free(fvp_ctx *c);
 // record a start / stop event of kind k on the context stream when profiling
 void prof_mark(fvp_ctx *c, int k, int stop);
 void prof_release(fvp_ctx *c);
+// a start / stop event pair of kind k for one launch (null when not profiling)
+void prof_pair(fvp_ctx *c, int k, hipEvent_t *a, hipEvent_t *b);
 // node count of a network's graph
 int mpnn_nodes(const fvp_mpnn *m);
 }  // namespace fvp

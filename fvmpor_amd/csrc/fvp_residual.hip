@@ -19,6 +19,7 @@
 //                  (fvmpor_ODE_impl.cpp:99-115)
 //   elem_kernel    per element, atomic scatter: the averaged-weight pass of
 //                  begin_step (fvmpor_ODE_impl.cpp:118-142), then finalize
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -691,26 +692,33 @@ static void launch_tile(fvp_ctx *c, const double *psi, const double *psip, doubl
     hipStream_t s = c->stream;
     if (c->rvariant == FVP_RESIDUAL_OWNER) {
         const int N = c->N;
-        prof_mark(c, 0, 0);
-        // one workgroup per tile, wide enough that large tiles still fill the CUs
+        // one workgroup per tile, wide enough that large tiles still fill the CUs; when
+        // profiling, the events are part of the dispatch (the kernel's own duration)
         const int threads = c->tile_nodes >= 1024 ? 512 : 256;
-        owner_kernel<DIM, JV><<<c->xcd_map ? per * 8 : c->ntiles, threads, sizeof(double) * c->tile_nodes, s>>>(
-            N, c->ntiles, c->tile_nodes, c->xcd_map ? per : 0, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
-            c->emask.p, c->xyz.p, psi, c->kr.p, c->t_eptr.p, c->t_bptr.p, c->bnode.p, c->bval.p, c->t_xoff.p,
-            c->t_xptr.p, c->x_ilist.p, c->xf.p, c->acc.p, jv);
-        prof_mark(c, 0, 1);
+        hipEvent_t ea, eb;
+        prof_pair(c, 0, &ea, &eb);
+        hipExtLaunchKernelGGL(owner_kernel<DIM, JV>, dim3(c->xcd_map ? per * 8 : c->ntiles), dim3(threads),
+                              sizeof(double) * c->tile_nodes, s, ea, eb, 0, N, c->ntiles, c->tile_nodes,
+                              c->xcd_map ? per : 0, c->zt, c->rho0, (const int4 *)c->conn.p,
+                              (const uint8_t *)c->ezone.p, (const double *)c->kmul.p, (const uint8_t *)c->emask.p,
+                              (const double *)c->xyz.p, psi, (const double *)c->kr.p, (const int *)c->t_eptr.p,
+                              (const int *)c->t_bptr.p, (const int *)c->bnode.p, (const double *)c->bval.p,
+                              (const unsigned short *)c->t_xoff.p, (const int *)c->t_xptr.p,
+                              (const int *)c->x_ilist.p, c->xf.p, c->acc.p, jv);
         owner_finalize<JV><<<c->ntiles, 256, sizeof(double) * c->tile_nodes, s>>>(
             N, c->ntiles, c->tile_nodes, c->acc.p, c->x_iptr.p, c->x_tgt.p, c->xf.p, c->invV.p, psi, psip, c->cap.p,
             c->t_dptr.p, c->dnode.p, c->dval.p, F, jv);
         return;
     }
-    prof_mark(c, 0, 0);
+    hipEvent_t ea, eb;
+    prof_pair(c, 0, &ea, &eb);
+    if (ea) (void)hipEventRecord(ea, s);
     tile_lds_kernel<DIM, JV><<<c->xcd_map ? per * 8 : c->ntiles, 256, c->tile_lds, c->stream>>>(
         c->N, c->ntiles, c->tile_nodes, c->xcd_map ? per : 0, c->win_max, c->ntz_max, c->zt, c->rho0, c->ezone.p, c->kmul.p, c->emask.p, c->xyz.p,
         psi, c->kr.p, c->t_eptr.p, c->t_hptr.p, c->t_hlist.p, c->t_wptr.p, c->t_whalo.p, c->t_vptr.p,
         (const ushort4 *)c->t_lconn.p, c->t_tz.p, c->invV.p, psip, c->cap.p, c->t_bptr.p, c->bnode.p, c->bval.p,
         c->t_dptr.p, c->dnode.p, c->dval.p, F, jv);
-    prof_mark(c, 0, 1);
+    if (eb) (void)hipEventRecord(eb, s);
 }
 
 int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, double *F, int averaging,
@@ -801,6 +809,22 @@ void prof_mark(fvp_ctx *c, int k, int stop) {
         (void)hipEventRecord(c->prof_ev[k][c->prof_used[k]].second, c->stream);
         ++c->prof_used[k];
     }
+}
+
+// a start / stop event pair of kind k for one launch (both null when not profiling)
+void prof_pair(fvp_ctx *c, int k, hipEvent_t *a, hipEvent_t *b) {
+    *a = *b = nullptr;
+    if (!c->prof) return;
+    if (c->prof_used[k] == 4096) prof_fold(c, k);
+    if (c->prof_used[k] == c->prof_ev[k].size()) {
+        hipEvent_t e0, e1;
+        if (hipEventCreate(&e0) != hipSuccess) return;
+        if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return; }
+        c->prof_ev[k].push_back({e0, e1});
+    }
+    *a = c->prof_ev[k][c->prof_used[k]].first;
+    *b = c->prof_ev[k][c->prof_used[k]].second;
+    ++c->prof_used[k];
 }
 
 void prof_release(fvp_ctx *c) {
