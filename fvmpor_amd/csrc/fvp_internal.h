@@ -37,6 +37,10 @@ struct DevBuf {
 
 struct fvp_jacdata;
 
+// profiling kinds: 0 residual element kernel, 1 whole residual, 2 preconditioner
+// solve, 3 J v element kernel
+constexpr int kProfKinds = 4;
+
 struct fvp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -106,10 +110,10 @@ struct fvp_ctx {
     // live timing: event pairs per kind (0 element kernel, 1 residual, 2 precon solve)
     long nres = 0;  // residual evaluations issued on this context
     int prof = 0;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[3];
-    size_t prof_used[3] = {0, 0, 0};
-    double prof_ms[3] = {0, 0, 0};
-    long prof_n[3] = {0, 0, 0};
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[kProfKinds];
+    size_t prof_used[kProfKinds] = {};
+    double prof_ms[kProfKinds] = {};
+    long prof_n[kProfKinds] = {};
 };
 
 namespace fvp {

@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(256) mp_gemm(int N, const bf16_t *__restrict__
                                                const bf16_t *__restrict__ W2, const float *__restrict__ bias,
                                                const bf16_t *__restrict__ Hin, bf16_t *__restrict__ Zb,
                                                bf16_t *__restrict__ Yb, float *__restrict__ Yf) {
-    __shared__ bf16_t wt[2][kH * kH];  // transposed weights: wt[n * K + k]
+    __shared__ __align__(16) bf16_t wt[2][kH * kH];  // transposed weights: wt[n * K + k]
     for (int i = threadIdx.x; i < K1 * kH; i += blockDim.x) {
         const int k = i / kH, n = i % kH;
         wt[0][n * K1 + k] = W1[i];
@@ -75,14 +75,11 @@ __global__ void __launch_bounds__(256) mp_gemm(int N, const bf16_t *__restrict__
         const int K = src ? K2 : K1;
         const bf16_t *w = wt[src];
         for (int kb = 0; kb < K; kb += 8) {
+            // 4 consecutive BF16 values per lane: one 8-byte load (K and kb + kq are multiples of 4)
             short4_t a = {0, 0, 0, 0};
-            if (row < N) {
-                const bf16_t *xp = X + (size_t)row * K + kb + kq;
-                a = short4_t{(short)xp[0], (short)xp[1], (short)xp[2], (short)xp[3]};
-            }
-            const bf16_t *b0p = w + i * K + kb + kq, *b1p = w + (32 + i) * K + kb + kq;
-            const short4_t b0 = {(short)b0p[0], (short)b0p[1], (short)b0p[2], (short)b0p[3]};
-            const short4_t b1 = {(short)b1p[0], (short)b1p[1], (short)b1p[2], (short)b1p[3]};
+            if (row < N) a = *reinterpret_cast<const short4_t *>(X + (size_t)row * K + kb + kq);
+            const short4_t b0 = *reinterpret_cast<const short4_t *>(w + i * K + kb + kq);
+            const short4_t b1 = *reinterpret_cast<const short4_t *>(w + (32 + i) * K + kb + kq);
             acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
             acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1, acc1, 0, 0, 0);
         }

@@ -696,7 +696,7 @@ static void launch_tile(fvp_ctx *c, const double *psi, const double *psip, doubl
         // profiling, the events are part of the dispatch (the kernel's own duration)
         const int threads = c->tile_nodes >= 1024 ? 512 : 256;
         hipEvent_t ea, eb;
-        prof_pair(c, 0, &ea, &eb);
+        prof_pair(c, JV ? 3 : 0, &ea, &eb);
         hipExtLaunchKernelGGL(owner_kernel<DIM, JV>, dim3(c->xcd_map ? per * 8 : c->ntiles), dim3(threads),
                               sizeof(double) * c->tile_nodes, s, ea, eb, 0, N, c->ntiles, c->tile_nodes,
                               c->xcd_map ? per : 0, c->zt, c->rho0, (const int4 *)c->conn.p,
@@ -711,7 +711,7 @@ static void launch_tile(fvp_ctx *c, const double *psi, const double *psip, doubl
         return;
     }
     hipEvent_t ea, eb;
-    prof_pair(c, 0, &ea, &eb);
+    prof_pair(c, JV ? 3 : 0, &ea, &eb);
     if (ea) (void)hipEventRecord(ea, s);
     tile_lds_kernel<DIM, JV><<<c->xcd_map ? per * 8 : c->ntiles, 256, c->tile_lds, c->stream>>>(
         c->N, c->ntiles, c->tile_nodes, c->xcd_map ? per : 0, c->win_max, c->ntz_max, c->zt, c->rho0, c->ezone.p, c->kmul.p, c->emask.p, c->xyz.p,
@@ -800,8 +800,8 @@ void prof_mark(fvp_ctx *c, int k, int stop) {
         if (c->prof_used[k] == 4096) prof_fold(c, k);
         if (c->prof_used[k] == c->prof_ev[k].size()) {
             hipEvent_t a, b;
-            if (hipEventCreate(&a) != hipSuccess) return;
-            if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }
+            if (hipEventCreateWithFlags(&a, hipEventDisableSystemFence) != hipSuccess) return;
+            if (hipEventCreateWithFlags(&b, hipEventDisableSystemFence) != hipSuccess) { (void)hipEventDestroy(a); return; }
             c->prof_ev[k].push_back({a, b});
         }
         (void)hipEventRecord(c->prof_ev[k][c->prof_used[k]].first, c->stream);
@@ -818,8 +818,8 @@ void prof_pair(fvp_ctx *c, int k, hipEvent_t *a, hipEvent_t *b) {
     if (c->prof_used[k] == 4096) prof_fold(c, k);
     if (c->prof_used[k] == c->prof_ev[k].size()) {
         hipEvent_t e0, e1;
-        if (hipEventCreate(&e0) != hipSuccess) return;
-        if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return; }
+        if (hipEventCreateWithFlags(&e0, hipEventDisableSystemFence) != hipSuccess) return;
+        if (hipEventCreateWithFlags(&e1, hipEventDisableSystemFence) != hipSuccess) { (void)hipEventDestroy(e0); return; }
         c->prof_ev[k].push_back({e0, e1});
     }
     *a = c->prof_ev[k][c->prof_used[k]].first;
@@ -828,7 +828,7 @@ void prof_pair(fvp_ctx *c, int k, hipEvent_t *a, hipEvent_t *b) {
 }
 
 void prof_release(fvp_ctx *c) {
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < kProfKinds; ++k) {
         for (auto &e : c->prof_ev[k]) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         c->prof_ev[k].clear();
         c->prof_used[k] = 0;
@@ -915,13 +915,13 @@ extern "C" int fvp_profile(fvp_ctx *c, int enable) {
     if (!c) { set_error("null context"); return -1; }
     FVP_HIP(hipSetDevice(c->device));
     FVP_HIP(hipStreamSynchronize(c->stream));
-    for (int k = 0; k < 3; ++k) { c->prof_used[k] = 0; c->prof_ms[k] = 0.; c->prof_n[k] = 0; }
+    for (int k = 0; k < kProfKinds; ++k) { c->prof_used[k] = 0; c->prof_ms[k] = 0.; c->prof_n[k] = 0; }
     c->prof = enable ? 1 : 0;
     return 0;
 }
 
 extern "C" int fvp_profile_get(fvp_ctx *c, int kind, double *total_ms, long *count) {
-    if (!c || kind < 0 || kind > 2) { set_error("invalid argument"); return -1; }
+    if (!c || kind < 0 || kind >= kProfKinds) { set_error("invalid argument"); return -1; }
     FVP_HIP(hipSetDevice(c->device));
     prof_fold(c, kind);
     if (total_ms) *total_ms = c->prof_ms[kind];

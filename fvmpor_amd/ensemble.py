@@ -140,7 +140,7 @@ def union_problem(problems):
     return prob, np.array(offsets, dtype=np.int64)
 
 
-def run_batch(problems, t_end=0.5, rtol=1e-6, atol=1e-6, precon=None, block_rows=0, stream=None, progress=1,
+def run_batch(problems, t_end=0.5, rtol=1e-6, atol=1e-6, precon=None, block_rows=0, stream=None, progress=20.0,
               log=None):
     """Advance one batch (instances of one dimension) to t_end on cuda:0.
     Returns per-instance final heads, the integrator statistics and the wall
@@ -149,6 +149,9 @@ def run_batch(problems, t_end=0.5, rtol=1e-6, atol=1e-6, precon=None, block_rows
 
     from .lib import PRECON_BJILU0_F32, Context, Integrator
     union, offsets = union_problem(problems)
+    if log is not None:
+        log(f"  union of {len(problems)} instances: {union.mesh.n_nodes} nodes, {union.mesh.n_elements} elements")
+    t_setup = time.perf_counter()
     ctx = Context(union)
     if stream is not None:
         ctx.set_stream(stream)
@@ -156,14 +159,20 @@ def run_batch(problems, t_end=0.5, rtol=1e-6, atol=1e-6, precon=None, block_rows
     integ = Integrator(ctx, psi0, rtol=rtol, atol=atol, max_order=3,
                        precon=PRECON_BJILU0_F32 if precon is None else precon, block_rows=block_rows)
     integ.set_segments(offsets)
+    if log is not None:
+        log(f"  set-up {time.perf_counter() - t_setup:.1f}s")
     out = torch.empty_like(psi0)
     torch.cuda.synchronize()
     w0 = time.perf_counter()
-    for q in range(1, progress + 1):  # advance in chunks so that long batches report progress
-        integ.advance(t_end * q / progress, out)
-        if log is not None and progress > 1:
+    last = w0
+    while integ.stats()["t"] < t_end * (1 - 1e-12):
+        integ.step(t_end)
+        if log is not None and time.perf_counter() - last > progress:  # long batches report progress
+            last = time.perf_counter()
             st = integ.stats()
-            log(f"  t={st['t']:.4g}: {st['steps']} steps, {time.perf_counter() - w0:.1f}s")
+            log(f"  t={st['t']:.4g} h={st['h_last']:.3g}: {st['steps']} steps, {st['gmres_iters']} GMRES "
+                f"iterations, {last - w0:.1f}s")
+    integ.advance(t_end, out)
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
     psi = out.cpu().numpy()
@@ -206,7 +215,7 @@ def main(argv=None):
         if not batch:
             continue
         heads, st, wall = run_batch(batch, t_end=args.t_end, rtol=args.rtol, atol=args.rtol,
-                                    precon=PRECON_BJILU0_F32, progress=10,
+                                    precon=PRECON_BJILU0_F32,
                                     log=lambda m: print(m, file=sys.stderr, flush=True))
         torch.cuda.synchronize()
         nodes = sum(p.mesh.n_nodes for p in batch)

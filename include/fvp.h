@@ -274,8 +274,9 @@ int fvp_autotune(fvp_ctx *ctx, double t, const double *psi, const double *psip, 
 /* ---------------------------------------------------------------------------
  * Live kernel timing with HIP events on the context's stream (the reference's
  * util::TimerCuda, timer_cuda.h:11-55).  While enabled every residual
- * evaluation records its element-flux kernel and the whole residual; kind 0 =
- * element kernel, 1 = whole residual, 2 = preconditioner solve. */
+ * evaluation records its element-flux kernel (events attached to the kernel's
+ * dispatch) and the whole residual; kind 0 = element kernel of F, 1 = whole
+ * residual (F or J v), 2 = preconditioner solve, 3 = element kernel of J v. */
 int fvp_profile(fvp_ctx *ctx, int enable);
 int fvp_profile_get(fvp_ctx *ctx, int kind, double *total_ms, long *count);
 
