@@ -180,13 +180,73 @@ def run_batch(problems, t_end=0.5, rtol=1e-6, atol=1e-6, precon=None, block_rows
     return heads, integ.stats(), wall
 
 
+def run_one(problem, t_end=0.5, rtol=1e-6, atol=1e-6, precon=None, overlap=1, budget_s=None):
+    """Advance one instance on its own context (own HIP stream) to t_end, or
+    until budget_s wall seconds have passed.  Returns a result dict."""
+    import torch
+
+    from .lib import PRECON_BJILU0_F32, Context, Integrator
+    w0 = time.perf_counter()
+    ctx = Context(problem)
+    psi0 = torch.tensor(problem.psi0, device="cuda:0")
+    torch.cuda.synchronize()
+    integ = Integrator(ctx, psi0, rtol=rtol, atol=atol, max_order=3,
+                       precon=PRECON_BJILU0_F32 if precon is None else precon, overlap=overlap)
+    w1 = time.perf_counter()
+    status = "done"
+    while integ.stats()["t"] < t_end * (1 - 1e-12):
+        if budget_s is not None and time.perf_counter() - w1 > budget_s:
+            status = "budget"
+            break
+        rc = integ.step(t_end)
+        if rc != 0:
+            status = f"failed ({rc})"
+            break
+    out = torch.empty_like(psi0)
+    ctx.synchronize()
+    st = integ.stats()
+    integ.advance(st["t"], out)
+    ctx.synchronize()
+    heads = out.cpu().numpy()
+    integ.close()
+    ctx.close()
+    return {"t": st["t"], "status": status, "steps": st["steps"], "newton_fails": st["newton_fails"],
+            "error_fails": st["error_fails"], "gmres_iters": st["gmres_iters"], "setup_s": w1 - w0,
+            "run_s": time.perf_counter() - w1, "finite": bool(np.isfinite(heads).all()), "heads": heads}
+
+
+def run_concurrent(problems, t_end=0.5, streams=4, budget_s=None, log=None, **kw):
+    """Every instance on its own context and HIP stream, `streams` of them in
+    flight at once (one host thread each; the library releases the
+    interpreter during its calls), largest first.  Each instance keeps its own
+    step size, order and Newton / linear-solver history, so a stiff instance
+    slows only itself.  Returns the per-instance results and the wall time."""
+    from concurrent.futures import ThreadPoolExecutor
+    order = sorted(range(len(problems)), key=lambda i: -problems[i].mesh.n_nodes)
+    res = [None] * len(problems)
+    w0 = time.perf_counter()
+
+    def work(i):
+        r = run_one(problems[i], t_end=t_end, budget_s=budget_s, **kw)
+        if log is not None:
+            log(f"  instance {i}: {problems[i].mesh.n_nodes} nodes, t={r['t']:.4g} ({r['status']}), "
+                f"{r['steps']} steps, {r['newton_fails']} Newton failures, {r['run_s']:.1f}s")
+        return i, r
+
+    with ThreadPoolExecutor(max_workers=streams) as pool:
+        for i, r in pool.map(work, order):
+            res[i] = r
+    return res, time.perf_counter() - w0
+
+
 def _build(d):
     return make_instance(d)
 
 
 def main(argv=None):
     """Run a Monte Carlo ensemble to t_end on cuda:0 and print one JSON line:
-    python -m fvmpor_amd.ensemble [--n 256] [--t-end 0.5] [--workers 16]."""
+    python -m fvmpor_amd.ensemble [--n 256] [--t-end 0.5] [--mode concurrent|batch]
+    [--streams 4] [--budget 30]."""
     import argparse
     import json
     import sys
@@ -200,35 +260,61 @@ def main(argv=None):
     ap.add_argument("--t-end", type=float, default=0.5)
     ap.add_argument("--n-min", type=int, default=5000)
     ap.add_argument("--n-max", type=int, default=60000)
-    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--workers", type=int, default=16, help="host processes building the meshes")
     ap.add_argument("--rtol", type=float, default=1e-6)
+    ap.add_argument("--mode", default="concurrent", choices=["concurrent", "batch"])
+    ap.add_argument("--streams", type=int, default=4, help="instances in flight (concurrent mode)")
+    ap.add_argument("--budget", type=float, default=30.0, help="wall seconds per instance (concurrent mode)")
     args = ap.parse_args(argv)
+
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
     draws = draw_instances(args.n, n_min=args.n_min, n_max=args.n_max)
     t0 = time.perf_counter()
     with ProcessPoolExecutor(max_workers=args.workers) as pool:
         probs = list(pool.map(_build, draws, chunksize=4))
-    print(f"built {len(probs)} meshes in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
-    out = {"instances": len(probs), "t_end": args.t_end, "batches": []}
-    wall_total = 0.0
-    for dim in (2, 3):
-        batch = [p for p in probs if p.mesh.dim == dim]
-        if not batch:
-            continue
-        heads, st, wall = run_batch(batch, t_end=args.t_end, rtol=args.rtol, atol=args.rtol,
-                                    precon=PRECON_BJILU0_F32,
-                                    log=lambda m: print(m, file=sys.stderr, flush=True))
-        torch.cuda.synchronize()
-        nodes = sum(p.mesh.n_nodes for p in batch)
-        wall_total += wall
-        out["batches"].append({"dim": dim, "instances": len(batch), "nodes": nodes, "wall_s": wall,
-                               "steps": st["steps"], "newton_iters": st["newton_iters"],
-                               "gmres_iters": st["gmres_iters"], "precon_setups": st["precon_setups"],
-                               "finite": bool(all(np.isfinite(h).all() for h in heads))})
-        print(f"{dim}D batch: {len(batch)} instances, {nodes} nodes, {wall:.2f}s, {st['steps']} steps",
-              file=sys.stderr, flush=True)
-    out["wall_s"] = wall_total
-    out["instance_days_per_s"] = len(probs) * args.t_end / wall_total
-    out["node_days_per_s"] = sum(p.mesh.n_nodes for p in probs) * args.t_end / wall_total
+    log(f"built {len(probs)} meshes in {time.perf_counter() - t0:.1f}s")
+    out = {"instances": len(probs), "t_end": args.t_end, "mode": args.mode, "rtol": args.rtol,
+           "nodes": int(sum(p.mesh.n_nodes for p in probs))}
+    if args.mode == "concurrent":
+        res, wall = run_concurrent(probs, t_end=args.t_end, streams=args.streams, budget_s=args.budget, log=log,
+                                   rtol=args.rtol, atol=args.rtol)
+        done = [i for i, r in enumerate(res) if r["status"] == "done"]
+        days = sum(r["t"] for r in res)
+        out.update({
+            "streams": args.streams, "budget_s_per_instance": args.budget, "wall_s": wall,
+            "completed": len(done), "instance_days": days,
+            "instance_days_per_s": days / wall,
+            "node_days_per_s": sum(r["t"] * p.mesh.n_nodes for r, p in zip(res, probs)) / wall,
+            "all_finite": bool(all(r["finite"] for r in res)),
+            "steps": int(sum(r["steps"] for r in res)), "newton_fails": int(sum(r["newton_fails"] for r in res)),
+            "gmres_iters": int(sum(r["gmres_iters"] for r in res)),
+            "stragglers": [{"index": draws[i].index, "dim": draws[i].dim, "nodes": probs[i].mesh.n_nodes,
+                            "sigma_lnK": round(draws[i].sigma_lnK, 3), "rain": round(draws[i].rain, 4),
+                            "t": res[i]["t"], "status": res[i]["status"], "steps": res[i]["steps"],
+                            "newton_fails": res[i]["newton_fails"]}
+                           for i in range(len(res)) if res[i]["status"] != "done"],
+        })
+    else:
+        out["batches"] = []
+        wall_total = 0.0
+        for dim in (2, 3):
+            batch = [p for p in probs if p.mesh.dim == dim]
+            if not batch:
+                continue
+            heads, st, wall = run_batch(batch, t_end=args.t_end, rtol=args.rtol, atol=args.rtol,
+                                        precon=PRECON_BJILU0_F32, log=log)
+            torch.cuda.synchronize()
+            nodes = sum(p.mesh.n_nodes for p in batch)
+            wall_total += wall
+            out["batches"].append({"dim": dim, "instances": len(batch), "nodes": nodes, "wall_s": wall,
+                                   "steps": st["steps"], "newton_iters": st["newton_iters"],
+                                   "gmres_iters": st["gmres_iters"], "precon_setups": st["precon_setups"],
+                                   "finite": bool(all(np.isfinite(h).all() for h in heads))})
+            log(f"{dim}D batch: {len(batch)} instances, {nodes} nodes, {wall:.2f}s, {st['steps']} steps")
+        out["wall_s"] = wall_total
+        out["instance_days_per_s"] = len(probs) * args.t_end / wall_total
+        out["node_days_per_s"] = out["nodes"] * args.t_end / wall_total
     print(json.dumps(out), flush=True)
 
 
