@@ -141,8 +141,12 @@ def run_window(args, prob, ctx, psi0, precon, t_stop=None, steps=None, learned=N
     if learned is not None:
         integ.set_learned(*learned)
     T_END = 1.0
+    def step(t_stop):
+        rc = integ.step(t_stop)
+        if rc != 0:
+            raise RuntimeError(f"integrator step failed with status {rc} at t={integ.stats()['t']:.4g}")
     for _ in range(args.warmup):
-        integ.step(T_END)
+        step(T_END)
     torch.cuda.synchronize()
     s0 = integ.stats()
     if profile:
@@ -151,7 +155,7 @@ def run_window(args, prob, ctx, psi0, precon, t_stop=None, steps=None, learned=N
     w0 = time.perf_counter()
     i = 0
     while (steps is not None and i < steps) or (steps is None and integ.stats()["t"] < t_stop):
-        integ.step(T_END if steps is not None else t_stop)
+        step(T_END if steps is not None else t_stop)
         i += 1
         if i % 10 == 0:
             s = integ.stats()

@@ -58,6 +58,8 @@ struct fvp_integrator {
     DevBuf<double> y2, ypn2, G2;
     float *rec_feat = nullptr, *rec_targ = nullptr;
     double learn_trust = 2., learn_band = 0.5;
+    bool learn_took = false;   // this attempt started Newton from the network's proposal
+    bool learn_block = false;  // retries of this step start from the predictor
 };
 
 namespace fvp {
@@ -298,12 +300,14 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
     };
     deriv();
     if (fvp::residual_impl(c, t, I->y.p, I->ypn.p, I->G.p, c->averaging)) return -1;
-    if (I->mp || I->rec_feat) {
+    I->learn_took = false;
+    const bool learn = I->mp && !I->learn_block;
+    if (learn || I->rec_feat) {
         float *x = I->rec_feat ? I->rec_feat : I->feat.p;
         learn_features<<<nb(4 * N), 256, 0, s>>>(N, H(I, 0), I->ypred.p, I->ewt.p, I->G.p, c->cap.p, cj,
                                              (float)std::log10(h), k / 3.f, I->stat.p, x);
         FVP_HIP(hipGetLastError());
-        if (I->mp) {
+        if (learn) {
             // learned candidate and its first Newton residual
             if (fvp_mpnn_forward(I->mp, x, I->ylearn.p)) return -1;
             if (c->ndir) zero_rows_f<<<(c->ndir + 255) / 256, 256, 0, s>>>(c->ndir, c->dnode.p, I->ylearn.p);
@@ -329,7 +333,7 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
         I->steps_since_setup = 0;
         I->ss = 20.;
     }
-    if (I->mp) {
+    if (learn) {
         // judge the learned candidate against the predictor by the size of the
         // Newton step each would start with, the preconditioned residual
         // P G (P ~ J^-1) in the error-weighted RMS norm; without a
@@ -346,6 +350,7 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
         ++I->st.learned_tried;
         if (g1 < g0) {
             ++I->st.learned_used;
+            I->learn_took = true;
             std::swap(I->y, I->y2);
             std::swap(I->ypn, I->ypn2);
             std::swap(I->G, I->G2);
@@ -472,6 +477,7 @@ static int one_step(fvp_integrator *I, double t_stop) {
         if (rc < 0) return -1;
         if (rc == 0) {
             if (accept(I, h)) return -1;
+            I->learn_block = false;
             // order and step selection
             int knew = k;
             double enew = ek;
@@ -495,6 +501,9 @@ static int one_step(fvp_integrator *I, double t_stop) {
             if (fails >= 3) { I->k = 1; I->steps_at_k = 0; }
         } else {
             ++I->st.newton_fails;
+            // fallback: a step whose Newton iteration failed from the network's
+            // start is retried from the predictor
+            if (I->learn_took) I->learn_block = true;
             I->h = h * 0.25;
             I->need_setup = true;
         }

@@ -538,14 +538,22 @@ __device__ __forceinline__ void bj_sweep(T *v, int l0, int l1, const int *__rest
                                          const T *__restrict__ dinv, const unsigned short *__restrict__ ja,
                                          const T *__restrict__ lu) {
     const int t = threadIdx.x;
+    if (l0 >= l1) return;
     BjSlot<T> ring[kBjAhead];
     bool have[kBjAhead];
+    // Every lane loads a slot for every level, unconditionally (lanes past the
+    // level's width, and levels past the last, load a valid slot they do not
+    // use): with no branch around the loads the compiler can count the
+    // outstanding loads exactly and wait only for the slot it consumes,
+    // instead of for all of them.
+    auto fetch = [&](int l, BjSlot<T> &w) {
+        const int lc = min(l, l1 - 1);
+        const int p0 = lptr[lc], cnt = lptr[lc + 1] - p0;
+        bj_slot_load<T, UPPER>(w, p0 + min(t, cnt - 1), rows, col, val, k0, kn, dinv);
+        return l < l1 && t < cnt;
+    };
 #pragma unroll
-    for (int d = 0; d < kBjAhead; ++d) {
-        const int l = l0 + d;
-        have[d] = l < l1 && t < lptr[l + 1] - lptr[l];
-        if (have[d]) bj_slot_load<T, UPPER>(ring[d], lptr[l] + t, rows, col, val, k0, kn, dinv);
-    }
+    for (int d = 0; d < kBjAhead; ++d) have[d] = fetch(l0 + d, ring[d]);
     for (int lb = l0; lb < l1; lb += kBjAhead) {
 #pragma unroll
         for (int d = 0; d < kBjAhead; ++d) {
@@ -558,9 +566,7 @@ __device__ __forceinline__ void bj_sweep(T *v, int l0, int l1, const int *__rest
                     bj_slot_load<T, UPPER>(w, p0 + tt, rows, col, val, k0, kn, dinv);
                     bj_slot_update<T, UPPER>(w, v, ja, lu);
                 }
-                const int ln = l + kBjAhead;  // refill this ring slot
-                have[d] = ln < l1 && t < lptr[ln + 1] - lptr[ln];
-                if (have[d]) bj_slot_load<T, UPPER>(ring[d], lptr[ln] + t, rows, col, val, k0, kn, dinv);
+                have[d] = fetch(l + kBjAhead, ring[d]);  // refill this ring slot
                 // one wavefront: the LDS executes its accesses in order, so only
                 // the compiler must keep them in program order
                 wave_lds_order();
