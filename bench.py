@@ -57,7 +57,8 @@ def parse():
     ap.add_argument("--atol", type=float, default=1e-6)
     ap.add_argument("--precon", default="bjilu0", choices=["bjilu0", "ilu0", "jacobi"],
                     help="preconditioner of the plain FP64 run")
-    ap.add_argument("--block-rows", type=int, default=0, help="block-Jacobi rows per block (0 = library default)")
+    ap.add_argument("--block-rows", type=int, default=1024,
+                    help="block-Jacobi rows per block (0 = library default; 1024 measured fastest on this box)")
     ap.add_argument("--overlap", type=int, default=1, help="block-Jacobi overlap layers (restricted additive Schwarz)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--launch", default="auto",
