@@ -475,6 +475,12 @@ __global__ void __launch_bounds__(kBjFactorThreads) bj_factor(
 // sequential solve does.  T is the factor's arithmetic type (double, or float
 // for the FP32 preconditioner); x and y stay FP64.
 constexpr int kBjRow = 12;
+#ifndef BJ_AHEAD_F32
+#define BJ_AHEAD_F32 8
+#endif
+#ifndef BJ_AHEAD_F64
+#define BJ_AHEAD_F64 4
+#endif
 
 // After a factorisation: copy the first kBjRow entries of every slot's row part
 // (zero beyond the row's count) and, for U, the row's inverted diagonal into
@@ -528,7 +534,10 @@ __device__ __forceinline__ void bj_slot_update(const BjSlot<T> &w, T *v, const u
 // One triangle of one block: thread t takes the t-th slot of each level; the
 // slots of the next kBjAhead levels are loaded ahead (their addresses do not
 // depend on the solution), so the level chain waits only on LDS.
-constexpr int kBjAhead = 4;
+// levels loaded ahead: deep enough to cover a memory round trip with the short
+// per-level arithmetic; FP64 slots are larger, so fewer of them fit the registers
+template <typename T>
+constexpr int bj_ahead() { return sizeof(T) == 4 ? BJ_AHEAD_F32 : BJ_AHEAD_F64; }
 
 template <typename T, bool UPPER>
 __device__ __forceinline__ void bj_sweep(T *v, int l0, int l1, const int *__restrict__ lptr,
@@ -539,6 +548,7 @@ __device__ __forceinline__ void bj_sweep(T *v, int l0, int l1, const int *__rest
                                          const T *__restrict__ lu) {
     const int t = threadIdx.x;
     if (l0 >= l1) return;
+    constexpr int kBjAhead = bj_ahead<T>();
     BjSlot<T> ring[kBjAhead];
     bool have[kBjAhead];
     // Every lane loads a slot for every level, unconditionally (lanes past the
