@@ -57,14 +57,18 @@ template <bool JV>
 __global__ void __launch_bounds__(256) vg_kernel(int N, ZoneTable zt, double rho0, const double *__restrict__ psi,
                                                  const uint8_t *__restrict__ zmask, const double *__restrict__ zw,
                                                  double *__restrict__ kr, double *__restrict__ cap,
-                                                 const double *__restrict__ v, const double *__restrict__ sig) {
+                                                 const double *__restrict__ v, const double *__restrict__ sig,
+                                                 double *__restrict__ hjv) {
     __shared__ double zp[kMaxZones * 10];
     if (threadIdx.x < 10 * zt.nz) zp[threadIdx.x] = zt.p[threadIdx.x / 10][threadIdx.x % 10];
     __syncthreads();
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     double h = psi[i];
-    if (JV) h = psi[i] + sig[0] * v[i];
+    if (JV) {
+        h = psi[i] + sig[0] * v[i];
+        hjv[i] = h;  // the perturbed head, gathered by the element kernel
+    }
     unsigned m = zmask[i];
     double phi = 0., Sw = 0., dSw = 0., Ss = 0.;
     while (m) {
@@ -307,7 +311,9 @@ __global__ void __launch_bounds__(512) owner_kernel(
     extern __shared__ double sm[];
     const int tile = per_xcd ? (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3) : blockIdx.x;
     if (tile >= ntiles) return;  // uniform for the whole workgroup
-    const double sig = JV ? jv.sig[0] : 0.;
+    // J v: psi is the perturbed head psi + sigma v written by the van Genuchten
+    // pass; the difference quotient is formed by the finalisation
+    (void)jv;
     const int lo = tile * tile_nodes, hi = min(N, lo + tile_nodes), nown = hi - lo;
     for (int i = threadIdx.x; i < nown; i += blockDim.x) sm[i] = 0.;
     __shared__ double zk[3 * kMaxZones];
@@ -326,7 +332,7 @@ __global__ void __launch_bounds__(512) owner_kernel(
         const int last = e1 - 1;
         ElemRec cur = load_elem<0>(e, conn, ezone, kmul, emask, nullptr);
         NodeVals<DIM> vc;
-        load_nodes<DIM, JV>(cur, N, xyz, psi, jv.v, sig, kr, vc);
+        load_nodes<DIM, false>(cur, N, xyz, psi, nullptr, 0., kr, vc);
         int dc[4];
         owner_exports<DIM>(cur, lo, hi, xd, xoff[e], dc);
         int en = min(e + bs, last);
@@ -335,7 +341,7 @@ __global__ void __launch_bounds__(512) owner_kernel(
         for (; e < e1; e += bs) {
             NodeVals<DIM> vn;
             int dn[4];
-            load_nodes<DIM, JV>(nxt, N, xyz, psi, jv.v, sig, kr, vn);
+            load_nodes<DIM, false>(nxt, N, xyz, psi, nullptr, 0., kr, vn);
             owner_exports<DIM>(nxt, lo, hi, xd, xn, dn);
             const int e2 = min(e + 2 * bs, last);
             const ElemRec nx2 = load_elem<0>(e2, conn, ezone, kmul, emask, nullptr);
@@ -701,7 +707,8 @@ static void launch_tile(fvp_ctx *c, const double *psi, const double *psip, doubl
                               sizeof(double) * c->tile_nodes, s, ea, eb, 0, N, c->ntiles, c->tile_nodes,
                               c->xcd_map ? per : 0, c->zt, c->rho0, (const int4 *)c->conn.p,
                               (const uint8_t *)c->ezone.p, (const double *)c->kmul.p, (const uint8_t *)c->emask.p,
-                              (const double *)c->xyz.p, psi, (const double *)c->kr.p, (const int *)c->t_eptr.p,
+                              (const double *)c->xyz.p, JV ? (const double *)c->hjv.p : psi,
+                              (const double *)c->kr.p, (const int *)c->t_eptr.p,
                               (const int *)c->t_bptr.p, (const int *)c->bnode.p, (const double *)c->bval.p,
                               (const unsigned short *)c->t_xoff.p, (const int *)c->t_xptr.p,
                               (const int *)c->x_ilist.p, c->xf.p, c->acc.p, jv);
@@ -734,10 +741,10 @@ int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, d
     prof_mark(c, 1, 0);
     if (jv)
         vg_kernel<true><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p,
-                                                        c->cap.p, jv->v, jv->sig);
+                                                        c->cap.p, jv->v, jv->sig, c->hjv.p);
     else
         vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p,
-                                                         c->cap.p, nullptr, nullptr);
+                                                         c->cap.p, nullptr, nullptr, nullptr);
     if (!averaging) {
         // tiled paths (element fluxes with the boundary fluxes, then the finalisation)
         const JvArgs none;
@@ -853,7 +860,7 @@ extern "C" int fvp_begin_step(fvp_ctx *c, double t, const double *psi, const dou
     const int N = c->N, T = c->T;
     // k_r at the state, then the averaged-weight face fluxes summed per edge
     vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p, c->cap.p,
-                                                     nullptr, nullptr);
+                                                     nullptr, nullptr, nullptr);
     if (c->dim == 3)
         elem_kernel<3, 2><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
                                                        c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p, nullptr,
