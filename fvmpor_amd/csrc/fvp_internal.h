@@ -63,7 +63,7 @@ struct fvp_ctx {
     fvp::DevBuf<double> zw;      // nz x N SCV-volume fraction of zone z in CV i
     fvp::DevBuf<double> kr;      // nz x N relative permeability per (zone, node)
     fvp::DevBuf<double> cap;     // N capacity coefficient ahh
-    fvp::DevBuf<double> hjv;     // N perturbed heads psi + sigma v of the last J v
+    fvp::DevBuf<double> nrec;    // N x 4 node records x, y, z, head (psi + sigma v in J v) of the last residual
     fvp::DevBuf<double> acc;     // N flux accumulator
     // boundary flux: nodes with a flux condition and their rho0 * sum(q.n A)
     int nb = 0;

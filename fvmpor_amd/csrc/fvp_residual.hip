@@ -58,16 +58,18 @@ __global__ void __launch_bounds__(256) vg_kernel(int N, ZoneTable zt, double rho
                                                  const uint8_t *__restrict__ zmask, const double *__restrict__ zw,
                                                  double *__restrict__ kr, double *__restrict__ cap,
                                                  const double *__restrict__ v, const double *__restrict__ sig,
-                                                 double *__restrict__ hjv) {
+                                                 double *__restrict__ nrec, const double *__restrict__ xyz) {
     __shared__ double zp[kMaxZones * 10];
     if (threadIdx.x < 10 * zt.nz) zp[threadIdx.x] = zt.p[threadIdx.x / 10][threadIdx.x % 10];
     __syncthreads();
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     double h = psi[i];
-    if (JV) {
-        h = psi[i] + sig[0] * v[i];
-        hjv[i] = h;  // the perturbed head, gathered by the element kernel
+    if (JV) h = psi[i] + sig[0] * v[i];
+    if (nrec) {  // the node record the element kernel gathers: x, y, z and the (perturbed) head
+        double2 *r = reinterpret_cast<double2 *>(nrec + 4 * (size_t)i);
+        r[0] = make_double2(xyz[3 * i], xyz[3 * i + 1]);
+        r[1] = make_double2(xyz[3 * i + 2], h);
     }
     unsigned m = zmask[i];
     double phi = 0., Sw = 0., dSw = 0., Ss = 0.;
@@ -151,6 +153,21 @@ __device__ __forceinline__ void load_nodes(const ElemRec &rec, int N, const doub
     for (int k = 0; k < DIM + 1; ++k) {
         load_pt(xyz, n[k], nv.px[k], nv.py[k], nv.pz[k]);
         nv.h[k] = JV ? psi[n[k]] + sig * v[n[k]] : psi[n[k]];
+        nv.k[k] = krz[n[k]];
+    }
+}
+
+// load_nodes from the packed node records (x, y, z, head; two 16-byte loads per node)
+template <int DIM>
+__device__ __forceinline__ void load_nodes_rec(const ElemRec &rec, int N, const double *__restrict__ nrec,
+                                               const double *__restrict__ kr, NodeVals<DIM> &nv) {
+    const int n[4] = {rec.c4.x, rec.c4.y, rec.c4.z, rec.c4.w};
+    const double *krz = kr + (size_t)rec.z * N;
+#pragma unroll
+    for (int k = 0; k < DIM + 1; ++k) {
+        const double2 *r = reinterpret_cast<const double2 *>(nrec + 4 * (size_t)n[k]);
+        const double2 a = r[0], b = r[1];
+        nv.px[k] = a.x; nv.py[k] = a.y; nv.pz[k] = b.x; nv.h[k] = b.y;
         nv.k[k] = krz[n[k]];
     }
 }
@@ -304,16 +321,18 @@ template <int DIM, bool JV>
 __global__ void __launch_bounds__(512) owner_kernel(
     int N, int ntiles, int tile_nodes, int per_xcd, ZoneTable zt, double rho0, const int4 *__restrict__ conn,
     const uint8_t *__restrict__ ezone, const double *__restrict__ kmul, const uint8_t *__restrict__ emask,
-    const double *__restrict__ xyz, const double *__restrict__ psi, const double *__restrict__ kr,
+    const double *__restrict__ xyz, const double *__restrict__ nrec, const double *__restrict__ kr,
     const int *__restrict__ eptr, const int *__restrict__ bptr, const int *__restrict__ bnode,
     const double *__restrict__ bval, const unsigned short *__restrict__ xoff, const int *__restrict__ xptr,
     const int *__restrict__ xdst, double *__restrict__ xf, double *__restrict__ acc_g, JvArgs jv) {
     extern __shared__ double sm[];
     const int tile = per_xcd ? (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3) : blockIdx.x;
     if (tile >= ntiles) return;  // uniform for the whole workgroup
-    // J v: psi is the perturbed head psi + sigma v written by the van Genuchten
-    // pass; the difference quotient is formed by the finalisation
+    // nrec: the packed node records (x, y, z, head) written by the van Genuchten
+    // pass, whose head is psi + sigma v for J v; the difference quotient is
+    // formed by the finalisation (xyz is read through the records)
     (void)jv;
+    (void)xyz;
     const int lo = tile * tile_nodes, hi = min(N, lo + tile_nodes), nown = hi - lo;
     for (int i = threadIdx.x; i < nown; i += blockDim.x) sm[i] = 0.;
     __shared__ double zk[3 * kMaxZones];
@@ -332,7 +351,7 @@ __global__ void __launch_bounds__(512) owner_kernel(
         const int last = e1 - 1;
         ElemRec cur = load_elem<0>(e, conn, ezone, kmul, emask, nullptr);
         NodeVals<DIM> vc;
-        load_nodes<DIM, false>(cur, N, xyz, psi, nullptr, 0., kr, vc);
+        load_nodes_rec<DIM>(cur, N, nrec, kr, vc);
         int dc[4];
         owner_exports<DIM>(cur, lo, hi, xd, xoff[e], dc);
         int en = min(e + bs, last);
@@ -341,7 +360,7 @@ __global__ void __launch_bounds__(512) owner_kernel(
         for (; e < e1; e += bs) {
             NodeVals<DIM> vn;
             int dn[4];
-            load_nodes<DIM, false>(nxt, N, xyz, psi, nullptr, 0., kr, vn);
+            load_nodes_rec<DIM>(nxt, N, nrec, kr, vn);
             owner_exports<DIM>(nxt, lo, hi, xd, xn, dn);
             const int e2 = min(e + 2 * bs, last);
             const ElemRec nx2 = load_elem<0>(e2, conn, ezone, kmul, emask, nullptr);
@@ -707,7 +726,7 @@ static void launch_tile(fvp_ctx *c, const double *psi, const double *psip, doubl
                               sizeof(double) * c->tile_nodes, s, ea, eb, 0, N, c->ntiles, c->tile_nodes,
                               c->xcd_map ? per : 0, c->zt, c->rho0, (const int4 *)c->conn.p,
                               (const uint8_t *)c->ezone.p, (const double *)c->kmul.p, (const uint8_t *)c->emask.p,
-                              (const double *)c->xyz.p, JV ? (const double *)c->hjv.p : psi,
+                              (const double *)c->xyz.p, (const double *)c->nrec.p,
                               (const double *)c->kr.p, (const int *)c->t_eptr.p,
                               (const int *)c->t_bptr.p, (const int *)c->bnode.p, (const double *)c->bval.p,
                               (const unsigned short *)c->t_xoff.p, (const int *)c->t_xptr.p,
@@ -741,10 +760,10 @@ int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, d
     prof_mark(c, 1, 0);
     if (jv)
         vg_kernel<true><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p,
-                                                        c->cap.p, jv->v, jv->sig, c->hjv.p);
+                                                        c->cap.p, jv->v, jv->sig, c->nrec.p, c->xyz.p);
     else
         vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p,
-                                                         c->cap.p, nullptr, nullptr, nullptr);
+                                                         c->cap.p, nullptr, nullptr, c->rvariant == FVP_RESIDUAL_OWNER ? c->nrec.p : nullptr, c->xyz.p);
     if (!averaging) {
         // tiled paths (element fluxes with the boundary fluxes, then the finalisation)
         const JvArgs none;
@@ -860,7 +879,7 @@ extern "C" int fvp_begin_step(fvp_ctx *c, double t, const double *psi, const dou
     const int N = c->N, T = c->T;
     // k_r at the state, then the averaged-weight face fluxes summed per edge
     vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p, c->cap.p,
-                                                     nullptr, nullptr, nullptr);
+                                                     nullptr, nullptr, nullptr, nullptr);
     if (c->dim == 3)
         elem_kernel<3, 2><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
                                                        c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p, nullptr,

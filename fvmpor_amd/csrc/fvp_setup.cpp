@@ -374,7 +374,7 @@ extern "C" int fvp_create(const fvp_problem *p, int device, fvp_ctx **out) {
     if (c->invV.upload(invV.data(), N) || c->zw.upload(zw.data(), zw.size()) ||
         c->zmask.upload(zmask.data(), N))
         return fail();
-    if (c->kr.alloc((size_t)c->nz * N) || c->cap.alloc(N) || c->hjv.alloc(N) || c->acc.alloc(N)) return fail();
+    if (c->kr.alloc((size_t)c->nz * N) || c->cap.alloc(N) || c->nrec.alloc(4 * (size_t)N) || c->acc.alloc(N)) return fail();
     FVP_HIP(hipMemset(c->kr.p, 0, sizeof(double) * c->kr.n));
 
     // ---- edges: sort (min, max, insertion index); first insertion fixes front/back
@@ -575,7 +575,7 @@ extern "C" int fvp_destroy(fvp_ctx *c) {
     prof_release(c);
     c->conn.release(); c->ezone.release(); c->emask.release(); c->eorient.release(); c->kmul.release();
     c->xyz.release(); c->invV.release(); c->zmask.release(); c->zw.release(); c->kr.release();
-    c->cap.release(); c->hjv.release(); c->acc.release(); c->bnode.release(); c->bval.release(); c->dnode.release();
+    c->cap.release(); c->nrec.release(); c->acc.release(); c->bnode.release(); c->bval.release(); c->dnode.release();
     c->dval.release(); c->t_wptr.release(); c->t_whalo.release(); c->t_vptr.release(); c->t_lconn.release();
     c->t_tz.release(); c->t_eptr.release(); c->t_hptr.release(); c->t_hlist.release(); c->t_bptr.release();
     c->t_dptr.release(); c->t_xoff.release(); c->t_xptr.release(); c->x_iptr.release(); c->x_ilist.release();
