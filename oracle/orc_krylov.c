@@ -142,10 +142,10 @@ void orc_gmres(int n, OpFn A, void *actx, OpFn M, void *mctx, const double *b, d
     double *H = (double *)calloc((size_t)(m + 1) * m, sizeof(double)); /* column-major (m+1) x m */
     double *Ht = (double *)calloc((size_t)(m + 1) * m, sizeof(double));
     double *y = (double *)calloc(m + 1, sizeof(double));
-    double *r = (double *)malloc(sizeof(double) * n);
-    double *w = (double *)malloc(sizeof(double) * n);
-    double *dx = (double *)malloc(sizeof(double) * n);
-    double *wtmp = (double *)malloc(sizeof(double) * n);
+    double *r = (double *)calloc(n, sizeof(double));
+    double *w = (double *)calloc(n, sizeof(double));
+    double *dx = (double *)calloc(n, sizeof(double));
+    double *wtmp = (double *)calloc(n, sizeof(double));
     int restarts = 0, converged = 0, total = 0;
 
     /* r = b - A x */
@@ -261,12 +261,12 @@ int orc_gmres_csr(int n, const int *ia, const int *ja, const double *a, int prec
     PcOp P = {n, precon_type, ia, ja, NULL, NULL, NULL};
     int ok = 1;
     if (precon_type == 5) {
-        P.lu = (double *)malloc(sizeof(double) * ia[n]);
+        P.lu = (double *)calloc(ia[n], sizeof(double));
         memcpy(P.lu, a, sizeof(double) * ia[n]);
         P.diagpos = (int *)malloc(sizeof(int) * n);
         ok = orc_ilu0_factor(n, ia, ja, P.lu, P.diagpos);
     } else if (precon_type == 2) {
-        P.J = (double *)malloc(sizeof(double) * n);
+        P.J = (double *)calloc(n, sizeof(double));
         ok = orc_jacobi_factor(n, ia, ja, a, P.J);
     }
     if (ok) {
@@ -288,7 +288,7 @@ int orc_arnoldi_csr(int n, const int *ia, const int *ja, const double *a, const 
     memset(Q, 0, sizeof(double) * (size_t)n * (m + 1));
     memset(H, 0, sizeof(double) * (size_t)(m + 1) * m);
     for (int i = 0; i < n; ++i) Q[i] = b[i] / beta;
-    double *w = (double *)malloc(sizeof(double) * n);
+    double *w = (double *)calloc(n, sizeof(double));
     int j = 0;
     while (j < m) {
         csr_op(&A, Q + (size_t)n * j, w);
@@ -376,10 +376,10 @@ void orc_fd_jacobian(OrcProblem *p, int N, double t, double c, double h, const d
                      const double *F0, const double *ewt, const int *ia, const int *ja, const int *colour,
                      int ncol, double *vals) {
     double eps = sqrt(2.220446049250313e-16);
-    double *shift = (double *)malloc(sizeof(double) * N);
-    double *ys = (double *)malloc(sizeof(double) * N);
-    double *yps = (double *)malloc(sizeof(double) * N);
-    double *Fs = (double *)malloc(sizeof(double) * N);
+    double *shift = (double *)calloc(N, sizeof(double));
+    double *ys = (double *)calloc(N, sizeof(double));
+    double *yps = (double *)calloc(N, sizeof(double));
+    double *Fs = (double *)calloc(N, sizeof(double));
     for (int j = 0; j < N; ++j) {
         double a = fabs(y[j]), b = fabs(h * yp[j]), w = 1.0 / ewt[j];
         shift[j] = eps * fmax(a, fmax(b, w));
@@ -425,9 +425,9 @@ static void jv_op(void *ctx, const double *v, double *out) {
 void orc_jv(OrcProblem *p, int N, double t, const double *y, const double *yp, const double *F0, double c,
             const double *v, double *out) {
     JvOp o = {p, N, t, c, y, yp, F0, NULL, NULL, NULL};
-    o.ys = (double *)malloc(sizeof(double) * N);
-    o.yps = (double *)malloc(sizeof(double) * N);
-    o.F1 = (double *)malloc(sizeof(double) * N);
+    o.ys = (double *)calloc(N, sizeof(double));
+    o.yps = (double *)calloc(N, sizeof(double));
+    o.F1 = (double *)calloc(N, sizeof(double));
     jv_op(&o, v, out);
     free(o.ys); free(o.yps); free(o.F1);
 }
@@ -438,18 +438,18 @@ int orc_newton_gmres(OrcProblem *p, int N, double t, const double *y, const doub
                      const int *ia, const int *ja, const double *jvals, int precon_type, const double *b, double *x,
                      const int *iparams, double tol, double *stats) {
     JvOp o = {p, N, t, c, y, yp, F0, NULL, NULL, NULL};
-    o.ys = (double *)malloc(sizeof(double) * N);
-    o.yps = (double *)malloc(sizeof(double) * N);
-    o.F1 = (double *)malloc(sizeof(double) * N);
+    o.ys = (double *)calloc(N, sizeof(double));
+    o.yps = (double *)calloc(N, sizeof(double));
+    o.F1 = (double *)calloc(N, sizeof(double));
     PcOp P = {N, precon_type, ia, ja, NULL, NULL, NULL};
     int ok = 1;
     if (precon_type == 5) {
-        P.lu = (double *)malloc(sizeof(double) * ia[N]);
+        P.lu = (double *)calloc(ia[N], sizeof(double));
         memcpy(P.lu, jvals, sizeof(double) * ia[N]);
         P.diagpos = (int *)malloc(sizeof(int) * N);
         ok = orc_ilu0_factor(N, ia, ja, P.lu, P.diagpos);
     } else if (precon_type == 2) {
-        P.J = (double *)malloc(sizeof(double) * N);
+        P.J = (double *)calloc(N, sizeof(double));
         ok = orc_jacobi_factor(N, ia, ja, jvals, P.J);
     }
     if (ok) {
