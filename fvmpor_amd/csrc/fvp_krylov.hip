@@ -9,15 +9,18 @@
 //    Gram-Schmidt fused into one multi-dot and one multi-axpy pass per sweep
 //    (applied twice when re-orthogonalising), so each Arnoldi step reads the
 //    basis twice instead of 2(j+1) times as modified Gram-Schmidt would.  The
-//    (m+1) x m Hessenberg system is reduced with Givens rotations on the host.
+//    (m+1) x m Hessenberg system is reduced with Givens rotations on the
+//    device; the host reads each step's residual estimate one step late.
 //  * Coloured finite-difference Jacobian (preconditioner_dss.cpp:284-372): one
 //    residual per colour of a distance-2 colouring.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <functional>
 #include <set>
+#include <utility>
 #include <vector>
 
 #include "fvp_blas.h"
@@ -73,7 +76,6 @@ struct fvp_csr {
 
 namespace fvp {
 
-constexpr int kBlocks = 512;
 constexpr int kThreads = 256;
 
 // ---- kernels ------------------------------------------------------------------
@@ -202,64 +204,128 @@ __device__ __forceinline__ double wave_reduce(double v) {
     return v;
 }
 
-// Block partials of w . Q[:, c] for c < nc (nc <= kChunk), and of w . w (stored
-// as column nc) when ww is set: part[c * gridDim.x + block].  Skipped entirely
-// when skip points at 0 (the second Gram-Schmidt pass was not needed).
-__global__ void __launch_bounds__(kThreads) mdot_kernel(long n, const double *__restrict__ w,
-                                                        const double *__restrict__ Q, long ldq, int nc, int ww,
-                                                        double *__restrict__ part, const double *__restrict__ skip) {
-    if (skip && skip[0] == 0.) return;
-    __shared__ double lds[kChunk + 1][kThreads / 64];
-    double acc[kChunk + 1];
-#pragma unroll
-    for (int c = 0; c <= kChunk; ++c) acc[c] = 0.;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const double wi = w[i];
-#pragma unroll
-        for (int c = 0; c < kChunk; ++c)
-            if (c < nc) acc[c] += wi * Q[c * ldq + i];
-        acc[kChunk] += wi * wi;
-    }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int c = 0; c <= kChunk; ++c) {
-        if (c < nc || (c == kChunk && ww)) {
-            const double v = wave_reduce(acc[c]);
-            if (lane == 0) lds[c][wid] = v;
-        }
-    }
-    __syncthreads();
-    const int c = threadIdx.x;
-    if (c < nc || (c == kChunk && ww)) {
-        double v = 0.;
-        for (int k = 0; k < kThreads / 64; ++k) v += lds[c][k];
-        part[(c < nc ? c : nc) * gridDim.x + blockIdx.x] = v;
-    }
+// Workgroups of the Gram-Schmidt passes: 2048 x 256 threads keep enough
+// 16-byte loads in flight to stream the basis at HBM rate (8 per CU).
+constexpr int kGsBlocks = 2048;
+
+// Leading dimension of the basis: columns padded to an even length, so that
+// every column starts 16-byte aligned for the pairwise loads.
+static inline long gs_ldq(long n) { return (n + 1) & ~1L; }
+static inline int gs_blocks(long n) {
+    return (int)std::max(1L, std::min((long)kGsBlocks, ((n >> 1) + kThreads - 1) / kThreads));
 }
 
-// w -= Q[:, 0..nc) coef, and block partials of the updated w . w into part[block]
-__global__ void __launch_bounds__(kThreads) axpy_norm(long n, double *__restrict__ w, const double *__restrict__ Q,
-                                                      long ldq, int nc, const double *__restrict__ coef,
-                                                      double *__restrict__ part, const double *__restrict__ skip) {
-    if (skip && skip[0] == 0.) return;
+__device__ __forceinline__ void block_partial(double v, double *__restrict__ out) {
     __shared__ double lds[kThreads / 64];
-    double acc = 0.;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        double s = 0.;
-        for (int c = 0; c < nc; ++c) s += coef[c] * Q[c * ldq + i];
-        const double wi = w[i] - s;
-        w[i] = wi;
-        acc += wi * wi;
-    }
-    const double v = wave_reduce(acc);
+    v = wave_reduce(v);
     if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = 0.;
         for (int k = 0; k < kThreads / 64; ++k) t += lds[k];
-        part[blockIdx.x] = t;
+        *out = t;
     }
 }
+
+// Block partials of w . Q[:, c] for c < NC and of w . w (stored as column NC
+// when ww is set): part[c * gridDim.x + block].  Each thread reads element
+// pairs (16-byte loads; the basis columns are padded to an even length, ldq),
+// every column load of a pair issued before the first product.  Skipped
+// entirely when skip[0] == 0 (second pass not needed).
+template <int NC>
+__global__ void __launch_bounds__(kThreads) mdot_kernel(long n, const double *__restrict__ w,
+                                                        const double *__restrict__ Q, long ldq, int ww,
+                                                        double *__restrict__ part, const double *__restrict__ skip) {
+    if (skip && skip[0] == 0.) return;
+    double acc[NC + 1];
+#pragma unroll
+    for (int c = 0; c <= NC; ++c) acc[c] = 0.;
+    const long np = n >> 1;
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+        const double2 a = reinterpret_cast<const double2 *>(w)[p];
+        double2 q[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) q[c] = reinterpret_cast<const double2 *>(Q + c * ldq)[p];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c] += a.x * q[c].x + a.y * q[c].y;
+        acc[NC] += a.x * a.x + a.y * a.y;
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const double a = w[n - 1];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c] += a * Q[c * ldq + n - 1];
+        acc[NC] += a * a;
+    }
+#pragma unroll
+    for (int c = 0; c <= NC; ++c) {
+        if (c < NC || ww) {
+            block_partial(acc[c], part + c * gridDim.x + blockIdx.x);
+            __syncthreads();
+        }
+    }
+}
+
+// w -= Q[:, 0..NC) coef, and block partials of the updated w . w into
+// part[block]; element pairs as in mdot_kernel.
+template <int NC>
+__global__ void __launch_bounds__(kThreads) axpy_norm(long n, double *__restrict__ w, const double *__restrict__ Q,
+                                                      long ldq, const double *__restrict__ coef,
+                                                      double *__restrict__ part, const double *__restrict__ skip) {
+    if (skip && skip[0] == 0.) return;
+    double h[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) h[c] = coef[c];
+    double acc = 0.;
+    const long np = n >> 1;
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+        double2 q[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) q[c] = reinterpret_cast<const double2 *>(Q + c * ldq)[p];
+        double2 a = reinterpret_cast<double2 *>(w)[p];
+        double sx = 0., sy = 0.;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) { sx += h[c] * q[c].x; sy += h[c] * q[c].y; }
+        a.x -= sx;
+        a.y -= sy;
+        reinterpret_cast<double2 *>(w)[p] = a;
+        acc += a.x * a.x + a.y * a.y;
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        double sx = 0.;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) sx += h[c] * Q[c * ldq + n - 1];
+        const double a = w[n - 1] - sx;
+        w[n - 1] = a;
+        acc += a * a;
+    }
+    block_partial(acc, part + blockIdx.x);
+}
+
+using MdotLaunch = void (*)(hipStream_t, int, long, const double *, const double *, long, int, double *,
+                           const double *);
+using AxpyLaunch = void (*)(hipStream_t, int, long, double *, const double *, long, const double *, double *,
+                           const double *);
+template <int NC>
+static void mdot_launch(hipStream_t s, int nb, long n, const double *w, const double *Q, long ldq, int ww,
+                        double *part, const double *skip) {
+    mdot_kernel<NC><<<nb, kThreads, 0, s>>>(n, w, Q, ldq, ww, part, skip);
+}
+template <int NC>
+static void axpy_launch(hipStream_t s, int nb, long n, double *w, const double *Q, long ldq, const double *coef,
+                        double *part, const double *skip) {
+    axpy_norm<NC><<<nb, kThreads, 0, s>>>(n, w, Q, ldq, coef, part, skip);
+}
+template <int... I>
+static constexpr std::array<MdotLaunch, sizeof...(I)> mdot_table(std::integer_sequence<int, I...>) {
+    return {&mdot_launch<I + 1>...};
+}
+template <int... I>
+static constexpr std::array<AxpyLaunch, sizeof...(I)> axpy_table(std::integer_sequence<int, I...>) {
+    return {&axpy_launch<I + 1>...};
+}
+// per column count 1..kChunk
+static const auto kMdot = mdot_table(std::make_integer_sequence<int, kChunk>{});
+static const auto kAxpy = axpy_table(std::make_integer_sequence<int, kChunk>{});
 
 // Selective reorthogonalisation (Kahan-Parlett "twice is enough"): a second
 // classical Gram-Schmidt pass when the first one cancelled more than half of
@@ -623,8 +689,8 @@ static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 static int ensure_work(fvp_csr *A, int m) {
     if (A->m_alloc >= m) return 0;
     long n = A->n;
-    if (A->Q.alloc((size_t)n * (m + 1)) || A->w.alloc(n) || A->r.alloc(n) || A->z.alloc(n) ||
-        A->part.alloc((size_t)kBlocks * (kChunk + 1)) || A->scal.alloc(16) || A->hdev.alloc(2 * (m + 2) + 2) ||
+    if (A->Q.alloc((size_t)gs_ldq(n) * (m + 1)) || A->w.alloc(n) || A->r.alloc(n) || A->z.alloc(n) ||
+        A->part.alloc((size_t)kGsBlocks * (kChunk + 1)) || A->scal.alloc(16) || A->hdev.alloc(2 * (m + 2) + 2) ||
         A->Hd.alloc((size_t)(m + 1) * m) || A->gd.alloc(m + 1) || A->csd.alloc(m) || A->snd.alloc(m) ||
         A->yd.alloc(m))
         return -1;
@@ -648,12 +714,24 @@ static int ensure_work(fvp_csr *A, int m) {
 // columns; the w . w term rides on the last chunk.  skip: see mdot_kernel.
 static void multidot(hipStream_t s, long n, const double *w, const double *Q, long ldq, int nc, double *part,
                      double *out, const double *skip) {
+    const int nb = gs_blocks(n);
     for (int c0 = 0; c0 < nc; c0 += kChunk) {
         const int k = std::min(kChunk, nc - c0);
         const int last = c0 + k == nc;
-        mdot_kernel<<<kBlocks, kThreads, 0, s>>>(n, w, Q + c0 * ldq, ldq, k, last, part, skip);
-        reduce_cols<<<k + last, kThreads, 0, s>>>(kBlocks, part, out + c0, 0, skip);
+        kMdot[k - 1](s, nb, n, w, Q + c0 * ldq, ldq, last, part, skip);
+        reduce_cols<<<k + last, kThreads, 0, s>>>(nb, part, out + c0, 0, skip);
     }
+}
+
+// w -= Q[:, 0..nc) coef in chunks of kChunk columns; *norm2 = the updated w . w
+static void multiaxpy(hipStream_t s, long n, double *w, const double *Q, long ldq, int nc, const double *coef,
+                      double *part, double *norm2, const double *skip) {
+    const int nb = gs_blocks(n);
+    for (int c0 = 0; c0 < nc; c0 += kChunk) {
+        const int k = std::min(kChunk, nc - c0);
+        kAxpy[k - 1](s, nb, n, w, Q + c0 * ldq, ldq, coef + c0, part, skip);
+    }
+    reduce_cols<<<1, kThreads, 0, s>>>(nb, part, norm2, 0, skip);
 }
 
 using Op = std::function<int(const double *, double *)>;
@@ -678,7 +756,7 @@ int gmres_device(fvp_csr *W, hipStream_t s, long n, const Op &A, const Op &M, co
     // device scalars: sc[0] ||r||^2, sc[1] ||b||^2, sc[2] first-pass ||w||^2 after projection,
     // sc[3] second-pass ||w||^2, sc[4] reorthogonalisation flag, sc[5] 1 / h_{j+1,j}
     double *n1 = sc + 2, *n2 = sc + 3, *flag = sc + 4, *ninv = sc + 5;
-    const long ldq = n;
+    const long ldq = gs_ldq(n);
     const int mode = p->reorthogonalise;
     fvp_gmres_stats stats{0, 0, 0, 0., 0.};
     double hn2[2] = {0., 0.};
@@ -708,13 +786,11 @@ int gmres_device(fvp_csr *W, hipStream_t s, long n, const Op &A, const Op &M, co
                 else if (A(qj, w)) return -1;
                 // first pass: h1 = Q^T w, ||w||^2 -> h1[j+1]; w -= Q h1, ||w||^2 -> n1
                 multidot(s, n, w, Q, ldq, j + 1, part, h1, nullptr);
-                axpy_norm<<<kBlocks, kThreads, 0, s>>>(n, w, Q, ldq, j + 1, h1, part, nullptr);
-                reduce_cols<<<1, kThreads, 0, s>>>(kBlocks, part, n1, 0, nullptr);
+                multiaxpy(s, n, w, Q, ldq, j + 1, h1, part, n1, nullptr);
                 if (mode) {
                     reorth_decide<<<1, 64, 0, s>>>(h1 + j + 1, n1, mode, flag);
                     multidot(s, n, w, Q, ldq, j + 1, part, h2, flag);
-                    axpy_norm<<<kBlocks, kThreads, 0, s>>>(n, w, Q, ldq, j + 1, h2, part, flag);
-                    reduce_cols<<<1, kThreads, 0, s>>>(kBlocks, part, n2, 0, flag);
+                    multiaxpy(s, n, w, Q, ldq, j + 1, h2, part, n2, flag);
                 }
                 arnoldi_update<<<1, 64, 0, s>>>(j, m, h1, h2, n1, n2, mode ? flag : nullptr, W->Hd.p, W->csd.p,
                                                 W->snd.p, W->gd.p, stat, ninv);

@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(256) vg_kernel(int N, ZoneTable zt, double rho
                                                  const uint8_t *__restrict__ zmask, const double *__restrict__ zw,
                                                  double *__restrict__ kr, double *__restrict__ cap,
                                                  const double *__restrict__ v, const double *__restrict__ sig,
-                                                 double *__restrict__ nrec, const double *__restrict__ xyz) {
+                                                 double *__restrict__ nrec) {
     __shared__ double zp[kMaxZones * 10];
     if (threadIdx.x < 10 * zt.nz) zp[threadIdx.x] = zt.p[threadIdx.x / 10][threadIdx.x % 10];
     __syncthreads();
@@ -66,11 +66,9 @@ __global__ void __launch_bounds__(256) vg_kernel(int N, ZoneTable zt, double rho
     if (i >= N) return;
     double h = psi[i];
     if (JV) h = psi[i] + sig[0] * v[i];
-    if (nrec) {  // the node record the element kernel gathers: x, y, z and the (perturbed) head
-        double2 *r = reinterpret_cast<double2 *>(nrec + 4 * (size_t)i);
-        r[0] = make_double2(xyz[3 * i], xyz[3 * i + 1]);
-        r[1] = make_double2(xyz[3 * i + 2], h);
-    }
+    // the head of the node record the element kernel gathers (x, y, z are
+    // written once at set-up)
+    if (nrec) nrec[4 * (size_t)i + 3] = h;
     unsigned m = zmask[i];
     double phi = 0., Sw = 0., dSw = 0., Ss = 0.;
     while (m) {
@@ -392,7 +390,7 @@ __global__ void __launch_bounds__(512) owner_kernel(
 // node), read coalesced and added to the nodes' own sums in LDS, then
 // F = acc / V - psi' ahh (JV: the difference quotient) and the Dirichlet rows.
 template <bool JV>
-__global__ void __launch_bounds__(256) owner_finalize(
+__global__ void __launch_bounds__(1024) owner_finalize(
     int N, int ntiles, int tile_nodes, const double *__restrict__ acc, const int *__restrict__ iptr,
     const int *__restrict__ xtgt, const double *__restrict__ xf, const double *__restrict__ invV,
     const double *__restrict__ psi, const double *__restrict__ psip, const double *__restrict__ cap,
@@ -731,7 +729,7 @@ static void launch_tile(fvp_ctx *c, const double *psi, const double *psip, doubl
                               (const int *)c->t_bptr.p, (const int *)c->bnode.p, (const double *)c->bval.p,
                               (const unsigned short *)c->t_xoff.p, (const int *)c->t_xptr.p,
                               (const int *)c->x_ilist.p, c->xf.p, c->acc.p, jv);
-        owner_finalize<JV><<<c->ntiles, 256, sizeof(double) * c->tile_nodes, s>>>(
+        owner_finalize<JV><<<c->ntiles, 1024, sizeof(double) * c->tile_nodes, s>>>(
             N, c->ntiles, c->tile_nodes, c->acc.p, c->x_iptr.p, c->x_tgt.p, c->xf.p, c->invV.p, psi, psip, c->cap.p,
             c->t_dptr.p, c->dnode.p, c->dval.p, F, jv);
         return;
@@ -760,10 +758,10 @@ int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, d
     prof_mark(c, 1, 0);
     if (jv)
         vg_kernel<true><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p,
-                                                        c->cap.p, jv->v, jv->sig, c->nrec.p, c->xyz.p);
+                                                        c->cap.p, jv->v, jv->sig, c->nrec.p);
     else
         vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p,
-                                                         c->cap.p, nullptr, nullptr, c->rvariant == FVP_RESIDUAL_OWNER ? c->nrec.p : nullptr, c->xyz.p);
+                                                         c->cap.p, nullptr, nullptr, c->rvariant == FVP_RESIDUAL_OWNER ? c->nrec.p : nullptr);
     if (!averaging) {
         // tiled paths (element fluxes with the boundary fluxes, then the finalisation)
         const JvArgs none;
@@ -879,7 +877,7 @@ extern "C" int fvp_begin_step(fvp_ctx *c, double t, const double *psi, const dou
     const int N = c->N, T = c->T;
     // k_r at the state, then the averaged-weight face fluxes summed per edge
     vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p, c->cap.p,
-                                                     nullptr, nullptr, nullptr, nullptr);
+                                                     nullptr, nullptr, nullptr);
     if (c->dim == 3)
         elem_kernel<3, 2><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
                                                        c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p, nullptr,

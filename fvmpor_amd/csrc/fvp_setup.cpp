@@ -374,7 +374,15 @@ extern "C" int fvp_create(const fvp_problem *p, int device, fvp_ctx **out) {
     if (c->invV.upload(invV.data(), N) || c->zw.upload(zw.data(), zw.size()) ||
         c->zmask.upload(zmask.data(), N))
         return fail();
-    if (c->kr.alloc((size_t)c->nz * N) || c->cap.alloc(N) || c->nrec.alloc(4 * (size_t)N) || c->acc.alloc(N)) return fail();
+    if (c->kr.alloc((size_t)c->nz * N) || c->cap.alloc(N) || c->acc.alloc(N)) return fail();
+    {
+        // node records x, y, z, head: the coordinates are written here once, the
+        // head by every residual
+        std::vector<double> rec(4 * (size_t)N, 0.0);
+        for (int i = 0; i < N; ++i)
+            for (int d = 0; d < 3; ++d) rec[4 * (size_t)i + d] = p->xyz[3 * (size_t)i + d];
+        if (c->nrec.upload(rec.data(), rec.size())) return fail();
+    }
     FVP_HIP(hipMemset(c->kr.p, 0, sizeof(double) * c->kr.n));
 
     // ---- edges: sort (min, max, insertion index); first insertion fixes front/back
