@@ -301,6 +301,51 @@ __global__ void __launch_bounds__(kThreads) axpy_norm(long n, double *__restrict
     block_partial(acc, part + blockIdx.x);
 }
 
+// The first pass's update and the second pass's dots in one read of the basis:
+// w -= Q[:, 0..NC) coef, then block partials of Q[:, c] . w (c < NC) and of
+// w . w (column NC) on the updated w.
+template <int NC>
+__global__ void __launch_bounds__(kThreads) axpy_mdot(long n, double *__restrict__ w, const double *__restrict__ Q,
+                                                      long ldq, const double *__restrict__ coef,
+                                                      double *__restrict__ part) {
+    double h[NC], acc[NC + 1];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) h[c] = coef[c];
+#pragma unroll
+    for (int c = 0; c <= NC; ++c) acc[c] = 0.;
+    const long np = n >> 1;
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+        double2 q[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) q[c] = reinterpret_cast<const double2 *>(Q + c * ldq)[p];
+        double2 a = reinterpret_cast<double2 *>(w)[p];
+        double sx = 0., sy = 0.;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) { sx += h[c] * q[c].x; sy += h[c] * q[c].y; }
+        a.x -= sx;
+        a.y -= sy;
+        reinterpret_cast<double2 *>(w)[p] = a;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c] += a.x * q[c].x + a.y * q[c].y;
+        acc[NC] += a.x * a.x + a.y * a.y;
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        double qt[NC], sx = 0.;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) { qt[c] = Q[c * ldq + n - 1]; sx += h[c] * qt[c]; }
+        const double a = w[n - 1] - sx;
+        w[n - 1] = a;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c] += a * qt[c];
+        acc[NC] += a * a;
+    }
+#pragma unroll
+    for (int c = 0; c <= NC; ++c) {
+        block_partial(acc[c], part + c * gridDim.x + blockIdx.x);
+        __syncthreads();
+    }
+}
+
 using MdotLaunch = void (*)(hipStream_t, int, long, const double *, const double *, long, int, double *,
                            const double *);
 using AxpyLaunch = void (*)(hipStream_t, int, long, double *, const double *, long, const double *, double *,
@@ -315,6 +360,15 @@ static void axpy_launch(hipStream_t s, int nb, long n, double *w, const double *
                         double *part, const double *skip) {
     axpy_norm<NC><<<nb, kThreads, 0, s>>>(n, w, Q, ldq, coef, part, skip);
 }
+template <int NC>
+static void axpy_mdot_launch(hipStream_t s, int nb, long n, double *w, const double *Q, long ldq, const double *coef,
+                             double *part, const double *) {
+    axpy_mdot<NC><<<nb, kThreads, 0, s>>>(n, w, Q, ldq, coef, part);
+}
+template <int... I>
+static constexpr std::array<AxpyLaunch, sizeof...(I)> axpy_mdot_table(std::integer_sequence<int, I...>) {
+    return {&axpy_mdot_launch<I + 1>...};
+}
 template <int... I>
 static constexpr std::array<MdotLaunch, sizeof...(I)> mdot_table(std::integer_sequence<int, I...>) {
     return {&mdot_launch<I + 1>...};
@@ -326,6 +380,7 @@ static constexpr std::array<AxpyLaunch, sizeof...(I)> axpy_table(std::integer_se
 // per column count 1..kChunk
 static const auto kMdot = mdot_table(std::make_integer_sequence<int, kChunk>{});
 static const auto kAxpy = axpy_table(std::make_integer_sequence<int, kChunk>{});
+static const auto kAxpyMdot = axpy_mdot_table(std::make_integer_sequence<int, kChunk>{});
 
 // Selective reorthogonalisation (Kahan-Parlett "twice is enough"): a second
 // classical Gram-Schmidt pass when the first one cancelled more than half of
@@ -786,14 +841,25 @@ int gmres_device(fvp_csr *W, hipStream_t s, long n, const Op &A, const Op &M, co
                 else if (A(qj, w)) return -1;
                 // first pass: h1 = Q^T w, ||w||^2 -> h1[j+1]; w -= Q h1, ||w||^2 -> n1
                 multidot(s, n, w, Q, ldq, j + 1, part, h1, nullptr);
-                multiaxpy(s, n, w, Q, ldq, j + 1, h1, part, n1, nullptr);
-                if (mode) {
-                    reorth_decide<<<1, 64, 0, s>>>(h1 + j + 1, n1, mode, flag);
-                    multidot(s, n, w, Q, ldq, j + 1, part, h2, flag);
+                if (mode && j + 1 <= kChunk) {
+                    // first pass's update fused with the second pass's dots (h2, and
+                    // ||w||^2 after the first pass into n1 = h2[j + 1]); the second
+                    // update only when the first pass cancelled (flag)
+                    kAxpyMdot[j](s, gs_blocks(n), n, w, Q, ldq, h1, part, nullptr);
+                    reduce_cols<<<j + 2, kThreads, 0, s>>>(gs_blocks(n), part, h2, 0, nullptr);
+                    reorth_decide<<<1, 64, 0, s>>>(h1 + j + 1, h2 + j + 1, mode, flag);
                     multiaxpy(s, n, w, Q, ldq, j + 1, h2, part, n2, flag);
+                } else {
+                    multiaxpy(s, n, w, Q, ldq, j + 1, h1, part, n1, nullptr);
+                    if (mode) {
+                        reorth_decide<<<1, 64, 0, s>>>(h1 + j + 1, n1, mode, flag);
+                        multidot(s, n, w, Q, ldq, j + 1, part, h2, flag);
+                        multiaxpy(s, n, w, Q, ldq, j + 1, h2, part, n2, flag);
+                    }
                 }
-                arnoldi_update<<<1, 64, 0, s>>>(j, m, h1, h2, n1, n2, mode ? flag : nullptr, W->Hd.p, W->csd.p,
-                                                W->snd.p, W->gd.p, stat, ninv);
+                const bool fused = mode && j + 1 <= kChunk;  // ||w||^2 after the first pass in h2[j + 1]
+                arnoldi_update<<<1, 64, 0, s>>>(j, m, h1, h2, fused ? h2 + j + 1 : n1, n2, mode ? flag : nullptr,
+                                                W->Hd.p, W->csd.p, W->snd.p, W->gd.p, stat, ninv);
                 vscale_dev<<<nblk(n, 256), 256, 0, s>>>(n, w, ninv, Q + (size_t)(j + 1) * ldq);
                 FVP_HIP(hipMemcpyAsync(W->hstat + 2 * j, stat, 2 * sizeof(double),
                                        hipMemcpyDeviceToHost, s));
