@@ -69,6 +69,21 @@ __global__ void __launch_bounds__(kRedThreads) jv_sigma(int np, const double *__
     }
 }
 
+// the same from ||psi||^2 already reduced (psi_n2[0]) and the partials of ||v||^2
+__global__ void __launch_bounds__(kRedThreads) jv_sigma_n2(int np, const double *__restrict__ psi_n2,
+                                                          const double *__restrict__ part_v, double *__restrict__ sig) {
+    __shared__ double lds[kRedThreads / 64];
+    double b = 0.;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) b += part_v[i];
+    b = block_sum(b, lds);
+    if (threadIdx.x == 0) {
+        const double sq_eps = 1.4901161193847656e-08;  // sqrt(2^-52)
+        double np_ = sqrt(psi_n2[0]), nv = sqrt(b);
+        if (np_ == 0.) np_ = 1.;
+        sig[0] = nv > 0. ? sq_eps * np_ / nv : 1.;
+    }
+}
+
 __global__ void jv_perturb(long n, const double *__restrict__ sig, double cj, const double *__restrict__ psi,
                            const double *__restrict__ psip, const double *__restrict__ v, double *__restrict__ y,
                            double *__restrict__ yp) {
@@ -80,10 +95,11 @@ __global__ void jv_perturb(long n, const double *__restrict__ sig, double cj, co
 }
 
 __global__ void jv_finish(long n, const double *__restrict__ sig, const double *__restrict__ F1,
-                          const double *__restrict__ F0, double *__restrict__ Jv) {
+                          const double *__restrict__ F0, const double *__restrict__ osc, double *__restrict__ Jv) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    Jv[i] = (F1[i] - F0[i]) / sig[0];
+    const double d = (F1[i] - F0[i]) / sig[0];
+    Jv[i] = osc ? d * osc[i] : d;
 }
 
 static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -99,26 +115,37 @@ int dot_async(hipStream_t s, long n, const double *x, const double *y, double *p
 
 using namespace fvp;
 
-extern "C" int fvp_jv(fvp_ctx *c, double t, const double *psi, const double *psip, const double *F0, double c_j,
-                      const double *v, double *Jv) {
-    if (!c || !psi || !psip || !F0 || !v || !Jv) { set_error("null argument"); return -1; }
-    FVP_HIP(hipSetDevice(c->device));
+namespace fvp {
+int jv_impl(fvp_ctx *c, double t, const double *psi, const double *psip, const double *F0, double c_j,
+            const double *v, double *Jv, const double *psi_n2, const double *osc) {
     hipStream_t s = c->stream;
     const long N = c->N;
     double *part = c->red.p;  // [0, 512) psi partials, [512, 1024) v partials, [1024] sigma
-    dot_partial<<<kRedBlocks, kRedThreads, 0, s>>>(N, psi, psi, part);
     dot_partial<<<kRedBlocks, kRedThreads, 0, s>>>(N, v, v, part + kRedBlocks);
-    jv_sigma<<<1, kRedThreads, 0, s>>>(kRedBlocks, part, part + kRedBlocks, part + 2 * kRedBlocks);
+    if (psi_n2)
+        jv_sigma_n2<<<1, kRedThreads, 0, s>>>(kRedBlocks, psi_n2, part + kRedBlocks, part + 2 * kRedBlocks);
+    else {
+        dot_partial<<<kRedBlocks, kRedThreads, 0, s>>>(N, psi, psi, part);
+        jv_sigma<<<1, kRedThreads, 0, s>>>(kRedBlocks, part, part + kRedBlocks, part + 2 * kRedBlocks);
+    }
     FVP_HIP(hipGetLastError());
     if (!c->averaging) {  // perturbation and difference quotient fused into the residual kernels
         JvArgs jv;
-        jv.v = v; jv.sig = part + 2 * kRedBlocks; jv.F0 = F0; jv.cj = c_j;
+        jv.v = v; jv.sig = part + 2 * kRedBlocks; jv.F0 = F0; jv.cj = c_j; jv.osc = osc;
         return residual_impl(c, t, psi, psip, Jv, 0, &jv);
     }
     jv_perturb<<<nblk(N, 256), 256, 0, s>>>(N, part + 2 * kRedBlocks, c_j, psi, psip, v, c->tmp1.p, c->tmp2.p);
     FVP_HIP(hipGetLastError());
     if (residual_impl(c, t, c->tmp1.p, c->tmp2.p, c->tmp3.p, c->averaging)) return -1;
-    jv_finish<<<nblk(N, 256), 256, 0, s>>>(N, part + 2 * kRedBlocks, c->tmp3.p, F0, Jv);
+    jv_finish<<<nblk(N, 256), 256, 0, s>>>(N, part + 2 * kRedBlocks, c->tmp3.p, F0, osc, Jv);
     FVP_HIP(hipGetLastError());
     return 0;
+}
+}  // namespace fvp
+
+extern "C" int fvp_jv(fvp_ctx *c, double t, const double *psi, const double *psip, const double *F0, double c_j,
+                      const double *v, double *Jv) {
+    if (!c || !psi || !psip || !F0 || !v || !Jv) { set_error("null argument"); return -1; }
+    FVP_HIP(hipSetDevice(c->device));
+    return jv_impl(c, t, psi, psip, F0, c_j, v, Jv, nullptr, nullptr);
 }

@@ -378,17 +378,15 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
         fvp_gmres_stats gs{};
         const double *yv = I->y.p, *ypv = I->ypn.p, *Gv = I->G.p, *ew = I->ewt.p;
         double *tmpv = I->tmp.p;
-        Op A = [=](const double *in, double *out) {
-            if (fvp_jv(c, t, yv, ypv, Gv, cj, in, out)) return -1;
-            mul_kernel<<<nb(N), 256, 0, s>>>(N, out, ew, out);
-            return 0;
-        };
+        // ||y||^2 once for the Newton iterate (the FD increment of every J v of this solve)
+        double *yn2 = I->scal.p + 4;
+        if (dot_async(s, N, yv, yv, I->part.p, yn2)) return -1;
+        // S J: the error weights scale the J v rows inside the residual kernels
+        Op A = [=](const double *in, double *out) { return fvp::jv_impl(c, t, yv, ypv, Gv, cj, in, out, yn2, ew); };
         Op M;
         if (I->p.precon_type != FVP_PRECON_NONE)
-            M = [=](const double *in, double *out) {
-                div_kernel<<<nb(N), 256, 0, s>>>(N, in, ew, tmpv);
-                return fvp_precon_solve(c, tmpv, out);
-            };
+            // M S^-1: the block solves divide by the weights as they load
+            M = [=](const double *in, double *out) { return fvp::precon_solve_div(c, in, ew, tmpv, out); };
         else
             M = [=](const double *in, double *out) {
                 div_kernel<<<nb(N), 256, 0, s>>>(N, in, ew, out);

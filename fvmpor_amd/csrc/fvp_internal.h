@@ -123,8 +123,16 @@ namespace fvp {
 // scalar), fused into the residual kernels.
 struct JvArgs {
     const double *v = nullptr, *sig = nullptr, *F0 = nullptr;
+    const double *osc = nullptr;  // optional: J v scaled row-wise by osc
     double cj = 0.;
 };
+// J v with two options for the integrator's scaled solve: psi_n2 (device
+// scalar, or null) is ||psi||^2 computed once for the Newton iterate, osc
+// (or null) scales the result row-wise.  fvp_jv is the plain form.
+int jv_impl(fvp_ctx *c, double t, const double *psi, const double *psip, const double *F0, double c_j,
+            const double *v, double *Jv, const double *psi_n2, const double *osc);
+// z = M^-1 (r / rdiv) with the context's preconditioner (tmp: a work vector of N)
+int precon_solve_div(fvp_ctx *c, const double *r, const double *rdiv, double *tmp, double *z);
 int build_tiles(fvp_ctx *c, int tile_nodes);
 int edge_areas(fvp_ctx *c);
 int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, double *F, int averaging,

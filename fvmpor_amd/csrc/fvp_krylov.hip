@@ -165,6 +165,11 @@ __global__ void vmul(long n, const double *__restrict__ a, const double *__restr
     if (i < n) c[i] = a[i] * b[i];
 }
 
+__global__ void vdiv(long n, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ c) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) c[i] = a[i] / b[i];
+}
+
 __global__ void vcopy(long n, const double *__restrict__ a, double *__restrict__ b) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i < n) b[i] = a[i];
@@ -724,13 +729,16 @@ __global__ void __launch_bounds__(64) bj_solve(const int *__restrict__ roff, con
                                                const unsigned short *__restrict__ ucol, const T *__restrict__ uval,
                                                const int *__restrict__ uk0, const unsigned short *__restrict__ ukn,
                                                const T *__restrict__ udinv, const double *__restrict__ x,
-                                               double *__restrict__ y) {
+                                               const double *__restrict__ xdiv, double *__restrict__ y) {
     extern __shared__ double smem[];
     const int b = blockIdx.x;
     const int base = roff[b];
     const int cnt = roff[b + 1] - base;
     T *v = (T *)smem;
-    for (int r = threadIdx.x; r < cnt; r += 64) v[r] = (T)x[rowid ? rowid[base + r] : base + r];
+    for (int r = threadIdx.x; r < cnt; r += 64) {
+        const int g = rowid ? rowid[base + r] : base + r;
+        v[r] = (T)(xdiv ? x[g] / xdiv[g] : x[g]);  // optional row scaling x / xdiv
+    }
     wave_lds_order();
     bj_sweep<T, false>(v, blev_l[b], blev_l[b + 1], lptr_l, rows_l, lcol, lval, lk0, lkn, nullptr, ja, lu);
     bj_sweep<T, true>(v, blev_u[b], blev_u[b + 1], lptr_u, rows_u, ucol, uval, uk0, ukn, udinv, ja, lu);
@@ -1197,9 +1205,17 @@ static int precon_build_impl(fvp_csr *A, int type) {
     return 0;
 }
 
-static int precon_apply_impl(fvp_csr *A, const double *x, double *y) {
+// y = M^-1 (x / xdiv) with xdiv optional (the block solves divide as they load)
+static int precon_apply_impl(fvp_csr *A, const double *x, double *y, const double *xdiv = nullptr,
+                             double *tmp = nullptr) {
     hipStream_t s = A->stream;
     const int n = A->n;
+    if (xdiv && A->ptype != FVP_PRECON_BJILU0 && A->ptype != FVP_PRECON_BJILU0_F32) {
+        if (!tmp) { set_error("internal: scaled preconditioner solve needs a work vector"); return -1; }
+        vdiv<<<nblk(n, 256), 256, 0, s>>>(n, x, xdiv, tmp);
+        x = tmp;
+        xdiv = nullptr;
+    }
     switch (A->ptype) {
     case FVP_PRECON_NONE:
         vcopy<<<nblk(n, 256), 256, 0, s>>>(n, x, y);
@@ -1211,13 +1227,13 @@ static int precon_apply_impl(fvp_csr *A, const double *x, double *y) {
         bj_solve<double><<<A->nblk, 64, sizeof(double) * A->b_maxrows, s>>>(
             A->b_roff.p, A->b_rowid.p, A->b_own.p, A->b_ja.p, A->b_lu.p, A->b_lev_l.p, A->b_lptr_l.p,
             A->b_rows_l.p, A->sl_col.p, A->sl_val.p, A->sl_k0.p, A->sl_kn.p, A->b_lev_u.p, A->b_lptr_u.p,
-            A->b_rows_u.p, A->su_col.p, A->su_val.p, A->su_k0.p, A->su_kn.p, A->su_dinv.p, x, y);
+            A->b_rows_u.p, A->su_col.p, A->su_val.p, A->su_k0.p, A->su_kn.p, A->su_dinv.p, x, xdiv, y);
         break;
     case FVP_PRECON_BJILU0_F32:
         bj_solve<float><<<A->nblk, 64, sizeof(float) * A->b_maxrows, s>>>(
             A->b_roff.p, A->b_rowid.p, A->b_own.p, A->b_ja.p, A->b_lu32.p, A->b_lev_l.p, A->b_lptr_l.p,
             A->b_rows_l.p, A->sl_col.p, A->sl_val32.p, A->sl_k0.p, A->sl_kn.p, A->b_lev_u.p, A->b_lptr_u.p,
-            A->b_rows_u.p, A->su_col.p, A->su_val32.p, A->su_k0.p, A->su_kn.p, A->su_dinv32.p, x, y);
+            A->b_rows_u.p, A->su_col.p, A->su_val32.p, A->su_k0.p, A->su_kn.p, A->su_dinv32.p, x, xdiv, y);
         break;
     case FVP_PRECON_ILU0: {
         const int L = 8;
@@ -1574,6 +1590,18 @@ extern "C" int fvp_precon_setup(fvp_ctx *c, double t, double c_j, double h, cons
     FVP_HIP(hipGetLastError());
     return precon_build_impl(jd->J, type);
 }
+
+namespace fvp {
+// z = M^-1 (r / rdiv): the integrator's scaled solve (tmp: a work vector of N)
+int precon_solve_div(fvp_ctx *c, const double *r, const double *rdiv, double *tmp, double *z) {
+    if (!c->jd || c->jd->J->ptype < 0) { set_error("fvp_precon_setup has not run"); return -1; }
+    fvp_csr_set_stream(c->jd->J, c->stream);
+    prof_mark(c, 2, 0);
+    int rc = precon_apply_impl(c->jd->J, r, z, rdiv, tmp);
+    prof_mark(c, 2, 1);
+    return rc;
+}
+}  // namespace fvp
 
 extern "C" int fvp_precon_solve(fvp_ctx *c, const double *r, double *z) {
     if (!c || !r || !z) { set_error("null argument"); return -1; }

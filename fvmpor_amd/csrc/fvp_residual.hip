@@ -29,6 +29,11 @@
 namespace fvp {
 
 // Host twin of the reference's saturation kernel (fvmpor.h:302-343), same operation order.
+// a J v row, scaled by the optional row weights of the integrator's scaled solve
+__device__ __forceinline__ double jv_scaled(const JvArgs &jv, int i, double d) {
+    return jv.osc ? d * jv.osc[i] : d;
+}
+
 __device__ __forceinline__ void vg_eval(double h, const double *zp, double &Sw, double &dSw, double &krw) {
     if (h >= 0.) { Sw = 1.; dSw = 0.; krw = 1.; return; }
     const double nVG = zp[Z_N], mVG = zp[Z_M], S_r = zp[Z_SR];
@@ -409,7 +414,7 @@ __global__ void __launch_bounds__(1024) owner_finalize(
         if (JV) {
             const double yp = psip[i] + jv.cj * (sig * jv.v[i]);
             const double F1 = a * invV[i] - yp * cap[i];
-            F[i] = (F1 - jv.F0[i]) / sig;
+            F[i] = jv_scaled(jv, i, (F1 - jv.F0[i]) / sig);
         } else
             F[i] = a * invV[i] - psip[i] * cap[i];
     }
@@ -418,7 +423,7 @@ __global__ void __launch_bounds__(1024) owner_finalize(
         const int i = dnode[k];
         if (JV) {
             const double y = psi[i] + sig * jv.v[i];
-            F[i] = ((dval[k] - y) - jv.F0[i]) / sig;
+            F[i] = jv_scaled(jv, i, ((dval[k] - y) - jv.F0[i]) / sig);
         } else
             F[i] = dval[k] - psi[i];
     }
@@ -554,7 +559,7 @@ __global__ void __launch_bounds__(256) tile_lds_kernel(
         if (JV) {
             const double yp = psip[i] + jv.cj * (sig * jv.v[i]);
             const double F1 = acc[i - lo] * invV[i] - yp * cap[i];
-            F[i] = (F1 - jv.F0[i]) / sig;
+            F[i] = jv_scaled(jv, i, (F1 - jv.F0[i]) / sig);
         } else
             F[i] = acc[i - lo] * invV[i] - psip[i] * cap[i];
     }
@@ -563,7 +568,7 @@ __global__ void __launch_bounds__(256) tile_lds_kernel(
         const int i = dnode[k];
         if (JV) {
             const double y = psi[i] + sig * jv.v[i];
-            F[i] = ((dval[k] - y) - jv.F0[i]) / sig;
+            F[i] = jv_scaled(jv, i, ((dval[k] - y) - jv.F0[i]) / sig);
         } else
             F[i] = dval[k] - psi[i];
     }
@@ -585,7 +590,7 @@ __global__ void finalize_kernel(int N, const double *__restrict__ acc, const dou
         const double sig = jv.sig[0];
         const double yp = psip[i] + jv.cj * (sig * jv.v[i]);
         const double F1 = acc[i] * invV[i] - yp * cap[i];
-        F[i] = (F1 - jv.F0[i]) / sig;
+        F[i] = jv_scaled(jv, i, (F1 - jv.F0[i]) / sig);
     } else
         F[i] = acc[i] * invV[i] - psip[i] * cap[i];
 }
@@ -599,7 +604,7 @@ __global__ void dirichlet_kernel(int nd, const int *__restrict__ dnode, const do
         if (JV) {
             const double sig = jv.sig[0];
             const double y = psi[i] + sig * jv.v[i];
-            F[i] = ((dval[k] - y) - jv.F0[i]) / sig;
+            F[i] = jv_scaled(jv, i, ((dval[k] - y) - jv.F0[i]) / sig);
         } else
             F[i] = dval[k] - psi[i];
     }
