@@ -36,17 +36,20 @@ __device__ __forceinline__ double jv_scaled(const JvArgs &jv, int i, double d) {
 
 __device__ __forceinline__ void vg_eval(double h, const double *zp, double &Sw, double &dSw, double &krw) {
     if (h >= 0.) { Sw = 1.; dSw = 0.; krw = 1.; return; }
+    // The reference's operation order, with each power x^p taken as
+    // exp(p ln x) on the already-rounded base (a few ulp from pow, which keeps
+    // extra precision internally and costs about twice as much in FP64).
     const double nVG = zp[Z_N], mVG = zp[Z_M], S_r = zp[Z_SR];
-    double d = pow(-zp[Z_ALPHA] * h, nVG);  // a = (alpha |h|)^n
-    double s = d + 1.;                      // b
+    double d = exp(nVG * log(-zp[Z_ALPHA] * h));  // a = (alpha |h|)^n
+    double s = d + 1.;                            // b
     double k = -1. / s;
     d /= s;
-    s = pow(s, -mVG);                       // Se
+    s = exp(-mVG * log(s));                       // Se = b^-m
     d *= s;
     d /= h;
     d *= -(1 - S_r) * (nVG - 1);
     k += 1.;
-    k = pow(k, mVG);
+    k = exp(mVG * log(k));                        // (1 - 1/b)^m
     k -= 1.;
     k = k * k;
     k *= sqrt(s);
