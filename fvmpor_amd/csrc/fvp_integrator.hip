@@ -60,9 +60,18 @@ struct fvp_integrator {
     double learn_trust = 2., learn_band = 0.5;
     bool learn_took = false;   // this attempt started Newton from the network's proposal
     bool learn_block = false;  // retries of this step start from the predictor
+    // back-off: after kLearnWeakRun tries in a row whose candidate did not at
+    // least halve the preconditioned residual, the network sits out learn_skip
+    // attempts (4, 8, ... up to 64; a strong try resets it)
+    int learn_weak = 0, learn_skip = 0, learn_backoff = 4;
 };
 
 namespace fvp {
+
+// learned start: a try is strong when the candidate's preconditioned residual
+// is below this share of the predictor's; weak tries in a row before a back-off
+constexpr double kLearnStrong = 0.5;
+constexpr int kLearnWeakRun = 3;
 
 struct Coef8 {
     double v[8];
@@ -301,7 +310,11 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
     deriv();
     if (fvp::residual_impl(c, t, I->y.p, I->ypn.p, I->G.p, c->averaging)) return -1;
     I->learn_took = false;
-    const bool learn = I->mp && !I->learn_block;
+    bool learn = I->mp && !I->learn_block;
+    if (learn && I->learn_skip > 0) {
+        --I->learn_skip;
+        learn = false;
+    }
     if (learn || I->rec_feat) {
         float *x = I->rec_feat ? I->rec_feat : I->feat.p;
         learn_features<<<nb(4 * N), 256, 0, s>>>(N, H(I, 0), I->ypred.p, I->ewt.p, I->G.p, c->cap.p, cj,
@@ -354,6 +367,14 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
             std::swap(I->y, I->y2);
             std::swap(I->ypn, I->ypn2);
             std::swap(I->G, I->G2);
+        }
+        if (g1 < kLearnStrong * g0) {
+            I->learn_weak = 0;
+            I->learn_backoff = 4;
+        } else if (++I->learn_weak >= kLearnWeakRun) {
+            I->learn_skip = I->learn_backoff;
+            I->learn_backoff = std::min(2 * I->learn_backoff, 64);
+            I->learn_weak = 0;
         }
     }
     fvp_csr *W = fvp::jacobian_handle(c);
@@ -654,6 +675,8 @@ extern "C" int fvp_integrator_set_learned(fvp_integrator *I, fvp_mpnn *m, const 
     I->mp = m;
     I->learn_trust = trust;
     I->learn_band = band;
+    I->learn_weak = I->learn_skip = 0;
+    I->learn_backoff = 4;
     return 0;
 }
 

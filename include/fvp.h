@@ -324,7 +324,10 @@ int fvp_mpnn_synchronize(fvp_mpnn *m);
  * smaller first Newton step estimate, the preconditioned residual P G in the
  * error-weighted RMS norm (the weighted residual without a preconditioner);
  * the local error estimate still measures the corrector against the
- * predictor.  m = null switches it off.  The network must be built on this
+ * predictor.  Back-off: after three tries in a row whose proposal did not at
+ * least halve that estimate, the network sits out 4 step attempts, then 8,
+ * ... up to 64 (a try that halves it resets this); learned_tried counts the
+ * attempts that ran it.  m = null switches it off.  The network must be built on this
  * context's node count; it runs on the context's stream from here on.
  * Defaults of the bindings: trust 2, band 0.5 (head units). */
 int fvp_integrator_set_learned(fvp_integrator *I, fvp_mpnn *m, const float *static_features, double trust,
