@@ -75,6 +75,7 @@ struct fvp_ctx {
     std::vector<int> bent_ptr, bent_tag;
     std::vector<double> bent_area, bent_nrm;
     // Dirichlet rows
+    int jac_mode = FVP_JACOBIAN_ELEMENT;  // FD Jacobian of fvp_precon_setup
     int ndir = 0;
     fvp::DevBuf<int> dnode;
     fvp::DevBuf<double> dval;
@@ -131,6 +132,10 @@ struct JvArgs {
 // (or null) scales the result row-wise.  fvp_jv is the plain form.
 int jv_impl(fvp_ctx *c, double t, const double *psi, const double *psip, const double *F0, double c_j,
             const double *v, double *Jv, const double *psi_n2, const double *osc);
+// element-local FD Jacobian into J (nnz values); the arrays as in fvp_jacdata
+int fd_jacobian_elem(fvp_ctx *c, double cj, const double *psi, const double *psip, const double *shift, double *ys,
+                     double *kr2, double *cap2, const int *pos16, const int *ia, const int *dpos, double *J,
+                     long nnz);
 // z = M^-1 (r / rdiv) with the context's preconditioner (tmp: a work vector of N)
 int precon_solve_div(fvp_ctx *c, const double *r, const double *rdiv, double *tmp, double *z);
 int build_tiles(fvp_ctx *c, int tile_nodes);

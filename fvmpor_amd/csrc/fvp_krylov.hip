@@ -1437,6 +1437,11 @@ struct fvp_jacdata {
     std::vector<int> eptr;             // colour -> entries
     DevBuf<int> cols, erow, epos, ecol;
     DevBuf<double> shift, ys, yps, Fs;
+    // element-local quotients: per element the entry of (row node i, moved
+    // node k) at 16 e + 4 i + k, per row its diagonal entry; k_r and capacity
+    // with every node moved by its shift
+    DevBuf<int> pos16, dpos;
+    DevBuf<double> kr2, cap2;
 };
 
 namespace fvp {
@@ -1534,6 +1539,24 @@ static int jac_init(fvp_ctx *c) {
     }
     jd->cptr = cptr;
     jd->eptr = eptr;
+    {
+        std::vector<int> pos16(16 * (size_t)T, 0), dpos(N);
+        auto at = [&](int row, int col) {
+            return (int)(std::lower_bound(ja.begin() + ia[row], ja.begin() + ia[row + 1], col) - ja.begin());
+        };
+        for (int e = 0; e < T; ++e) {
+            const int n[4] = {conn[e].x, conn[e].y, conn[e].z, conn[e].w};
+            for (int i = 0; i < npe; ++i)
+                for (int k = 0; k < npe; ++k) pos16[16 * (size_t)e + 4 * i + k] = at(n[i], n[k]);
+        }
+        for (int i = 0; i < N; ++i) dpos[i] = at(i, i);
+        if (jd->pos16.upload(pos16.data(), pos16.size()) || jd->dpos.upload(dpos.data(), N) ||
+            jd->kr2.alloc((size_t)c->nz * N) || jd->cap2.alloc(N)) {
+            fvp_csr_destroy(jd->J);
+            delete jd;
+            return -1;
+        }
+    }
     if (jd->cols.upload(cols.data(), N) || jd->erow.upload(erow.data(), erow.size()) ||
         jd->epos.upload(epos.data(), epos.size()) || jd->ecol.upload(ecol.data(), ecol.size()) ||
         jd->shift.alloc(N) || jd->ys.alloc(N) || jd->yps.alloc(N) || jd->Fs.alloc(N)) {
@@ -1557,6 +1580,7 @@ void jac_free(fvp_ctx *c) {
     fvp_csr_destroy(jd->J);
     jd->cols.release(); jd->erow.release(); jd->epos.release(); jd->ecol.release();
     jd->shift.release(); jd->ys.release(); jd->yps.release(); jd->Fs.release();
+    jd->pos16.release(); jd->dpos.release(); jd->kr2.release(); jd->cap2.release();
     delete jd;
     c->jd = nullptr;
 }
@@ -1574,6 +1598,12 @@ extern "C" int fvp_precon_setup(fvp_ctx *c, double t, double c_j, double h, cons
     const int N = c->N;
     const double eps = 1.4901161193847656e-08;  // sqrt(2^-52)
     shift_kernel<<<nblk(N, 256), 256, 0, s>>>(N, eps, h, psi, psip, ewt, jd->shift.p);
+    if (!c->averaging && c->jac_mode == FVP_JACOBIAN_ELEMENT) {
+        if (fd_jacobian_elem(c, c_j, psi, psip, jd->shift.p, jd->ys.p, jd->kr2.p, jd->cap2.p, jd->pos16.p,
+                             jd->J->ia.p, jd->dpos.p, jd->J->a.p, jd->J->nnz))
+            return -1;
+        return precon_build_impl(jd->J, type);
+    }
     vcopy<<<nblk(N, 256), 256, 0, s>>>(N, psi, jd->ys.p);
     vcopy<<<nblk(N, 256), 256, 0, s>>>(N, psip, jd->yps.p);
     for (int k = 0; k < jd->ncol; ++k) {
@@ -1612,6 +1642,13 @@ extern "C" int fvp_precon_solve(fvp_ctx *c, const double *r, double *z) {
     int rc = precon_apply_impl(c->jd->J, r, z);
     prof_mark(c, 2, 1);
     return rc;
+}
+
+extern "C" int fvp_set_jacobian_mode(fvp_ctx *c, int mode) {
+    if (!c) { set_error("null argument"); return -1; }
+    if (mode != FVP_JACOBIAN_ELEMENT && mode != FVP_JACOBIAN_COLOURED) { set_error("invalid Jacobian mode"); return -1; }
+    c->jac_mode = mode;
+    return 0;
 }
 
 extern "C" int fvp_jacobian(fvp_ctx *c, fvp_csr **A) {

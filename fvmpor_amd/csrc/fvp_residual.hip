@@ -753,6 +753,113 @@ static void launch_tile(fvp_ctx *c, const double *psi, const double *psip, doubl
     if (eb) (void)hipEventRecord(eb, s);
 }
 
+// Element-local finite-difference Jacobian, -(dF/dpsi + c_j dF/dpsi') with
+// the coloured method's increments (shift) and upwind bits: every element is
+// evaluated at the base state and once per node with that node's head moved
+// by its shift (k_r from the second van Genuchten pass, kr2), and
+// (f_base - f_moved) V^-1 / shift goes to the entry (row node, moved node)
+// through pos16[16 e + 4 i + k].  The capacity term and the Dirichlet rows are
+// set by fd_node_kernel and fd_dirichlet.  The residual is a sum of element
+// terms, so these are the coloured quotients up to the rounding of the sums.
+template <int DIM>
+__global__ void __launch_bounds__(256) fd_elem_kernel(int T, int N, ZoneTable zt, double rho0,
+                                                      const int4 *__restrict__ conn, const uint8_t *__restrict__ ezone,
+                                                      const double *__restrict__ kmul, const uint8_t *__restrict__ emask,
+                                                      const double *__restrict__ nrec, const double *__restrict__ kr,
+                                                      const double *__restrict__ kr2, const double *__restrict__ shift,
+                                                      const double *__restrict__ invV, const int *__restrict__ pos16,
+                                                      double *__restrict__ J) {
+    __shared__ double zk[3 * kMaxZones];
+    if (threadIdx.x < 3 * zt.nz) zk[threadIdx.x] = zt.p[threadIdx.x / 3][ZK_X + threadIdx.x % 3];
+    __syncthreads();
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= T) return;
+    const ElemRec rec = load_elem<0>(e, conn, ezone, kmul, emask, nullptr);
+    NodeVals<DIM> nv;
+    load_nodes_rec<DIM>(rec, N, nrec, kr, nv);
+    int n[4];
+    double f0[4];
+    element_fluxes_vals<DIM, 0>(e, rec, nv, zk, rho0, nullptr, n, f0);
+    const double *k2 = kr2 + (size_t)rec.z * N;
+    double w[DIM + 1];
+#pragma unroll
+    for (int i = 0; i < DIM + 1; ++i) w[i] = invV[n[i]];
+#pragma unroll
+    for (int k = 0; k < DIM + 1; ++k) {
+        const double sk = shift[n[k]], hk = nv.h[k], kk = nv.k[k];
+        nv.h[k] = hk + sk;
+        nv.k[k] = k2[n[k]];
+        int n1[4];
+        double f1[4];
+        element_fluxes_vals<DIM, 0>(e, rec, nv, zk, rho0, nullptr, n1, f1);
+        nv.h[k] = hk;
+        nv.k[k] = kk;
+#pragma unroll
+        for (int i = 0; i < DIM + 1; ++i) atomicAdd(&J[pos16[16 * (size_t)e + 4 * i + k]], (f0[i] - f1[i]) * w[i] / sk);
+    }
+}
+
+// diagonal: the capacity term -psi' a_hh(psi) moved with psi_i and psi'_i + c_j shift_i
+__global__ void fd_node_kernel(int N, double cj, const double *__restrict__ psip, const double *__restrict__ shift,
+                               const double *__restrict__ cap, const double *__restrict__ cap2,
+                               const int *__restrict__ dpos, double *__restrict__ J) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double sh = shift[i];
+    J[dpos[i]] += ((psip[i] + cj * sh) * cap2[i] - psip[i] * cap[i]) / sh;
+}
+
+// Dirichlet rows F = psi_D - psi: zero off the diagonal, the quotient on it
+__global__ void fd_dirichlet(int nd, const int *__restrict__ dnode, const double *__restrict__ dval,
+                             const double *__restrict__ psi, const double *__restrict__ shift,
+                             const int *__restrict__ ia, const int *__restrict__ dpos, double *__restrict__ J) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nd) return;
+    const int i = dnode[k];
+    for (int p = ia[i]; p < ia[i + 1]; ++p) J[p] = 0.;
+    J[dpos[i]] = ((dval[k] - psi[i]) - (dval[k] - (psi[i] + shift[i]))) / shift[i];
+}
+
+__global__ void fd_fill(long n, double *__restrict__ a) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.;
+}
+
+__global__ void fd_add(long n, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ c) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) c[i] = a[i] + b[i];
+}
+
+int fd_jacobian_elem(fvp_ctx *c, double cj, const double *psi, const double *psip, const double *shift, double *ys,
+                     double *kr2, double *cap2, const int *pos16, const int *ia, const int *dpos, double *J,
+                     long nnz) {
+    hipStream_t s = c->stream;
+    const int N = c->N, T = c->T;
+    fd_fill<<<blocks(nnz, 256), 256, 0, s>>>(nnz, J);
+    // base state: k_r, capacity and the node records' heads
+    vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p, c->cap.p,
+                                                     nullptr, nullptr, c->nrec.p);
+    // every node moved by its shift (each node is moved once, in its own column)
+    fd_add<<<blocks(N, 256), 256, 0, s>>>(N, psi, shift, ys);
+    vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, ys, c->zmask.p, c->zw.p, kr2, cap2, nullptr,
+                                                     nullptr, nullptr);
+    if (c->dim == 3)
+        fd_elem_kernel<3><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, (const int4 *)c->conn.p,
+                                                          (const uint8_t *)c->ezone.p, c->kmul.p,
+                                                          (const uint8_t *)c->emask.p, c->nrec.p, c->kr.p, kr2, shift,
+                                                          c->invV.p, pos16, J);
+    else
+        fd_elem_kernel<2><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, (const int4 *)c->conn.p,
+                                                          (const uint8_t *)c->ezone.p, c->kmul.p,
+                                                          (const uint8_t *)c->emask.p, c->nrec.p, c->kr.p, kr2, shift,
+                                                          c->invV.p, pos16, J);
+    fd_node_kernel<<<blocks(N, 256), 256, 0, s>>>(N, cj, psip, shift, c->cap.p, cap2, dpos, J);
+    if (c->ndir)
+        fd_dirichlet<<<blocks(c->ndir, 256), 256, 0, s>>>(c->ndir, c->dnode.p, c->dval.p, psi, shift, ia, dpos, J);
+    FVP_HIP(hipGetLastError());
+    return 0;
+}
+
 int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, double *F, int averaging,
                   const JvArgs *jv) {
     (void)t;

@@ -52,6 +52,7 @@ SIGNATURES = {
     "fvp_precon_block_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fvp_set_block_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fvp_set_overlap": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "fvp_set_jacobian_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fvp_precon_overlap": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fvp_set_residual_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "fvp_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, _c_int_p,
@@ -99,6 +100,7 @@ SIGNATURES = {
 }
 
 PRECON_NONE, PRECON_JACOBI, PRECON_ILU0, PRECON_BJILU0, PRECON_BJILU0_F32 = 0, 2, 5, 7, 8
+JACOBIAN_ELEMENT, JACOBIAN_COLOURED = 0, 1
 RESIDUAL_OWNER, RESIDUAL_WINDOW = 0, 1
 BLOCK_ROWS = 512
 ERR_PIVOT, ERR_NOCONV = 1, 2
@@ -328,6 +330,11 @@ def _ctx_set_overlap(self, layers):
     _check(lib().fvp_set_overlap(self.h, int(layers)), "fvp_set_overlap")
 
 
+def _ctx_set_jacobian_mode(self, mode):
+    """JACOBIAN_ELEMENT (default) or JACOBIAN_COLOURED: how precon_setup forms the FD quotients."""
+    _check(lib().fvp_set_jacobian_mode(self.h, int(mode)), "fvp_set_jacobian_mode")
+
+
 def _ctx_set_block_rows(self, rows):
     _check(lib().fvp_set_block_rows(self.h, int(rows)), "fvp_set_block_rows")
 
@@ -378,6 +385,7 @@ Context.jacobian = _ctx_jacobian
 Context.colours = _ctx_colours
 Context.set_block_rows = _ctx_set_block_rows
 Context.set_overlap = _ctx_set_overlap
+Context.set_jacobian_mode = _ctx_set_jacobian_mode
 Context.set_residual_launch = _ctx_set_residual_launch
 Context.autotune = _ctx_autotune
 Context.newton_gmres = _ctx_newton_gmres

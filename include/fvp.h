@@ -174,16 +174,30 @@ int fvp_gmres_csr(fvp_csr *A, const double *b, double *x, const fvp_gmres_params
  * Newton-Krylov pieces on a problem context (IDA psetup / psolve and SPGMR,
  * ida_integrator.h:478-587, preconditioner_dss.cpp:284-405). */
 
-/* Coloured finite-difference Jacobian of the iteration matrix, stored as
+/* Finite-difference Jacobian of the iteration matrix, stored as
  * -(dF/dpsi + c_j dF/dpsi') like the reference (preconditioner_dss.cpp:317-357),
  * then factored with the given preconditioner type.  ewt: IDA error weights
- * (device).  Returns FVP_ERR_PIVOT if the factorisation meets a zero pivot. */
+ * (device).  Returns FVP_ERR_PIVOT if the factorisation meets a zero pivot.
+ * The quotients use the reference's increments; fvp_set_jacobian_mode picks
+ * how they are formed (element-local by default, see below). */
 int fvp_precon_setup(fvp_ctx *ctx, double t, double c_j, double h, const double *psi, const double *psip,
                      const double *F0, const double *ewt, int type);
 /* z = M^{-1} r with the last setup (device vectors) */
 int fvp_precon_solve(fvp_ctx *ctx, const double *r, double *z);
 /* the context's Jacobian as a CSR handle (pattern = node adjacency plus diagonal) */
 int fvp_jacobian(fvp_ctx *ctx, fvp_csr **A);
+/* How fvp_precon_setup forms the difference quotients.
+ * FVP_JACOBIAN_COLOURED: one full residual per colour of the greedy distance-2
+ * colouring, as preconditioner_dss.cpp:284-372 does.
+ * FVP_JACOBIAN_ELEMENT (default): every element is evaluated at the base state
+ * and once per node with that node moved by its increment, in one launch, and
+ * the differences are summed into the entries.  The residual is a sum of
+ * element terms plus node-local terms, so these are the same quotients up to
+ * the rounding of the sums.  A context with averaged k_r always uses the
+ * coloured form.  No counterpart in the reference. */
+#define FVP_JACOBIAN_ELEMENT 0
+#define FVP_JACOBIAN_COLOURED 1
+int fvp_set_jacobian_mode(fvp_ctx *ctx, int mode);
 /* rows per block / overlap layers of the context's block-Jacobi preconditioner
  * (see fvp_precon_block_rows, fvp_precon_overlap) */
 int fvp_set_block_rows(fvp_ctx *ctx, int rows);

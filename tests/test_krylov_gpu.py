@@ -157,6 +157,63 @@ def test_fd_jacobian_and_newton_solve(which, request):
     print(f"{which}: colours={nc_g} gmres its gpu={st['iterations']} oracle={sto['iterations']}")
 
 
+@pytest.mark.parametrize("which", ["slice2d", "box3d_small", "cassion", "slice2d_dirichlet"])
+def test_element_local_jacobian_matches_coloured(which, request):
+    """The element-local quotients (the default) against the coloured ones
+    (preconditioner_dss.cpp's form) on the same state: the same Jacobian up to
+    the rounding of the residual sums, Dirichlet rows included."""
+    import torch
+    import dataclasses
+    from fvmpor_amd.lib import Context, JACOBIAN_COLOURED, JACOBIAN_ELEMENT, PRECON_ILU0
+    if which == "slice2d_dirichlet":  # the slice with boundary tag 2 held at a prescribed head
+        base = request.getfixturevalue("slice2d")
+        bc = base.bc.copy()
+        bc[2, 0] = -1.0
+        prob = dataclasses.replace(base, bc_type=np.array([0, 3, 1], dtype=np.int32), bc=bc)
+    else:
+        prob = request.getfixturevalue(which)
+    ctx = Context(prob)
+    if which == "slice2d_dirichlet":
+        assert ctx.ndir > 0
+    op = O.OracleProblem(prob)
+    rng = np.random.default_rng(11)
+    y = prob.psi0 - 0.3 + rng.uniform(-0.2, 0.2, op.N)
+    yp = rng.normal(scale=0.05, size=op.N)
+    ty, typ = _dev(torch, y), _dev(torch, yp)
+    ctx.begin_step(0.0, ty, typ)
+    F0 = torch.empty_like(ty)
+    ctx.residual(0.0, ty, typ, F0)
+    ewt = _dev(torch, 1.0 / (1e-6 * np.abs(y) + 1e-6))
+    ia, ja = O.jac_pattern(op.omesh)
+    J = ctx.jacobian()
+    vals = {}
+    for mode in (JACOBIAN_COLOURED, JACOBIAN_ELEMENT, JACOBIAN_COLOURED):
+        ctx.set_jacobian_mode(mode)
+        assert ctx.precon_setup(0.0, 12.0, 0.05, ty, typ, F0, ewt, PRECON_ILU0) == 0
+        vals.setdefault(mode, []).append(J.get_values(len(ja)))
+    col, el = vals[JACOBIAN_COLOURED][0], vals[JACOBIAN_ELEMENT][0]
+    rowmax = np.repeat(np.maximum.reduceat(np.abs(col), ia[:-1]), np.diff(ia))
+    err = (np.abs(el - col) / rowmax).max()
+    # the residual's sums are not bitwise repeatable (atomics in the element
+    # kernel): the coloured form against itself shows part of the FD noise
+    noise = (np.abs(vals[JACOBIAN_COLOURED][1] - col) / rowmax).max()
+    print(f"{which}: element-local vs coloured, max |diff| / row max = {err:.2e} (coloured repeat {noise:.2e})")
+    # the quotients' rounding noise (heads differ by sqrt(eps)-sized shifts on top
+    # of elevations) is the bar of the oracle FD test; a missing term (capacity,
+    # moved k_r) would show at O(1)
+    assert err < 1e-4 and noise < 1e-4
+    if which == "slice2d_dirichlet":
+        # Dirichlet rows: the same quotient on the diagonal, exact zeros elsewhere
+        # (the rows whose coloured off-diagonal entries are all exactly zero)
+        zero_rows = np.array([np.all(col[ia[i]:ia[i + 1]][ja[ia[i]:ia[i + 1]] != i] == 0) for i in range(op.N)])
+        assert zero_rows.sum() > 0
+        for i in np.nonzero(zero_rows)[0]:
+            sl = slice(ia[i], ia[i + 1])
+            assert np.array_equal(el[sl], col[sl]), i
+    with pytest.raises(Exception):
+        ctx.set_jacobian_mode(7)
+
+
 @pytest.fixture(scope="module")
 def box_mid():
     from fvmpor_amd.problems import box3d_problem
