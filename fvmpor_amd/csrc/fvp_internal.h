@@ -134,8 +134,8 @@ int jv_impl(fvp_ctx *c, double t, const double *psi, const double *psip, const d
             const double *v, double *Jv, const double *psi_n2, const double *osc);
 // element-local FD Jacobian into J (nnz values); the arrays as in fvp_jacdata
 int fd_jacobian_elem(fvp_ctx *c, double cj, const double *psi, const double *psip, const double *shift, double *ys,
-                     double *kr2, double *cap2, const int *pos16, const int *ia, const int *dpos, double *J,
-                     long nnz);
+                     double *kr2, double *cap2, double *contrib, const long *cptr, const int *cidx, const int *ia,
+                     const int *dpos, double *J, long nnz);
 // z = M^-1 (r / rdiv) with the context's preconditioner (tmp: a work vector of N)
 int precon_solve_div(fvp_ctx *c, const double *r, const double *rdiv, double *tmp, double *z);
 int build_tiles(fvp_ctx *c, int tile_nodes);

@@ -1437,11 +1437,13 @@ struct fvp_jacdata {
     std::vector<int> eptr;             // colour -> entries
     DevBuf<int> cols, erow, epos, ecol;
     DevBuf<double> shift, ys, yps, Fs;
-    // element-local quotients: per element the entry of (row node i, moved
-    // node k) at 16 e + 4 i + k, per row its diagonal entry; k_r and capacity
-    // with every node moved by its shift
-    DevBuf<int> pos16, dpos;
-    DevBuf<double> kr2, cap2;
+    // element-local quotients: contribution slots 16 e + 4 i + k (row node i,
+    // moved node k), for each entry the list of its slots (cptr / cidx, in
+    // element order), per row its diagonal entry; k_r and capacity with every
+    // node moved by its shift
+    DevBuf<long> cptr2;
+    DevBuf<int> cidx, dpos;
+    DevBuf<double> contrib, kr2, cap2;
 };
 
 namespace fvp {
@@ -1540,17 +1542,34 @@ static int jac_init(fvp_ctx *c) {
     jd->cptr = cptr;
     jd->eptr = eptr;
     {
-        std::vector<int> pos16(16 * (size_t)T, 0), dpos(N);
+        // each entry's contribution slots, counting-sorted by entry (element order within)
+        const long nnz = ia[N];
+        std::vector<int> pos((size_t)npe * npe * T), dpos(N), cidx((size_t)npe * npe * T);
+        std::vector<long> cp(nnz + 1, 0);
         auto at = [&](int row, int col) {
             return (int)(std::lower_bound(ja.begin() + ia[row], ja.begin() + ia[row + 1], col) - ja.begin());
         };
         for (int e = 0; e < T; ++e) {
             const int n[4] = {conn[e].x, conn[e].y, conn[e].z, conn[e].w};
             for (int i = 0; i < npe; ++i)
-                for (int k = 0; k < npe; ++k) pos16[16 * (size_t)e + 4 * i + k] = at(n[i], n[k]);
+                for (int k = 0; k < npe; ++k) {
+                    const int p = at(n[i], n[k]);
+                    pos[((size_t)e * npe + i) * npe + k] = p;
+                    ++cp[p + 1];
+                }
+        }
+        for (long p = 0; p < nnz; ++p) cp[p + 1] += cp[p];
+        {
+            std::vector<long> fill(cp.begin(), cp.end() - 1);
+            for (int e = 0; e < T; ++e)
+                for (int i = 0; i < npe; ++i)
+                    for (int k = 0; k < npe; ++k)
+                        cidx[fill[pos[((size_t)e * npe + i) * npe + k]]++] = 16 * e + 4 * i + k;
         }
         for (int i = 0; i < N; ++i) dpos[i] = at(i, i);
-        if (jd->pos16.upload(pos16.data(), pos16.size()) || jd->dpos.upload(dpos.data(), N) ||
+        if ((long)16 * T > INT32_MAX) { set_error("too many elements for the FD Jacobian slots"); fvp_csr_destroy(jd->J); delete jd; return -1; }
+        if (jd->cptr2.upload(cp.data(), cp.size()) || jd->cidx.upload(cidx.data(), cidx.size()) ||
+            jd->dpos.upload(dpos.data(), N) || jd->contrib.alloc(16 * (size_t)T) ||
             jd->kr2.alloc((size_t)c->nz * N) || jd->cap2.alloc(N)) {
             fvp_csr_destroy(jd->J);
             delete jd;
@@ -1580,7 +1599,8 @@ void jac_free(fvp_ctx *c) {
     fvp_csr_destroy(jd->J);
     jd->cols.release(); jd->erow.release(); jd->epos.release(); jd->ecol.release();
     jd->shift.release(); jd->ys.release(); jd->yps.release(); jd->Fs.release();
-    jd->pos16.release(); jd->dpos.release(); jd->kr2.release(); jd->cap2.release();
+    jd->cptr2.release(); jd->cidx.release(); jd->dpos.release(); jd->contrib.release();
+    jd->kr2.release(); jd->cap2.release();
     delete jd;
     c->jd = nullptr;
 }
@@ -1599,8 +1619,8 @@ extern "C" int fvp_precon_setup(fvp_ctx *c, double t, double c_j, double h, cons
     const double eps = 1.4901161193847656e-08;  // sqrt(2^-52)
     shift_kernel<<<nblk(N, 256), 256, 0, s>>>(N, eps, h, psi, psip, ewt, jd->shift.p);
     if (!c->averaging && c->jac_mode == FVP_JACOBIAN_ELEMENT) {
-        if (fd_jacobian_elem(c, c_j, psi, psip, jd->shift.p, jd->ys.p, jd->kr2.p, jd->cap2.p, jd->pos16.p,
-                             jd->J->ia.p, jd->dpos.p, jd->J->a.p, jd->J->nnz))
+        if (fd_jacobian_elem(c, c_j, psi, psip, jd->shift.p, jd->ys.p, jd->kr2.p, jd->cap2.p, jd->contrib.p,
+                             jd->cptr2.p, jd->cidx.p, jd->J->ia.p, jd->dpos.p, jd->J->a.p, jd->J->nnz))
             return -1;
         return precon_build_impl(jd->J, type);
     }

@@ -757,18 +757,20 @@ static void launch_tile(fvp_ctx *c, const double *psi, const double *psip, doubl
 // the coloured method's increments (shift) and upwind bits: every element is
 // evaluated at the base state and once per node with that node's head moved
 // by its shift (k_r from the second van Genuchten pass, kr2), and
-// (f_base - f_moved) V^-1 / shift goes to the entry (row node, moved node)
-// through pos16[16 e + 4 i + k].  The capacity term and the Dirichlet rows are
-// set by fd_node_kernel and fd_dirichlet.  The residual is a sum of element
-// terms, so these are the coloured quotients up to the rounding of the sums.
+// (f_base - f_moved) V^-1 / shift for (row node i, moved node k) is stored in
+// contrib[16 e + 4 i + k]; fd_gather sums each entry's contributions in a
+// fixed order (no atomics, so the Jacobian is repeatable bit for bit).  The
+// capacity term and the Dirichlet rows are set by fd_node_kernel and
+// fd_dirichlet.  The residual is a sum of element terms, so these are the
+// coloured quotients up to the rounding of the sums.
 template <int DIM>
 __global__ void __launch_bounds__(256) fd_elem_kernel(int T, int N, ZoneTable zt, double rho0,
                                                       const int4 *__restrict__ conn, const uint8_t *__restrict__ ezone,
                                                       const double *__restrict__ kmul, const uint8_t *__restrict__ emask,
                                                       const double *__restrict__ nrec, const double *__restrict__ kr,
                                                       const double *__restrict__ kr2, const double *__restrict__ shift,
-                                                      const double *__restrict__ invV, const int *__restrict__ pos16,
-                                                      double *__restrict__ J) {
+                                                      const double *__restrict__ invV,
+                                                      double *__restrict__ contrib) {
     __shared__ double zk[3 * kMaxZones];
     if (threadIdx.x < 3 * zt.nz) zk[threadIdx.x] = zt.p[threadIdx.x / 3][ZK_X + threadIdx.x % 3];
     __syncthreads();
@@ -795,8 +797,18 @@ __global__ void __launch_bounds__(256) fd_elem_kernel(int T, int N, ZoneTable zt
         nv.h[k] = hk;
         nv.k[k] = kk;
 #pragma unroll
-        for (int i = 0; i < DIM + 1; ++i) atomicAdd(&J[pos16[16 * (size_t)e + 4 * i + k]], (f0[i] - f1[i]) * w[i] / sk);
+        for (int i = 0; i < DIM + 1; ++i) contrib[16 * (size_t)e + 4 * i + k] = (f0[i] - f1[i]) * w[i] / sk;
     }
+}
+
+// J[p] = sum of the contributions listed for entry p (cptr / cidx, in element order)
+__global__ void fd_gather(long nnz, const long *__restrict__ cptr, const int *__restrict__ cidx,
+                          const double *__restrict__ contrib, double *__restrict__ J) {
+    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (p >= nnz) return;
+    double s = 0.;
+    for (long q = cptr[p]; q < cptr[p + 1]; ++q) s += contrib[cidx[q]];
+    J[p] = s;
 }
 
 // diagonal: the capacity term -psi' a_hh(psi) moved with psi_i and psi'_i + c_j shift_i
@@ -820,22 +832,16 @@ __global__ void fd_dirichlet(int nd, const int *__restrict__ dnode, const double
     J[dpos[i]] = ((dval[k] - psi[i]) - (dval[k] - (psi[i] + shift[i]))) / shift[i];
 }
 
-__global__ void fd_fill(long n, double *__restrict__ a) {
-    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i < n) a[i] = 0.;
-}
-
 __global__ void fd_add(long n, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ c) {
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i < n) c[i] = a[i] + b[i];
 }
 
 int fd_jacobian_elem(fvp_ctx *c, double cj, const double *psi, const double *psip, const double *shift, double *ys,
-                     double *kr2, double *cap2, const int *pos16, const int *ia, const int *dpos, double *J,
-                     long nnz) {
+                     double *kr2, double *cap2, double *contrib, const long *cptr, const int *cidx, const int *ia,
+                     const int *dpos, double *J, long nnz) {
     hipStream_t s = c->stream;
     const int N = c->N, T = c->T;
-    fd_fill<<<blocks(nnz, 256), 256, 0, s>>>(nnz, J);
     // base state: k_r, capacity and the node records' heads
     vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p, c->cap.p,
                                                      nullptr, nullptr, c->nrec.p);
@@ -847,12 +853,13 @@ int fd_jacobian_elem(fvp_ctx *c, double cj, const double *psi, const double *psi
         fd_elem_kernel<3><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, (const int4 *)c->conn.p,
                                                           (const uint8_t *)c->ezone.p, c->kmul.p,
                                                           (const uint8_t *)c->emask.p, c->nrec.p, c->kr.p, kr2, shift,
-                                                          c->invV.p, pos16, J);
+                                                          c->invV.p, contrib);
     else
         fd_elem_kernel<2><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, (const int4 *)c->conn.p,
                                                           (const uint8_t *)c->ezone.p, c->kmul.p,
                                                           (const uint8_t *)c->emask.p, c->nrec.p, c->kr.p, kr2, shift,
-                                                          c->invV.p, pos16, J);
+                                                          c->invV.p, contrib);
+    fd_gather<<<blocks(nnz, 256), 256, 0, s>>>(nnz, cptr, cidx, contrib, J);
     fd_node_kernel<<<blocks(N, 256), 256, 0, s>>>(N, cj, psip, shift, c->cap.p, cap2, dpos, J);
     if (c->ndir)
         fd_dirichlet<<<blocks(c->ndir, 256), 256, 0, s>>>(c->ndir, c->dnode.p, c->dval.p, psi, shift, ia, dpos, J);

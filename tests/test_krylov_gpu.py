@@ -187,7 +187,7 @@ def test_element_local_jacobian_matches_coloured(which, request):
     ia, ja = O.jac_pattern(op.omesh)
     J = ctx.jacobian()
     vals = {}
-    for mode in (JACOBIAN_COLOURED, JACOBIAN_ELEMENT, JACOBIAN_COLOURED):
+    for mode in (JACOBIAN_COLOURED, JACOBIAN_ELEMENT, JACOBIAN_COLOURED, JACOBIAN_ELEMENT):
         ctx.set_jacobian_mode(mode)
         assert ctx.precon_setup(0.0, 12.0, 0.05, ty, typ, F0, ewt, PRECON_ILU0) == 0
         vals.setdefault(mode, []).append(J.get_values(len(ja)))
@@ -202,6 +202,8 @@ def test_element_local_jacobian_matches_coloured(which, request):
     # of elevations) is the bar of the oracle FD test; a missing term (capacity,
     # moved k_r) would show at O(1)
     assert err < 1e-4 and noise < 1e-4
+    # the element-local form sums each entry in a fixed order: repeatable bit for bit
+    assert np.array_equal(vals[JACOBIAN_ELEMENT][1], el)
     if which == "slice2d_dirichlet":
         # Dirichlet rows: the same quotient on the diagonal, exact zeros elsewhere
         # (the rows whose coloured off-diagonal entries are all exactly zero)
