@@ -117,14 +117,18 @@ using namespace fvp;
 
 namespace fvp {
 int jv_impl(fvp_ctx *c, double t, const double *psi, const double *psip, const double *F0, double c_j,
-            const double *v, double *Jv, const double *psi_n2, const double *osc) {
+            const double *v, double *Jv, const double *psi_n2, const double *osc, const double *v_n2,
+            int v_count) {
     hipStream_t s = c->stream;
     const long N = c->N;
     double *part = c->red.p;  // [0, 512) psi partials, [512, 1024) v partials, [1024] sigma
-    dot_partial<<<kRedBlocks, kRedThreads, 0, s>>>(N, v, v, part + kRedBlocks);
-    if (psi_n2)
+    if (psi_n2 && v_n2)
+        jv_sigma_n2<<<1, kRedThreads, 0, s>>>(v_count, psi_n2, v_n2, part + 2 * kRedBlocks);
+    else if (psi_n2) {
+        dot_partial<<<kRedBlocks, kRedThreads, 0, s>>>(N, v, v, part + kRedBlocks);
         jv_sigma_n2<<<1, kRedThreads, 0, s>>>(kRedBlocks, psi_n2, part + kRedBlocks, part + 2 * kRedBlocks);
-    else {
+    } else {
+        dot_partial<<<kRedBlocks, kRedThreads, 0, s>>>(N, v, v, part + kRedBlocks);
         dot_partial<<<kRedBlocks, kRedThreads, 0, s>>>(N, psi, psi, part);
         jv_sigma<<<1, kRedThreads, 0, s>>>(kRedBlocks, part, part + kRedBlocks, part + 2 * kRedBlocks);
     }

@@ -402,12 +402,28 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
         // ||y||^2 once for the Newton iterate (the FD increment of every J v of this solve)
         double *yn2 = I->scal.p + 4;
         if (dot_async(s, N, yv, yv, I->part.p, yn2)) return -1;
-        // S J: the error weights scale the J v rows inside the residual kernels
-        Op A = [=](const double *in, double *out) { return fvp::jv_impl(c, t, yv, ypv, Gv, cj, in, out, yn2, ew); };
+        // S J: the error weights scale the J v rows inside the residual kernels.
+        // GMRES applies J to the vector the preconditioner solve has just written
+        // (its ||z||^2 partials come with it); any other input is reduced afresh.
+        struct LastSolve {
+            const double *z = nullptr, *zn2 = nullptr;
+            int count = 0;
+        } last;
+        LastSolve *ls = &last;
+        Op A = [=](const double *in, double *out) {
+            const bool fresh = in == ls->z && ls->zn2;
+            ls->z = nullptr;
+            return fvp::jv_impl(c, t, yv, ypv, Gv, cj, in, out, yn2, ew, fresh ? ls->zn2 : nullptr,
+                                fresh ? ls->count : 0);
+        };
         Op M;
         if (I->p.precon_type != FVP_PRECON_NONE)
             // M S^-1: the block solves divide by the weights as they load
-            M = [=](const double *in, double *out) { return fvp::precon_solve_div(c, in, ew, tmpv, out); };
+            M = [=](const double *in, double *out) {
+                int rc = fvp::precon_solve_div(c, in, ew, tmpv, out, &ls->zn2, &ls->count);
+                ls->z = out;
+                return rc;
+            };
         else
             M = [=](const double *in, double *out) {
                 div_kernel<<<nb(N), 256, 0, s>>>(N, in, ew, out);

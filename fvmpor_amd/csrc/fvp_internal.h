@@ -127,17 +127,24 @@ struct JvArgs {
     const double *osc = nullptr;  // optional: J v scaled row-wise by osc
     double cj = 0.;
 };
-// J v with two options for the integrator's scaled solve: psi_n2 (device
-// scalar, or null) is ||psi||^2 computed once for the Newton iterate, osc
-// (or null) scales the result row-wise.  fvp_jv is the plain form.
+// J v with options for the integrator's scaled solve: psi_n2 (device scalar,
+// or null) is ||psi||^2 computed once for the Newton iterate; v_n2 (or null)
+// holds v_count partial sums of ||v||^2 already formed by the solve that wrote
+// v (used only with psi_n2); osc (or null) scales the result row-wise.
+// fvp_jv is the plain form.
 int jv_impl(fvp_ctx *c, double t, const double *psi, const double *psip, const double *F0, double c_j,
-            const double *v, double *Jv, const double *psi_n2, const double *osc);
+            const double *v, double *Jv, const double *psi_n2, const double *osc, const double *v_n2 = nullptr,
+            int v_count = 0);
 // element-local FD Jacobian into J (nnz values); the arrays as in fvp_jacdata
 int fd_jacobian_elem(fvp_ctx *c, double cj, const double *psi, const double *psip, const double *shift, double *ys,
                      double *kr2, double *cap2, double *contrib, const long *cptr, const int *cidx, const int *ia,
                      const int *dpos, double *J, long nnz);
-// z = M^-1 (r / rdiv) with the context's preconditioner (tmp: a work vector of N)
-int precon_solve_div(fvp_ctx *c, const double *r, const double *rdiv, double *tmp, double *z);
+// z = M^-1 (r / rdiv) with the context's preconditioner (tmp: a work vector of
+// N).  With zn2 given, the block-Jacobi solves also leave partial sums of
+// ||z||^2: *zn2 points at them and *zn2_count is their number (null and 0 for
+// other preconditioners).
+int precon_solve_div(fvp_ctx *c, const double *r, const double *rdiv, double *tmp, double *z,
+                     const double **zn2 = nullptr, int *zn2_count = nullptr);
 int build_tiles(fvp_ctx *c, int tile_nodes);
 int edge_areas(fvp_ctx *c);
 int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, double *F, int averaging,

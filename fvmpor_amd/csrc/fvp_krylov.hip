@@ -50,6 +50,7 @@ struct fvp_csr {
     int overlap = 0;                        // layers of neighbouring rows added to each block (RAS)
     int nblk = 0, b_maxnz = 0, b_maxrows = 0, lds_max = 0;
     DevBuf<int> b_roff, b_rowid, b_own;     // per block: first local row; global row of each local row; owned range
+    DevBuf<double> b_zn2;                   // per block: its share of ||z||^2 of the last scaled solve
     size_t b_lds = 0;
     DevBuf<int> b_ia, b_src, b_diag;        // row pointers into the block arrays, source entry, diagonal position
     DevBuf<unsigned short> b_ja;            // local column
@@ -460,9 +461,14 @@ __global__ void gmres_backsolve(int k, int m, const double *__restrict__ H, cons
 }
 
 // out[c] (+)= sum over blocks of part[c * nb + block]
+// With flag set, the block of the last column (||w||^2 after a first
+// Gram-Schmidt pass) also decides the second pass: flag = 1 when that norm
+// fell below half of ww[0] (||w||^2 before the pass), or always when ww is null.
 __global__ void __launch_bounds__(kThreads) reduce_cols(int nb, const double *__restrict__ part,
                                                         double *__restrict__ out, int accumulate,
-                                                        const double *__restrict__ skip) {
+                                                        const double *__restrict__ skip,
+                                                        const double *__restrict__ ww = nullptr,
+                                                        double *__restrict__ flag = nullptr) {
     if (skip && skip[0] == 0.) return;
     __shared__ double lds[kThreads / 64];
     int c = blockIdx.x;
@@ -476,6 +482,7 @@ __global__ void __launch_bounds__(kThreads) reduce_cols(int nb, const double *__
         double v = 0.;
         for (int k = 0; k < (int)(blockDim.x >> 6); ++k) v += lds[k];
         out[c] = accumulate ? out[c] + v : v;
+        if (flag && c == (int)gridDim.x - 1) flag[0] = (!ww || v < 0.5 * ww[0]) ? 1. : 0.;
     }
 }
 
@@ -729,7 +736,8 @@ __global__ void __launch_bounds__(64) bj_solve(const int *__restrict__ roff, con
                                                const unsigned short *__restrict__ ucol, const T *__restrict__ uval,
                                                const int *__restrict__ uk0, const unsigned short *__restrict__ ukn,
                                                const T *__restrict__ udinv, const double *__restrict__ x,
-                                               const double *__restrict__ xdiv, double *__restrict__ y) {
+                                               const double *__restrict__ xdiv, double *__restrict__ y,
+                                               double *__restrict__ yn2) {
     extern __shared__ double smem[];
     const int b = blockIdx.x;
     const int base = roff[b];
@@ -743,7 +751,16 @@ __global__ void __launch_bounds__(64) bj_solve(const int *__restrict__ roff, con
     bj_sweep<T, false>(v, blev_l[b], blev_l[b + 1], lptr_l, rows_l, lcol, lval, lk0, lkn, nullptr, ja, lu);
     bj_sweep<T, true>(v, blev_u[b], blev_u[b + 1], lptr_u, rows_u, ucol, uval, uk0, ukn, udinv, ja, lu);
     const int olo = own[2 * b], ohi = own[2 * b + 1];
-    for (int r = olo + threadIdx.x; r < ohi; r += 64) y[rowid ? rowid[base + r] : base + r] = (double)v[r];
+    double q = 0.;
+    for (int r = olo + threadIdx.x; r < ohi; r += 64) {
+        const double yr = (double)v[r];
+        y[rowid ? rowid[base + r] : base + r] = yr;
+        q += yr * yr;
+    }
+    if (yn2) {  // this block's share of ||y||^2 (owned rows partition the rows)
+        q = wave_reduce(q);
+        if (threadIdx.x == 0) yn2[b] = q;
+    }
 }
 
 static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -854,8 +871,8 @@ int gmres_device(fvp_csr *W, hipStream_t s, long n, const Op &A, const Op &M, co
                     // ||w||^2 after the first pass into n1 = h2[j + 1]); the second
                     // update only when the first pass cancelled (flag)
                     kAxpyMdot[j](s, gs_blocks(n), n, w, Q, ldq, h1, part, nullptr);
-                    reduce_cols<<<j + 2, kThreads, 0, s>>>(gs_blocks(n), part, h2, 0, nullptr);
-                    reorth_decide<<<1, 64, 0, s>>>(h1 + j + 1, h2 + j + 1, mode, flag);
+                    reduce_cols<<<j + 2, kThreads, 0, s>>>(gs_blocks(n), part, h2, 0, nullptr,
+                                                           mode == 2 ? nullptr : h1 + j + 1, flag);
                     multiaxpy(s, n, w, Q, ldq, j + 1, h2, part, n2, flag);
                 } else {
                     multiaxpy(s, n, w, Q, ldq, j + 1, h1, part, n1, nullptr);
@@ -1205,9 +1222,10 @@ static int precon_build_impl(fvp_csr *A, int type) {
     return 0;
 }
 
-// y = M^-1 (x / xdiv) with xdiv optional (the block solves divide as they load)
+// y = M^-1 (x / xdiv) with xdiv optional (the block solves divide as they load);
+// yn2 (optional, at least nblk doubles): the block solves' partials of ||y||^2
 static int precon_apply_impl(fvp_csr *A, const double *x, double *y, const double *xdiv = nullptr,
-                             double *tmp = nullptr) {
+                             double *tmp = nullptr, double *yn2 = nullptr) {
     hipStream_t s = A->stream;
     const int n = A->n;
     if (xdiv && A->ptype != FVP_PRECON_BJILU0 && A->ptype != FVP_PRECON_BJILU0_F32) {
@@ -1227,13 +1245,13 @@ static int precon_apply_impl(fvp_csr *A, const double *x, double *y, const doubl
         bj_solve<double><<<A->nblk, 64, sizeof(double) * A->b_maxrows, s>>>(
             A->b_roff.p, A->b_rowid.p, A->b_own.p, A->b_ja.p, A->b_lu.p, A->b_lev_l.p, A->b_lptr_l.p,
             A->b_rows_l.p, A->sl_col.p, A->sl_val.p, A->sl_k0.p, A->sl_kn.p, A->b_lev_u.p, A->b_lptr_u.p,
-            A->b_rows_u.p, A->su_col.p, A->su_val.p, A->su_k0.p, A->su_kn.p, A->su_dinv.p, x, xdiv, y);
+            A->b_rows_u.p, A->su_col.p, A->su_val.p, A->su_k0.p, A->su_kn.p, A->su_dinv.p, x, xdiv, y, yn2);
         break;
     case FVP_PRECON_BJILU0_F32:
         bj_solve<float><<<A->nblk, 64, sizeof(float) * A->b_maxrows, s>>>(
             A->b_roff.p, A->b_rowid.p, A->b_own.p, A->b_ja.p, A->b_lu32.p, A->b_lev_l.p, A->b_lptr_l.p,
             A->b_rows_l.p, A->sl_col.p, A->sl_val32.p, A->sl_k0.p, A->sl_kn.p, A->b_lev_u.p, A->b_lptr_u.p,
-            A->b_rows_u.p, A->su_col.p, A->su_val32.p, A->su_k0.p, A->su_kn.p, A->su_dinv32.p, x, xdiv, y);
+            A->b_rows_u.p, A->su_col.p, A->su_val32.p, A->su_k0.p, A->su_kn.p, A->su_dinv32.p, x, xdiv, y, yn2);
         break;
     case FVP_PRECON_ILU0: {
         const int L = 8;
@@ -1300,7 +1318,7 @@ extern "C" int fvp_csr_destroy(fvp_csr *A) {
     if (A->hstat) (void)hipHostFree(A->hstat);
     for (hipEvent_t e : A->ev) (void)hipEventDestroy(e);
     A->b_ia.release(); A->b_src.release(); A->b_diag.release(); A->b_ja.release(); A->b_lu.release(); A->b_lu32.release();
-    A->b_roff.release(); A->b_rowid.release(); A->b_own.release();
+    A->b_roff.release(); A->b_rowid.release(); A->b_own.release(); A->b_zn2.release();
     A->sl_col.release(); A->su_col.release(); A->sl_val.release(); A->su_val.release(); A->su_dinv.release();
     A->sl_val32.release(); A->su_val32.release(); A->su_dinv32.release(); A->sl_k0.release(); A->su_k0.release();
     A->sl_kn.release(); A->su_kn.release();
@@ -1643,12 +1661,22 @@ extern "C" int fvp_precon_setup(fvp_ctx *c, double t, double c_j, double h, cons
 
 namespace fvp {
 // z = M^-1 (r / rdiv): the integrator's scaled solve (tmp: a work vector of N)
-int precon_solve_div(fvp_ctx *c, const double *r, const double *rdiv, double *tmp, double *z) {
+int precon_solve_div(fvp_ctx *c, const double *r, const double *rdiv, double *tmp, double *z, const double **zn2,
+                     int *zn2_count) {
     if (!c->jd || c->jd->J->ptype < 0) { set_error("fvp_precon_setup has not run"); return -1; }
-    fvp_csr_set_stream(c->jd->J, c->stream);
+    fvp_csr *A = c->jd->J;
+    fvp_csr_set_stream(A, c->stream);
+    const bool bj = A->ptype == FVP_PRECON_BJILU0 || A->ptype == FVP_PRECON_BJILU0_F32;
+    double *part = nullptr;
+    if (zn2 && bj) {
+        if ((size_t)A->nblk > A->b_zn2.n && A->b_zn2.alloc(A->nblk)) return -1;
+        part = A->b_zn2.p;
+    }
     prof_mark(c, 2, 0);
-    int rc = precon_apply_impl(c->jd->J, r, z, rdiv, tmp);
+    int rc = precon_apply_impl(A, r, z, rdiv, tmp, part);
     prof_mark(c, 2, 1);
+    if (zn2) *zn2 = part;
+    if (zn2_count) *zn2_count = part ? A->nblk : 0;
     return rc;
 }
 }  // namespace fvp
