@@ -8,7 +8,9 @@ predictor increment, log10 h, order); this module supplies the static
 columns and drives data generation (the plain solver in recording mode) and
 training (forward, backward and Adam on the device).  The plain predictor
 stays the fallback: the integrator keeps whichever starting point has the
-smaller first Newton residual.  No counterpart in the reference.
+smaller preconditioned first Newton residual, and backs off (skips the
+network for 4, 8, ... up to 64 step attempts) after three tries in a row
+whose proposal did not halve it.  No counterpart in the reference.
 
 Static feature columns (N x 16, columns 0..4 are left to the integrator):
   5  height, normalised to [0, 1] over the mesh
