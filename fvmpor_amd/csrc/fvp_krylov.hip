@@ -1455,12 +1455,13 @@ struct fvp_jacdata {
     std::vector<int> eptr;             // colour -> entries
     DevBuf<int> cols, erow, epos, ecol;
     DevBuf<double> shift, ys, yps, Fs;
-    // element-local quotients: contribution slots 16 e + 4 i + k (row node i,
-    // moved node k), for each entry the list of its slots (cptr / cidx, in
-    // element order), per row its diagonal entry; k_r and capacity with every
-    // node moved by its shift
+    // element-local quotients: the contribution of (element e, row node i,
+    // moved node k) goes to srank[16 e + 4 i + k], so that each entry's
+    // contributions lie contiguous in [cptr2[p], cptr2[p + 1]) in element
+    // order; per row its diagonal entry; k_r and capacity with every node
+    // moved by its shift
     DevBuf<long> cptr2;
-    DevBuf<int> cidx, dpos;
+    DevBuf<int> srank, dpos;
     DevBuf<double> contrib, kr2, cap2;
 };
 
@@ -1562,7 +1563,7 @@ static int jac_init(fvp_ctx *c) {
     {
         // each entry's contribution slots, counting-sorted by entry (element order within)
         const long nnz = ia[N];
-        std::vector<int> pos((size_t)npe * npe * T), dpos(N), cidx((size_t)npe * npe * T);
+        std::vector<int> pos((size_t)npe * npe * T), dpos(N), srank(16 * (size_t)T, 0);
         std::vector<long> cp(nnz + 1, 0);
         auto at = [&](int row, int col) {
             return (int)(std::lower_bound(ja.begin() + ia[row], ja.begin() + ia[row + 1], col) - ja.begin());
@@ -1582,12 +1583,12 @@ static int jac_init(fvp_ctx *c) {
             for (int e = 0; e < T; ++e)
                 for (int i = 0; i < npe; ++i)
                     for (int k = 0; k < npe; ++k)
-                        cidx[fill[pos[((size_t)e * npe + i) * npe + k]]++] = 16 * e + 4 * i + k;
+                        srank[16 * (size_t)e + 4 * i + k] = (int)fill[pos[((size_t)e * npe + i) * npe + k]]++;
         }
         for (int i = 0; i < N; ++i) dpos[i] = at(i, i);
         if ((long)16 * T > INT32_MAX) { set_error("too many elements for the FD Jacobian slots"); fvp_csr_destroy(jd->J); delete jd; return -1; }
-        if (jd->cptr2.upload(cp.data(), cp.size()) || jd->cidx.upload(cidx.data(), cidx.size()) ||
-            jd->dpos.upload(dpos.data(), N) || jd->contrib.alloc(16 * (size_t)T) ||
+        if (jd->cptr2.upload(cp.data(), cp.size()) || jd->srank.upload(srank.data(), srank.size()) ||
+            jd->dpos.upload(dpos.data(), N) || jd->contrib.alloc((size_t)npe * npe * T) ||
             jd->kr2.alloc((size_t)c->nz * N) || jd->cap2.alloc(N)) {
             fvp_csr_destroy(jd->J);
             delete jd;
@@ -1617,7 +1618,7 @@ void jac_free(fvp_ctx *c) {
     fvp_csr_destroy(jd->J);
     jd->cols.release(); jd->erow.release(); jd->epos.release(); jd->ecol.release();
     jd->shift.release(); jd->ys.release(); jd->yps.release(); jd->Fs.release();
-    jd->cptr2.release(); jd->cidx.release(); jd->dpos.release(); jd->contrib.release();
+    jd->cptr2.release(); jd->srank.release(); jd->dpos.release(); jd->contrib.release();
     jd->kr2.release(); jd->cap2.release();
     delete jd;
     c->jd = nullptr;
@@ -1638,7 +1639,7 @@ extern "C" int fvp_precon_setup(fvp_ctx *c, double t, double c_j, double h, cons
     shift_kernel<<<nblk(N, 256), 256, 0, s>>>(N, eps, h, psi, psip, ewt, jd->shift.p);
     if (!c->averaging && c->jac_mode == FVP_JACOBIAN_ELEMENT) {
         if (fd_jacobian_elem(c, c_j, psi, psip, jd->shift.p, jd->ys.p, jd->kr2.p, jd->cap2.p, jd->contrib.p,
-                             jd->cptr2.p, jd->cidx.p, jd->J->ia.p, jd->dpos.p, jd->J->a.p, jd->J->nnz))
+                             jd->cptr2.p, jd->srank.p, jd->J->ia.p, jd->dpos.p, jd->J->a.p, jd->J->nnz))
             return -1;
         return precon_build_impl(jd->J, type);
     }

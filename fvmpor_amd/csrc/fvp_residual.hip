@@ -757,9 +757,10 @@ static void launch_tile(fvp_ctx *c, const double *psi, const double *psip, doubl
 // the coloured method's increments (shift) and upwind bits: every element is
 // evaluated at the base state and once per node with that node's head moved
 // by its shift (k_r from the second van Genuchten pass, kr2), and
-// (f_base - f_moved) V^-1 / shift for (row node i, moved node k) is stored in
-// contrib[16 e + 4 i + k]; fd_gather sums each entry's contributions in a
-// fixed order (no atomics, so the Jacobian is repeatable bit for bit).  The
+// (f_base - f_moved) V^-1 / shift for (row node i, moved node k) is stored at
+// contrib[srank[16 e + 4 i + k]], where the entries' contributions lie
+// contiguous and in element order; fd_gather sums each entry's range (no
+// atomics, so the Jacobian is repeatable bit for bit).  The
 // capacity term and the Dirichlet rows are set by fd_node_kernel and
 // fd_dirichlet.  The residual is a sum of element terms, so these are the
 // coloured quotients up to the rounding of the sums.
@@ -770,6 +771,7 @@ __global__ void __launch_bounds__(256) fd_elem_kernel(int T, int N, ZoneTable zt
                                                       const double *__restrict__ nrec, const double *__restrict__ kr,
                                                       const double *__restrict__ kr2, const double *__restrict__ shift,
                                                       const double *__restrict__ invV,
+                                                      const int *__restrict__ srank,
                                                       double *__restrict__ contrib) {
     __shared__ double zk[3 * kMaxZones];
     if (threadIdx.x < 3 * zt.nz) zk[threadIdx.x] = zt.p[threadIdx.x / 3][ZK_X + threadIdx.x % 3];
@@ -797,17 +799,17 @@ __global__ void __launch_bounds__(256) fd_elem_kernel(int T, int N, ZoneTable zt
         nv.h[k] = hk;
         nv.k[k] = kk;
 #pragma unroll
-        for (int i = 0; i < DIM + 1; ++i) contrib[16 * (size_t)e + 4 * i + k] = (f0[i] - f1[i]) * w[i] / sk;
+        for (int i = 0; i < DIM + 1; ++i) contrib[srank[16 * (size_t)e + 4 * i + k]] = (f0[i] - f1[i]) * w[i] / sk;
     }
 }
 
-// J[p] = sum of the contributions listed for entry p (cptr / cidx, in element order)
-__global__ void fd_gather(long nnz, const long *__restrict__ cptr, const int *__restrict__ cidx,
-                          const double *__restrict__ contrib, double *__restrict__ J) {
+// J[p] = sum of entry p's contributions contrib[cptr[p] .. cptr[p + 1]) (element order)
+__global__ void fd_gather(long nnz, const long *__restrict__ cptr, const double *__restrict__ contrib,
+                          double *__restrict__ J) {
     const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (p >= nnz) return;
     double s = 0.;
-    for (long q = cptr[p]; q < cptr[p + 1]; ++q) s += contrib[cidx[q]];
+    for (long q = cptr[p]; q < cptr[p + 1]; ++q) s += contrib[q];
     J[p] = s;
 }
 
@@ -838,7 +840,7 @@ __global__ void fd_add(long n, const double *__restrict__ a, const double *__res
 }
 
 int fd_jacobian_elem(fvp_ctx *c, double cj, const double *psi, const double *psip, const double *shift, double *ys,
-                     double *kr2, double *cap2, double *contrib, const long *cptr, const int *cidx, const int *ia,
+                     double *kr2, double *cap2, double *contrib, const long *cptr, const int *srank, const int *ia,
                      const int *dpos, double *J, long nnz) {
     hipStream_t s = c->stream;
     const int N = c->N, T = c->T;
@@ -853,13 +855,13 @@ int fd_jacobian_elem(fvp_ctx *c, double cj, const double *psi, const double *psi
         fd_elem_kernel<3><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, (const int4 *)c->conn.p,
                                                           (const uint8_t *)c->ezone.p, c->kmul.p,
                                                           (const uint8_t *)c->emask.p, c->nrec.p, c->kr.p, kr2, shift,
-                                                          c->invV.p, contrib);
+                                                          c->invV.p, srank, contrib);
     else
         fd_elem_kernel<2><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, (const int4 *)c->conn.p,
                                                           (const uint8_t *)c->ezone.p, c->kmul.p,
                                                           (const uint8_t *)c->emask.p, c->nrec.p, c->kr.p, kr2, shift,
-                                                          c->invV.p, contrib);
-    fd_gather<<<blocks(nnz, 256), 256, 0, s>>>(nnz, cptr, cidx, contrib, J);
+                                                          c->invV.p, srank, contrib);
+    fd_gather<<<blocks(nnz, 256), 256, 0, s>>>(nnz, cptr, contrib, J);
     fd_node_kernel<<<blocks(N, 256), 256, 0, s>>>(N, cj, psip, shift, c->cap.p, cap2, dpos, J);
     if (c->ndir)
         fd_dirichlet<<<blocks(c->ndir, 256), 256, 0, s>>>(c->ndir, c->dnode.p, c->dval.p, psi, shift, ia, dpos, J);
