@@ -60,6 +60,38 @@ def test_gmres_full_krylov_is_exact():
     np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-9, atol=1e-10)
 
 
+
+@pytest.mark.parametrize("precon", [0, 2, 5])
+def test_gmres_does_not_depend_on_heap_contents(precon):
+    """The CRS gemv scales its output by beta = 0 before summing, as the
+    reference's does, so a work vector taken uninitialised from the heap could
+    carry a NaN into GMRES.  Freed heap chunks of the work vectors' size are
+    filled with NaN bytes (0xFF) right before each solve; the solve must still
+    converge to the known answer."""
+    import ctypes
+    n = 1000  # 8 KB vectors: below the allocator's mmap threshold, so freed chunks are reused
+    rng = np.random.default_rng(1)
+    rows = [np.unique(np.concatenate([[i], rng.integers(0, n, 6)])) for i in range(n)]
+    ia = np.zeros(n + 1, dtype=np.int32)
+    ia[1:] = np.cumsum([len(r) for r in rows])
+    ja = np.concatenate(rows).astype(np.int32)
+    a = rng.uniform(-1, 1, len(ja))
+    a[ja == np.repeat(np.arange(n), np.diff(ia))] = 10.0
+    b = O.csr_gemv(ia, ja, a, np.ones(n))
+    libc = ctypes.CDLL(None)
+    libc.malloc.restype = ctypes.c_void_p
+    libc.malloc.argtypes = [ctypes.c_size_t]
+    libc.free.argtypes = [ctypes.c_void_p]
+    for _ in range(3):
+        chunks = [libc.malloc(8 * n) for _ in range(32)]
+        for c in chunks:
+            ctypes.memset(c, 0xFF, 8 * n)
+        for c in chunks:
+            libc.free(c)
+        x, st = O.gmres_csr(ia, ja, a, b, precon=precon, m=10, max_restarts=50, tol=1e-12)
+        assert st["converged"], st
+        assert np.abs(x - 1.0).max() < 1e-9
+
 @pytest.mark.parametrize("reorth", [0, 1])
 def test_arnoldi_relations(cassion_mesh, reorth):
     ia, ja, a = _cassion_matrix(cassion_mesh, shift=1.0)
