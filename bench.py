@@ -7,8 +7,11 @@ path, at rtol = atol = 1e-6.
 
 A "step" is one accepted time step of the integrator (begin-step upwinding,
 predictor, Newton iterations each with a residual, matrix-free GMRES with
-block-Jacobi ILU(0) of the coloured FD Jacobian).  Headline run: FP32
-preconditioner + learned initial iterate; W untimed warm-up steps from t = 0,
+block-Jacobi ILU(0) of the FD Jacobian, whose quotients use the coloured
+method's increments and are formed element by element).  Headline run: FP32
+preconditioner + learned initial iterate (the integrator backs off from the
+network when its proposals do not halve the preconditioned first residual;
+"learned_start_used" counts its tries); W untimed warm-up steps from t = 0,
 then exactly K timed steps; value = simulated days advanced in the timed steps
 / wall seconds.  The plain FP64 run and the FP32-preconditioner-only run
 integrate the same simulated window (W warm-up steps from t = 0, then to the
