@@ -333,7 +333,7 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
         }
     }
     // preconditioner setup when stale (IDA: first step, after a failure, c_j moved by > 25%)
-    double cjratio = I->cj_setup > 0 ? cj / I->cj_setup : 0.;
+    const double cjratio = I->cj_setup > 0 ? cj / I->cj_setup : 0.;
     if (I->p.precon_type != FVP_PRECON_NONE &&
         (I->need_setup || cjratio < 0.75 || cjratio > 1.25 || I->steps_since_setup >= 20)) {
         int rc = fvp_precon_setup(c, t, cj, h, I->y.p, I->ypn.p, I->G.p, I->ewt.p, I->p.precon_type);
@@ -341,7 +341,6 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
         if (rc > 0) return 2;
         ++I->st.precon_setups;
         I->cj_setup = cj;
-        cjratio = 1.;
         I->need_setup = false;
         I->steps_since_setup = 0;
         I->ss = 20.;
@@ -437,11 +436,12 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
         // convergence test judges the result); otherwise a recoverable failure
         // and a smaller step
         if (rc > 0 && !(gs.error < 0.1)) return 2;
-        // IDA damps the update by 2/(1+cjratio) with a stale iteration matrix
-        double damp = 2. / (1. + cjratio);
-        axpby_kernel<<<nb(N), 256, 0, s>>>(N, damp, I->delta.p, 1., I->y.p);
+        // J v is formed at the current c_j, so the Newton update is taken as
+        // solved; IDA scales by 2/(1+c_j ratio) only for a stored (stale)
+        // iteration matrix, not for its matrix-free SPGMR solve
+        axpby_kernel<<<nb(N), 256, 0, s>>>(N, 1., I->delta.p, 1., I->y.p);
         ++I->st.newton_iters;
-        double delnrm = wrms(I, I->delta.p) * damp;
+        double delnrm = wrms(I, I->delta.p);
         if (m == 0) {
             oldnrm = delnrm;
             if (delnrm <= 1e-4 * I->p.newton_tol || I->ss * delnrm <= I->p.newton_tol) { conv = true; break; }

@@ -79,6 +79,8 @@ struct fvp_ctx {
     int ndir = 0;
     fvp::DevBuf<int> dnode;
     fvp::DevBuf<double> dval;
+    std::vector<int> dtag_h;       // boundary tag that sets each Dirichlet row
+    std::vector<double> delev_h;   // elevation of each Dirichlet node (y in 2D, z in 3D)
     // node tiles of the residual kernel (see fvp_setup.cpp)
     int ntiles = 0, tile_nodes = fvp::kTileNodes, xcd_map = 1;  // tile size, XCD-grouped tile order
     int rvariant = FVP_RESIDUAL_OWNER;  // residual kernel variant
@@ -151,6 +153,7 @@ int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, d
                   const JvArgs *jv = nullptr);
 int tile_kernel_attrs(fvp_ctx *c);
 int upload_bc_values(fvp_ctx *c);
+int upload_dirichlet_values(fvp_ctx *c);
 void jac_free(fvp_ctx *c);
 // record a start / stop event of kind k on the context stream when profiling
 void prof_mark(fvp_ctx *c, int k, int stop);

@@ -92,6 +92,19 @@ int upload_bc_values(fvp_ctx *c) {
     return c->bval.upload(val.data(), val.size());
 }
 
+// Dirichlet values per Dirichlet row from the current boundary rows: psi_D
+// (type 1) or the hydrostatic head (1 + eta)(b - elevation) (type 4)
+// (fvmpor.h:430-451, definitions.h:114-146).
+int upload_dirichlet_values(fvp_ctx *c) {
+    std::vector<double> dv(c->ndir);
+    for (int k = 0; k < c->ndir; ++k) {
+        const int tag = c->dtag_h[k];
+        const double *b = &c->bc[(size_t)tag * FVP_BC_FIELDS];
+        dv[k] = c->bc_type[tag] == 1 ? b[0] : (1. + b[2]) * (b[1] - c->delev_h[k]);
+    }
+    return c->dval.upload(dv.data(), dv.size());
+}
+
 // Node tiles: tile t owns nodes [t*TILE, (t+1)*TILE); it processes its primary
 // elements (lowest node inside, a contiguous range) and the halo elements that
 // reach into it from other tiles.  Rebuilt when the tile size changes.
@@ -551,7 +564,8 @@ extern "C" int fvp_create(const fvp_problem *p, int device, fvp_ctx **out) {
     // ---- Dirichlet rows (fvmpor.h:404-451)
     {
         std::vector<int> dn;
-        std::vector<double> dv;
+        c->dtag_h.clear();
+        c->delev_h.clear();
         if (p->node_tag_ptr && p->node_tags) {
             for (int i = 0; i < N; ++i) {
                 int dtag = 0;
@@ -560,14 +574,13 @@ extern "C" int fvp_create(const fvp_problem *p, int device, fvp_ctx **out) {
                     if (tag > 0 && tag < c->ntags && (c->bc_type[tag] == 1 || c->bc_type[tag] == 4)) dtag = tag;
                 }
                 if (!dtag) continue;
-                const double *b = &c->bc[dtag * FVP_BC_FIELDS];
-                double el = dim == 2 ? p->xyz[3 * i + 1] : p->xyz[3 * i + 2];
                 dn.push_back(i);
-                dv.push_back(c->bc_type[dtag] == 1 ? b[0] : (1. + b[2]) * (b[1] - el));
+                c->dtag_h.push_back(dtag);
+                c->delev_h.push_back(dim == 2 ? p->xyz[3 * i + 1] : p->xyz[3 * i + 2]);
             }
         }
         c->ndir = (int)dn.size();
-        if (c->dnode.upload(dn.data(), dn.size()) || c->dval.upload(dv.data(), dv.size())) return fail();
+        if (c->dnode.upload(dn.data(), dn.size()) || upload_dirichlet_values(c)) return fail();
     }
     if (build_tiles(c, c->tile_nodes) || edge_areas(c)) return fail();
     if (c->tmp1.alloc(N) || c->tmp2.alloc(N) || c->tmp3.alloc(N) || c->red.alloc(4096)) return fail();
@@ -620,5 +633,5 @@ extern "C" int fvp_set_bc(fvp_ctx *c, const double *bc) {
     FVP_HIP(hipSetDevice(c->device));
     FVP_HIP(hipStreamSynchronize(c->stream));
     c->bc.assign(bc, bc + (size_t)c->ntags * FVP_BC_FIELDS);
-    return upload_bc_values(c);
+    return upload_bc_values(c) || upload_dirichlet_values(c) ? -1 : 0;
 }

@@ -225,6 +225,22 @@ def box_mesh_3d(nx: int = 160, ny: int = 160, nz: int = 40, lx: float = 100.0, l
     return Mesh(3, pts, conn, ftags, ptr, tags, np.zeros(conn.shape[0], dtype=np.int32))
 
 
+def retag_boundary(mesh: Mesh, tagger) -> Mesh:
+    """Mesh whose boundary faces carry ``tagger(centroids)`` (an int array, one
+    tag > 0 per face, from the (n, 3) face centroids); node tags follow."""
+    faces = TRI_FACES if mesh.dim == 2 else TET_FACES
+    be, bf = np.nonzero(mesh.face_tags)
+    fn = mesh.conn[be][np.arange(len(be))[:, None], faces[bf]]
+    cen = mesh.xyz[fn].mean(axis=1)
+    tags = np.asarray(tagger(cen), dtype=np.int32)
+    if tags.shape != (len(be),) or np.any(tags <= 0):
+        raise ValueError("tagger must give one positive tag per boundary face")
+    ftags = np.zeros_like(mesh.face_tags)
+    ftags[be, bf] = tags
+    ptr, ntags = _node_tags_from_faces(mesh.n_nodes, mesh.conn, ftags, mesh.dim)
+    return Mesh(mesh.dim, mesh.xyz, mesh.conn, ftags, ptr, ntags, mesh.zone)
+
+
 def gaussian_field(shape, spacing, corr, sigma, seed):
     """Stationary Gaussian field with exponential covariance
     sigma^2 exp(-sqrt(sum((dx_k / corr_k)^2))) on a regular grid, by FFT of a

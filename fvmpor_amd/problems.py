@@ -16,7 +16,7 @@ import dataclasses
 
 import numpy as np
 
-from .meshgen import Mesh, box_mesh_2d, box_mesh_3d, gaussian_field, renumber_nodes, tile_order
+from .meshgen import Mesh, box_mesh_2d, box_mesh_3d, gaussian_field, renumber_nodes, retag_boundary, tile_order
 
 ZONE_FIELDS = ("Kxx", "Kyy", "Kzz", "phi", "alpha", "n", "Sr", "Ss")
 BC_FIELDS = ("value", "b", "eta", "qx", "qy", "qz")
@@ -114,3 +114,81 @@ def box3d_problem(nx=160, ny=160, nz=40, seed_mesh=2, seed_field=3, seed_layers=
     bc[2, 0] = -rain
     psi0 = 5.0 - mesh.xyz[:, 2]
     return Problem("box3d", mesh, zones, ezone, ekmul, bc_type, bc, 1000.0, psi0, 1.0, "day")
+
+
+def _dirichlet_heads(mesh: Mesh, bc_type, bc, psi):
+    """psi with every Dirichlet node set to its boundary value (the last
+    type-1/4 tag of the node decides, as in fvmpor.h:404-451)."""
+    psi = psi.copy()
+    el = mesh.xyz[:, 1] if mesh.dim == 2 else mesh.xyz[:, 2]
+    for i in range(mesh.n_nodes):
+        for tag in mesh.node_tags[mesh.node_tag_ptr[i]:mesh.node_tag_ptr[i + 1]]:
+            if bc_type[tag] == 1:
+                psi[i] = bc[tag, 0]
+            elif bc_type[tag] == 4:
+                psi[i] = (1.0 + bc[tag, 2]) * (bc[tag, 1] - el[i])
+    return psi
+
+
+def slice2d_bc_problem(nx=41, ny=17, seed=1) -> Problem:
+    """The 2D loam slice with every boundary type of definitions.h:114-146 on
+    its own side: bottom directional flux (6), top strip x in [4, 6] m
+    prescribed flux (3), top left seepage (7), left side Dirichlet (1), right
+    side hydrostatic (4), top right shore (8)."""
+    lx, ly = 10.0, 5.0
+    mesh = box_mesh_2d(nx, ny, lx, ly, seed=seed)
+
+    def tagger(c):
+        x, y = c[:, 0], c[:, 1]
+        t = np.full(len(c), 1)
+        top = np.isclose(y, ly)
+        t[top & (x < 4.0)] = 3
+        t[top & (x >= 4.0) & (x <= 6.0)] = 2
+        t[top & (x > 6.0)] = 6
+        t[np.isclose(x, 0.0)] = 4
+        t[np.isclose(x, lx)] = 5
+        return t
+
+    mesh = retag_boundary(mesh, tagger)
+    alpha, n, tr, ts = SOIL_CLASSES["loam"]
+    zones = np.array([_zone_row(0.25, ts, alpha, n, tr / ts, 1e-5)])
+    bc_type = np.array([0, 6, 3, 7, 1, 4, 8], dtype=np.int32)
+    bc = np.zeros((7, 6))
+    bc[1, 3:6] = (0.0, 0.02, 0.0)      # directional flux through the bottom
+    bc[2, 0] = -0.05                   # infiltration on the strip
+    bc[3, 0] = 0.01                    # seepage outflow
+    bc[4, 0] = -0.5                    # fixed head on the left side
+    bc[5, 1:3] = (1.5, 0.0)            # hydrostatic right side, water table at 1.5 m
+    psi0 = _dirichlet_heads(mesh, bc_type, bc, 1.0 - mesh.xyz[:, 1])
+    return Problem("slice2d_bc", mesh, zones, np.zeros(mesh.n_elements, dtype=np.int32),
+                   np.ones(mesh.n_elements), bc_type, bc, 1000.0, psi0, 1.0, "day")
+
+
+def box3d_bc_problem(nx=12, ny=10, nz=6, seed=2) -> Problem:
+    """The heterogeneous 3D box with one boundary type per face: x = 0
+    Dirichlet (1), x = 100 m hydrostatic (4), y = 0 directional flux (6),
+    y = 100 m seepage (7), bottom shore (8), top rain (3)."""
+    base = box3d_problem(nx=nx, ny=ny, nz=nz, seed_mesh=seed)
+    lx, ly, lz = 100.0, 100.0, 20.0
+
+    def tagger(c):
+        t = np.full(len(c), 3)
+        t[np.isclose(c[:, 2], 0.0)] = 6
+        t[np.isclose(c[:, 1], ly)] = 5
+        t[np.isclose(c[:, 1], 0.0)] = 4
+        t[np.isclose(c[:, 0], lx)] = 2
+        t[np.isclose(c[:, 0], 0.0)] = 1
+        t[np.isclose(c[:, 2], lz)] = 7
+        return t
+
+    mesh = retag_boundary(base.mesh, tagger)
+    bc_type = np.array([0, 1, 4, 3, 6, 7, 8, 3], dtype=np.int32)
+    bc = np.zeros((8, 6))
+    bc[1, 0] = -2.0                    # fixed head on x = 0
+    bc[2, 1:3] = (6.0, 0.0)            # hydrostatic x = lx, water table at 6 m
+    bc[3, 0] = 0.0                     # tag 3 is not used by the tagger
+    bc[4, 3:6] = (0.0, 0.05, 0.01)     # directional flux through y = 0
+    bc[5, 0] = 0.005                   # seepage outflow through y = ly
+    bc[7, 0] = -0.02                   # rain on the top
+    psi0 = _dirichlet_heads(mesh, bc_type, bc, 5.0 - mesh.xyz[:, 2])
+    return Problem("box3d_bc", mesh, base.zones, base.ezone, base.ekmul, bc_type, bc, 1000.0, psi0, 1.0, "day")

@@ -79,6 +79,10 @@ def lib():
         L.orc_integrate_replay.restype = ctypes.c_int
         L.orc_integrate_replay.argtypes = [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                            _dp, ctypes.c_int, _dp, _ip, _dp, ctypes.c_int, _dp]
+        L.orc_integrate_replay_opts.restype = ctypes.c_int
+        L.orc_integrate_replay_opts.argtypes = [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+                                                ctypes.c_double, _dp, ctypes.c_int, _dp, _ip, _dp, ctypes.c_int,
+                                                _dp, _dp]
         L.orc_initial_derivative.argtypes = [_vp, ctypes.c_int, ctypes.c_double, _dp, _dp]
         _LIB = L
     return _LIB
@@ -303,6 +307,33 @@ def integrate_replay(oprob, psi0, hs, ks, wfs, t0=0.0, rtol=1e-8, atol=1e-8, new
     if n < 0:
         raise RuntimeError("oracle replay: Newton failed")
     return out, n
+
+
+class ReplayOpts(ctypes.Structure):
+    _fields_ = [("newton_tol", ctypes.c_double), ("lin_rel", ctypes.c_double), ("lin_abs", ctypes.c_double),
+                ("jac_policy", ctypes.c_int), ("ida_test", ctypes.c_int), ("max_newton", ctypes.c_int),
+                ("dim_krylov", ctypes.c_int), ("max_restarts", ctypes.c_int), ("own_upwind", ctypes.c_int),
+                ("verbose", ctypes.c_int)]
+
+
+def integrate_replay_opts(oprob, psi0, hs, ks, wfs=None, t0=0.0, rtol=1e-8, atol=1e-8, newton_tol=1e-7,
+                          lin_rel=1e-12, lin_abs=0.0, jac_policy=0, ida_test=False, max_newton=30, dim_krylov=30,
+                          max_restarts=20, verbose=False):
+    """integrate_replay with the Newton / GMRES / Jacobian-refresh options of
+    orc_integrate_replay_opts; wfs = None freezes the oracle's own upwind
+    weights at each step start.  Returns (psi, stats) with stats Newton
+    iterations, GMRES iterations, Jacobian set-ups, residual evaluations."""
+    out = np.zeros(oprob.N)
+    st = np.zeros(4)
+    own = wfs is None
+    wfs = _f64(np.zeros(1) if own else np.asarray(wfs).reshape(-1))
+    o = ReplayOpts(newton_tol, lin_rel, lin_abs, jac_policy, int(ida_test), max_newton, dim_krylov, max_restarts,
+                   int(own), int(verbose))
+    n = lib().orc_integrate_replay_opts(oprob.h, float(rtol), float(atol), ctypes.byref(o), float(t0), _f64(psi0),
+                                        len(hs), _f64(hs), _i32(ks), wfs, oprob.NE, out, st)
+    if n < 0:
+        raise RuntimeError("oracle replay: Newton failed")
+    return out, dict(newton_iters=int(st[0]), gmres_iters=int(st[1]), jac_setups=int(st[2]), residuals=int(st[3]))
 
 
 def initial_derivative(oprob, t0, psi0):

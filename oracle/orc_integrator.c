@@ -15,12 +15,14 @@
 #include "orc.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 typedef struct OrcProblem OrcProblem;
 void orc_residual(OrcProblem *p, double t, const double *h, const double *hp, double *res);
 void orc_set_weights(OrcProblem *p, const double *wf);
+void orc_begin_step(OrcProblem *p, double t, const double *psi, const double *psip);
 void orc_get_node_fields(const OrcProblem *p, double *ahh, double *Sw, double *dSw);
 int orc_num_dirichlet(const OrcProblem *p);
 void orc_dirichlet(const OrcProblem *p, int *nodes, double *vals);
@@ -103,12 +105,45 @@ static void bdf_weights(const double *tau, int k, double *a) {
     }
 }
 
+/* Options of a replay.
+ *  newton_tol   Newton stopping test ||delta||_WRMS <= newton_tol (ida_test = 0),
+ *               or IDA's rate-based test ss * ||delta|| <= newton_tol (ida_test = 1,
+ *               ss = 20 after a new iteration matrix, then rate / (1 - rate)).
+ *  lin_rel      GMRES relative tolerance (lin_abs = 0), or
+ *  lin_abs      IDA's linear tolerance lin_abs * sqrt(N) on the scaled residual
+ *               (eplifac * epsNewt; IDASpgmr(ida_mem, 0) at ida_integrator.h:252
+ *               keeps IDA's defaults 0.05 and 0.33).
+ *  jac_policy   0: coloured FD Jacobian + ILU(0) at every Newton iteration;
+ *               1: once per step; 2: IDA's lsetup policy (first step, c_j moved
+ *               by more than 25 %, after a failed Newton iteration, every 20 steps).
+ *  max_newton, dim_krylov, max_restarts: iteration limits.
+ *  own_upwind   1: freeze the upwind weights of every step from the state at its
+ *               start (preprocess_timestep, fvmpor_ODE_impl.cpp:118-142) instead
+ *               of taking them from wfs.
+ *  verbose      1: one progress line per Newton iteration on stderr. */
+typedef struct {
+    double newton_tol, lin_rel, lin_abs;
+    int jac_policy, ida_test, max_newton, dim_krylov, max_restarts, own_upwind, verbose;
+} OrcReplayOpts;
+
 /* Replay nsteps steps of sizes hs[] and orders ks[], upwind front weights
  * wfs[step * NE ...] per step.  rtol/atol define the weights of the scaled
- * linear solves and the Newton stopping test ||delta||_WRMS <= newton_tol.
- * Returns the number of Newton iterations, or -1 on a failed step. */
+ * linear solves.  Returns the number of Newton iterations (stats, if not
+ * null, receives Newton iterations, GMRES iterations, Jacobian set-ups and
+ * residual evaluations), or -1 on a failed step. */
+int orc_integrate_replay_opts(OrcProblem *p, double rtol, double atol, const OrcReplayOpts *o, double t0,
+                              const double *psi0, int nsteps, const double *hs, const int *ks, const double *wfs,
+                              int NE, double *psi_out, double *stats);
+
 int orc_integrate_replay(OrcProblem *p, double rtol, double atol, double newton_tol, double t0, const double *psi0,
                          int nsteps, const double *hs, const int *ks, const double *wfs, int NE, double *psi_out) {
+    OrcReplayOpts o = {newton_tol, 1e-12, 0., 0, 0, 30, 30, 20, 0, 0};
+    return orc_integrate_replay_opts(p, rtol, atol, &o, t0, psi0, nsteps, hs, ks, wfs, NE, psi_out, NULL);
+}
+
+int orc_integrate_replay_opts(OrcProblem *p, double rtol, double atol, const OrcReplayOpts *o, double t0,
+                              const double *psi0, int nsteps, const double *hs, const int *ks, const double *wfs,
+                              int NE, double *psi_out, double *stats) {
     OrcMesh *m = orc_problem_mesh(p);
     const int N = m->N;
     enum { NH = 7 };
@@ -144,10 +179,16 @@ int orc_integrate_replay(OrcProblem *p, double rtol, double atol, double newton_
     orc_set_weights(p, half);
     orc_initial_derivative(p, N, t0, hist[0], yp);
     free(half);
+    long n_gmres = 0, n_setup = 0, n_res = 0;
+    double cj_setup = -1., ss = 20.;
+    int since_setup = 0, need_setup = 1;
     for (int st = 0; st < nsteps && !status; ++st) {
         double h = hs[st];
-        int k = ks[st];
-        orc_set_weights(p, wfs + (size_t)st * NE);
+        /* an order above the number of stored solutions (a replay that starts
+         * inside a run) falls back to the highest order the history allows */
+        int k = ks[st] < nh ? ks[st] : nh;
+        if (o->own_upwind) orc_begin_step(p, th[0], hist[0], yp);
+        else orc_set_weights(p, wfs + (size_t)st * NE);
         for (int i = 0; i < N; ++i) ewt[i] = 1. / (rtol * fabs(hist[0][i]) + atol);
         double tau[NH + 1];
         tau[0] = th[0] + h;
@@ -173,23 +214,59 @@ int orc_integrate_replay(OrcProblem *p, double rtol, double atol, double newton_
         }
         const double cj = a[0];
         int conv = 0;
-        for (int it = 0; it < 30; ++it) {
+        double oldnrm = 0.;
+        for (int it = 0; it < o->max_newton; ++it) {
             for (int i = 0; i < N; ++i) ypn[i] = cj * y[i] + beta[i];
             orc_residual(p, tau[0], y, ypn, G);
-            orc_fd_jacobian(p, N, tau[0], cj, h, y, ypn, G, ewt, ia, ja, col, ncol, J);
-            if (!orc_ilu0_factor(N, ia, ja, J, dp)) { status = -1; break; }
-            for (int i = 0; i < N; ++i) { b[i] = -G[i] * ewt[i]; dl[i] = 0.; }
+            ++n_res;
+            const double ratio = cj_setup > 0 ? cj / cj_setup : 0.;
+            int setup = o->jac_policy == 0 || (o->jac_policy == 1 && it == 0) ||
+                        (o->jac_policy == 2 && it == 0 &&
+                         (need_setup || ratio < 0.75 || ratio > 1.25 || since_setup >= 20));
+            if (setup) {
+                orc_fd_jacobian(p, N, tau[0], cj, h, y, ypn, G, ewt, ia, ja, col, ncol, J);
+                n_res += ncol;
+                ++n_setup;
+                if (!orc_ilu0_factor(N, ia, ja, J, dp)) { status = -1; break; }
+                cj_setup = cj;
+                need_setup = 0;
+                since_setup = 0;
+                ss = 20.;
+            }
+            double bn = 0.;
+            for (int i = 0; i < N; ++i) { b[i] = -G[i] * ewt[i]; dl[i] = 0.; bn += b[i] * b[i]; }
+            bn = sqrt(bn);
             op.t = tau[0]; op.c = cj; op.y = y; op.yp = ypn; op.F0 = G; op.ewt = ewt;
             SPc P = {N, ia, ja, J, dp, ewt, tmp};
-            OrcGmres run = {30, 20, 1, 1e-12, 0, 0, 0, 0., 0.};
+            double tol = o->lin_rel;
+            if (o->lin_abs > 0.) tol = bn > 0. ? o->lin_abs * sqrt((double)N) / bn : 1.;
+            OrcGmres run = {o->dim_krylov, o->max_restarts, 1, tol, 0, 0, 0, 0., 0.};
             orc_gmres(N, sjv, &op, spc, &P, b, dl, &run);
+            n_gmres += run.total_iters;
+            n_res += run.total_iters;
             double dn = 0.;
             for (int i = 0; i < N; ++i) { y[i] += dl[i]; dn += (dl[i] * ewt[i]) * (dl[i] * ewt[i]); }
             dn = sqrt(dn / N);
             ++newton_total;
-            if (dn <= newton_tol) { conv = 1; break; }
+            if (o->verbose) {
+                fprintf(stderr, "oracle replay: step %d Newton %d: %d GMRES iterations, |delta| %.3e\n", st, it,
+                        run.total_iters, dn);
+                fflush(stderr);
+            }
+            if (!o->ida_test) {
+                if (dn <= o->newton_tol) { conv = 1; break; }
+            } else if (it == 0) {
+                oldnrm = dn;
+                if (dn <= 1e-4 * o->newton_tol || ss * dn <= o->newton_tol) { conv = 1; break; }
+            } else {
+                double rate = pow(dn / oldnrm, 1.0 / it);
+                if (rate > 0.9) break;
+                ss = fmax(rate / (1. - rate), 0.05);
+                if (ss * dn <= o->newton_tol) { conv = 1; break; }
+            }
         }
         if (!conv) { status = -1; break; }
+        ++since_setup;
         for (int i = 0; i < N; ++i) ypn[i] = cj * y[i] + beta[i];
         double *old = hist[NH - 1];
         for (int r = NH - 1; r > 0; --r) { hist[r] = hist[r - 1]; th[r] = th[r - 1]; }
@@ -198,6 +275,9 @@ int orc_integrate_replay(OrcProblem *p, double rtol, double atol, double newton_
         memcpy(hist[0], y, sizeof(double) * N);
         memcpy(yp, ypn, sizeof(double) * N);
         if (nh < NH) ++nh;
+    }
+    if (stats) {
+        stats[0] = newton_total; stats[1] = (double)n_gmres; stats[2] = (double)n_setup; stats[3] = (double)n_res;
     }
     memcpy(psi_out, hist[0], sizeof(double) * N);
     for (int i = 0; i < NH; ++i) free(hist[i]);

@@ -33,6 +33,18 @@ def slice2d():
 
 
 @pytest.fixture(scope="session")
+def slice2d_bc():
+    from fvmpor_amd.problems import slice2d_bc_problem
+    return slice2d_bc_problem()
+
+
+@pytest.fixture(scope="session")
+def box3d_bc():
+    from fvmpor_amd.problems import box3d_bc_problem
+    return box3d_bc_problem()
+
+
+@pytest.fixture(scope="session")
 def box3d_small():
     from fvmpor_amd.problems import box3d_problem
     return box3d_problem(nx=14, ny=12, nz=8)

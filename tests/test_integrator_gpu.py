@@ -12,15 +12,24 @@ pytestmark = pytest.mark.gpu
 
 
 def _problem(which):
-    from fvmpor_amd.problems import box3d_problem, slice2d_problem
-    if which == "slice2d":
+    """Replay cases: the slice and the small box with specific storage off
+    (S_s = 0 reproduces the reference's capacity, fvmpor.h:943-944) and on
+    (S_s = 1e-5 /m, the workloads' value), and the slice with every boundary
+    type (Dirichlet, hydrostatic, directional flux, seepage, shore)."""
+    from fvmpor_amd.problems import box3d_problem, slice2d_bc_problem, slice2d_problem
+    base = which.replace("_ss", "")
+    if base == "slice2d":
         p = slice2d_problem(nx=41, ny=17)
+        T = 0.1
+    elif base == "slice2d_bc":
+        p = slice2d_bc_problem()
         T = 0.1
     else:
         p = box3d_problem(nx=12, ny=10, nz=6, rain=0.05)
         T = 0.05
-    p.zones = p.zones.copy()
-    p.zones[:, 7] = 0.0
+    if not which.endswith("_ss") and base != "slice2d_bc":
+        p.zones = p.zones.copy()
+        p.zones[:, 7] = 0.0
     return p, T
 
 
@@ -46,7 +55,7 @@ def _gpu_run(prob, T, **kw):
     return out.cpu().numpy(), np.array(hs), np.array(ks, dtype=np.int32), np.array(wfs), integ.stats()
 
 
-@pytest.mark.parametrize("which", ["slice2d", "box3d"])
+@pytest.mark.parametrize("which", ["slice2d", "box3d", "slice2d_ss", "box3d_ss", "slice2d_bc"])
 def test_final_head_matches_oracle_replay(which):
     prob, T = _problem(which)
     psi_g, hs, ks, wfs, st = _gpu_run(prob, T)
@@ -57,6 +66,55 @@ def test_final_head_matches_oracle_replay(which):
     print(f"{which}: steps={len(hs)} orders={np.bincount(ks)} rel={rel:.2e} stats={st}")
     assert rel <= 1e-9
     assert set(ks.tolist()) <= {1, 2, 3}
+
+
+def test_unpreconditioned_run_matches_oracle_replay():
+    """FVP_PRECON_NONE: GMRES on the error-weight-scaled operator alone; the
+    Newton updates are taken undamped (the J v operator is formed at the
+    current c_j), and the run replays in the oracle like the preconditioned
+    ones."""
+    from fvmpor_amd.lib import PRECON_NONE
+    prob, _ = _problem("slice2d_ss")
+    T = 0.02
+    psi_g, hs, ks, wfs, st = _gpu_run(prob, T, precon=PRECON_NONE, dim_krylov=40, max_restarts=20)
+    assert st["precon_setups"] == 0
+    op = O.OracleProblem(prob)
+    psi_o, _ = O.integrate_replay(op, prob.psi0, hs, ks, wfs, rtol=1e-6, atol=1e-6, newton_tol=1e-7)
+    rel = np.abs(psi_g - psi_o).max() / np.abs(psi_o).max()
+    print(f"no preconditioner: steps={len(hs)} rel={rel:.2e} stats={st}")
+    assert rel <= 1e-9
+
+
+def test_slice_run_to_one_day_matches_oracle_replay():
+    """The 2D slice workload as specified: 101 x 41 lattice, loam with
+    S_s = 1e-5, 0.05 m/day on the strip, global ILU(0), rtol = atol = 1e-8,
+    to t = 1 day; the oracle replays the whole step sequence."""
+    import torch
+    from fvmpor_amd.lib import Context, Integrator, PRECON_ILU0
+    from fvmpor_amd.problems import slice2d_problem
+    prob = slice2d_problem()
+    ctx = Context(prob)
+    psi0 = torch.tensor(prob.psi0, device=device())
+    integ = Integrator(ctx, psi0, rtol=1e-8, atol=1e-8, precon=PRECON_ILU0, newton_tol=1e-4)
+    hs, ks, wfs = [], [], []
+    t = 0.0
+    while t < 1.0:
+        assert integ.step(1.0) == 0
+        st = integ.stats()
+        hs.append(st["h_last"])
+        ks.append(st["order_last"])
+        wfs.append(ctx.get_upwind()[0])
+        t = st["t"]
+    out = torch.empty_like(psi0)
+    integ.advance(1.0, out)
+    torch.cuda.synchronize()
+    psi_g = out.cpu().numpy()
+    op = O.OracleProblem(prob)
+    psi_o, rs = O.integrate_replay_opts(op, prob.psi0, np.array(hs), np.array(ks, dtype=np.int32), np.array(wfs),
+                                        rtol=1e-8, atol=1e-8, newton_tol=1e-5, lin_rel=1e-11, jac_policy=1)
+    rel = np.abs(psi_g - psi_o).max() / np.abs(psi_o).max()
+    print(f"slice to 1 day: {len(hs)} steps, GPU stats {integ.stats()}, oracle {rs}, rel={rel:.2e}")
+    assert rel <= 1e-9
 
 
 def test_mass_balance_of_device_run():
@@ -75,7 +133,7 @@ def test_mass_balance_of_device_run():
     assert abs(gained - inflow) < 1e-3 * inflow, (gained, inflow)
 
 
-def test_jacobi_and_unpreconditioned_runs_agree():
+def test_ilu0_and_jacobi_runs_agree():
     from fvmpor_amd.lib import PRECON_JACOBI
     prob, T = _problem("slice2d")
     a, *_ = _gpu_run(prob, 0.02)
@@ -104,3 +162,25 @@ def test_fp32_block_jacobi_run_agrees():
     rel = np.abs(a - b).max() / np.abs(a).max()
     print(f"fp32 vs fp64 block-Jacobi: final head rel diff {rel:.2e}, gmres iters {sb['gmres_iters']} vs {sa['gmres_iters']}")
     assert rel < 1e-6
+
+
+@pytest.mark.parametrize("index", [0, 1])
+def test_ensemble_draw_instance_matches_oracle_replay(index):
+    """Instances of the 256-instance ensemble draw at their drawn sizes (5k to
+    60k nodes; index 0 is a 2D slice, 1 a 3D box) with the ensemble's FP32
+    block-Jacobi preconditioner: the device run over the first 0.005 day,
+    replayed by the oracle."""
+    from fvmpor_amd.ensemble import draw_instances, make_instance
+    from fvmpor_amd.lib import PRECON_BJILU0_F32
+    d = draw_instances()[index]
+    prob = make_instance(d)
+    assert 5000 <= d.n_nodes <= 60000
+    T = 0.005
+    psi_g, hs, ks, wfs, st = _gpu_run(prob, T, precon=PRECON_BJILU0_F32)
+    op = O.OracleProblem(prob)
+    psi_o, rs = O.integrate_replay_opts(op, prob.psi0, hs, ks, wfs, rtol=1e-6, atol=1e-6, newton_tol=1e-7,
+                                        lin_rel=1e-11, jac_policy=1)
+    rel = np.abs(psi_g - psi_o).max() / np.abs(psi_o).max()
+    print(f"draw {index}: dim {d.dim}, N={prob.mesh.n_nodes}, sigma {d.sigma_lnK:.2f}, rain {d.rain:.3f}, "
+          f"steps={len(hs)}, rel={rel:.2e}, oracle {rs}")
+    assert rel <= 1e-9

@@ -66,3 +66,44 @@ def test_initial_derivative_balances_residual(tiny_slice):
     F = op.residual(0.0, prob.psi0, yp)
     ahh, _, _ = op.node_fields()
     assert np.abs(F[ahh != 0]).max() < 1e-9 * np.abs(F).max() + 1e-9
+
+
+def test_replay_options_reach_the_same_bdf_solution(tiny_slice):
+    """The replay's Newton / GMRES / Jacobian-refresh options change the work,
+    not the BDF solution: a Jacobian per step instead of per Newton iteration
+    agrees to the Newton tolerance, and IDA's loose tests (rate-based Newton
+    test at 0.33, linear tolerance 0.05 * 0.33 * sqrt(N)) to the integration
+    tolerance."""
+    prob = tiny_slice
+    op = O.OracleProblem(prob)
+    n = 12
+    hs = np.full(n, 0.1 / n)
+    ks = np.minimum(np.arange(1, n + 1), 3).astype(np.int32)
+    wfs = np.full((n, op.NE), 0.5)
+    ref, _ = O.integrate_replay(op, prob.psi0, hs, ks, wfs, rtol=1e-6, atol=1e-6, newton_tol=1e-8)
+    once, s1 = O.integrate_replay_opts(op, prob.psi0, hs, ks, wfs, rtol=1e-6, atol=1e-6, newton_tol=1e-8,
+                                       jac_policy=1)
+    ida, s2 = O.integrate_replay_opts(op, prob.psi0, hs, ks, wfs, rtol=1e-6, atol=1e-6, newton_tol=0.33,
+                                      lin_abs=0.05 * 0.33, jac_policy=2, ida_test=True, dim_krylov=20,
+                                      max_restarts=5)
+    scale = np.abs(ref).max()
+    assert s1["jac_setups"] == n and s2["jac_setups"] < n
+    assert np.abs(once - ref).max() <= 1e-10 * scale
+    assert np.abs(ida - ref).max() <= 1e-5 * scale
+    assert s2["newton_iters"] < s1["newton_iters"]
+
+
+def test_replay_with_its_own_upwinding(tiny_slice):
+    """wfs = None: the replay freezes the weights from each step's starting
+    state itself; for one step from t = 0 these are the weights begin_step
+    gives at (psi0, psi0') and the two replays coincide."""
+    prob = tiny_slice
+    op = O.OracleProblem(prob)
+    yp0 = O.initial_derivative(op, 0.0, prob.psi0)
+    op.begin_step(0.0, prob.psi0, yp0)
+    wf, _ = op.weights()
+    hs, ks = np.array([0.01]), np.array([1], dtype=np.int32)
+    a, _ = O.integrate_replay_opts(op, prob.psi0, hs, ks, wf[None, :], rtol=1e-6, atol=1e-6)
+    b, _ = O.integrate_replay_opts(op, prob.psi0, hs, ks, None, rtol=1e-6, atol=1e-6)
+    assert np.array_equal(a, b)
+    assert set(np.unique(wf)) <= {0.0, 1.0}

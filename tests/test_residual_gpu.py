@@ -16,7 +16,8 @@ from oracle.oracle import OracleProblem
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
-CASES = ["cassion", "slice2d", "box3d_small"]
+# slice2d_bc / box3d_bc carry boundary types 1, 3, 4, 6, 7 and 8 (definitions.h:114-146)
+CASES = ["cassion", "slice2d", "box3d_small", "slice2d_bc", "box3d_bc"]
 
 
 def _state(prob, seed):
@@ -112,7 +113,7 @@ def test_set_upwind_roundtrip_and_bc_update(slice2d):
     assert _rel(F.cpu().numpy(), Fc, op.flux_scale()) <= TOL
 
 
-@pytest.mark.parametrize("which", ["slice2d", "box3d_small"])
+@pytest.mark.parametrize("which", ["slice2d", "box3d_small", "slice2d_bc", "box3d_bc"])
 def test_jv_matches_oracle_difference_quotient(which, request):
     prob = request.getfixturevalue(which)
     ctx, dev, torch = _gpu(prob)
@@ -179,3 +180,29 @@ def test_every_launch_shape_and_the_autotuner_keep_parity(which, request):
     pick = ctx.autotune(0.0, tp, tpp)
     rel = check(pick)
     print(f"{which}: autotuner picked {pick}, rel={rel:.2e}")
+
+
+@pytest.mark.parametrize("which", ["slice2d_bc", "box3d_bc"])
+def test_set_bc_updates_every_boundary_type(which, request):
+    """New values for every tag, Dirichlet (1) and hydrostatic (4) rows
+    included, through fvp_set_bc: the residual follows the oracle built with
+    the new rows."""
+    prob = copy.copy(request.getfixturevalue(which))
+    ctx, dev, torch = _gpu(prob)
+    psi, psip = _state(prob, 7)
+    bc = prob.bc.copy()
+    bc[:, 0] = bc[:, 0] * 1.7 - 0.3
+    bc[:, 1] += 0.75
+    bc[:, 2] = 0.1
+    bc[:, 3:6] = bc[:, 3:6] * -2.0 + 0.01
+    ctx.set_bc(bc)
+    prob.bc = bc
+    op = OracleProblem(prob)
+    Fc = op.residual(0.0, psi, psip)
+    tp, tpp = torch.tensor(psi, device=dev), torch.tensor(psip, device=dev)
+    F = torch.empty_like(tp)
+    ctx.residual(0.0, tp, tpp, F)
+    torch.cuda.synchronize()
+    Fg = F.cpu().numpy()
+    assert ctx.ndir > 0
+    assert _rel(Fg, Fc, op.flux_scale()) <= TOL
