@@ -132,9 +132,11 @@ def _dirichlet_heads(mesh: Mesh, bc_type, bc, psi):
 
 def slice2d_bc_problem(nx=41, ny=17, seed=1) -> Problem:
     """The 2D loam slice with every boundary type of definitions.h:114-146 on
-    its own side: bottom directional flux (6), top strip x in [4, 6] m
-    prescribed flux (3), top left seepage (7), left side Dirichlet (1), right
-    side hydrostatic (4), top right shore (8)."""
+    its own part of the boundary, consistent with the hydrostatic start
+    psi0 = 1 - y where they fix a head: bottom Dirichlet psi = 1 (1), left
+    side hydrostatic with the water table at 1 m (4), top left directional
+    flux (6), top strip x in [4, 6] m prescribed flux (3), top right seepage
+    (7), right side shore (8)."""
     lx, ly = 10.0, 5.0
     mesh = box_mesh_2d(nx, ny, lx, ly, seed=seed)
 
@@ -142,23 +144,23 @@ def slice2d_bc_problem(nx=41, ny=17, seed=1) -> Problem:
         x, y = c[:, 0], c[:, 1]
         t = np.full(len(c), 1)
         top = np.isclose(y, ly)
-        t[top & (x < 4.0)] = 3
-        t[top & (x >= 4.0) & (x <= 6.0)] = 2
-        t[top & (x > 6.0)] = 6
-        t[np.isclose(x, 0.0)] = 4
-        t[np.isclose(x, lx)] = 5
+        t[top & (x < 4.0)] = 2
+        t[top & (x >= 4.0) & (x <= 6.0)] = 3
+        t[top & (x > 6.0)] = 4
+        t[np.isclose(x, 0.0)] = 5
+        t[np.isclose(x, lx)] = 6
         return t
 
     mesh = retag_boundary(mesh, tagger)
     alpha, n, tr, ts = SOIL_CLASSES["loam"]
     zones = np.array([_zone_row(0.25, ts, alpha, n, tr / ts, 1e-5)])
-    bc_type = np.array([0, 6, 3, 7, 1, 4, 8], dtype=np.int32)
+    bc_type = np.array([0, 1, 6, 3, 7, 4, 8], dtype=np.int32)
     bc = np.zeros((7, 6))
-    bc[1, 3:6] = (0.0, 0.02, 0.0)      # directional flux through the bottom
-    bc[2, 0] = -0.05                   # infiltration on the strip
-    bc[3, 0] = 0.01                    # seepage outflow
-    bc[4, 0] = -0.5                    # fixed head on the left side
-    bc[5, 1:3] = (1.5, 0.0)            # hydrostatic right side, water table at 1.5 m
+    bc[1, 0] = 1.0                     # bottom: fixed head, psi0 there
+    bc[2, 3:6] = (0.0, -0.01, 0.0)     # top left: directional flux, downwards (into the slice)
+    bc[3, 0] = -0.05                   # top strip: infiltration
+    bc[4, 0] = 0.005                   # top right: seepage outflow
+    bc[5, 1:3] = (1.0, 0.0)            # left side: hydrostatic, water table at 1 m
     psi0 = _dirichlet_heads(mesh, bc_type, bc, 1.0 - mesh.xyz[:, 1])
     return Problem("slice2d_bc", mesh, zones, np.zeros(mesh.n_elements, dtype=np.int32),
                    np.ones(mesh.n_elements), bc_type, bc, 1000.0, psi0, 1.0, "day")
