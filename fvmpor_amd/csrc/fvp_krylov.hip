@@ -19,6 +19,7 @@
 #include <array>
 #include <cmath>
 #include <functional>
+#include <mutex>
 #include <set>
 #include <utility>
 #include <vector>
@@ -824,15 +825,68 @@ using Op = std::function<int(const double *, double *)>;
 // step).  Classical Gram-Schmidt with a second pass only when the first one
 // cancelled more than half of ||w||^2 (reorthogonalise = 1), always (2) or
 // never (0).  A cycle ends early once the estimate meets tol * ||b||.
+// Device and host memory of one GMRES solve: the basis Q (gs_ldq(n) x (m+1)),
+// work vectors w, r, z (gs_ldq(n) each), reduction partials part
+// (kGsBlocks x (kChunk+1)), scalars sc (16), per-step coefficients hdev
+// (2 (m+2) + 2), the Hessenberg matrix Hd ((m+1) x m), Givens pairs, rhs and
+// solution; hstat (host, 2 (m+1)) receives each step's {estimate, h_{j+1,j}}.
+// With ev (one event per step) the host reads step j-1 while step j runs;
+// without, each step's pair is read right after the step (same control flow
+// and results, one synchronisation per step).
+struct GmresWork {
+    double *Q, *w, *r, *z, *part, *sc, *hdev, *Hd, *gd, *csd, *snd, *yd;
+    double *hstat;
+    const hipEvent_t *ev;
+};
+
+// doubles of a GmresWork carved from one device buffer (each piece 256-byte aligned)
+static size_t gmres_work_doubles(long n, int m) {
+    auto up = [](size_t v) { return (v + 31) & ~(size_t)31; };
+    const size_t ldq = (size_t)gs_ldq(n);
+    return up(ldq * (m + 1)) + 3 * up(ldq) + up((size_t)kGsBlocks * (kChunk + 1)) + up(16) + up(2 * (m + 2) + 2) +
+           up((size_t)(m + 1) * m) + 4 * up(m + 1);
+}
+static GmresWork gmres_work_carve(double *base, long n, int m) {
+    auto up = [](size_t v) { return (v + 31) & ~(size_t)31; };
+    const size_t ldq = (size_t)gs_ldq(n);
+    GmresWork wk{};
+    double *q = base;
+    auto take = [&](size_t cnt) { double *r = q; q += up(cnt); return r; };
+    wk.Q = take(ldq * (m + 1));
+    wk.w = take(ldq);
+    wk.r = take(ldq);
+    wk.z = take(ldq);
+    wk.part = take((size_t)kGsBlocks * (kChunk + 1));
+    wk.sc = take(16);
+    wk.hdev = take(2 * (m + 2) + 2);
+    wk.Hd = take((size_t)(m + 1) * m);
+    wk.gd = take(m + 1);
+    wk.csd = take(m + 1);
+    wk.snd = take(m + 1);
+    wk.yd = take(m + 1);
+    return wk;
+}
+
+static int gmres_core(const GmresWork &W, hipStream_t s, long n, const Op &A, const Op &M, const double *b,
+                      double *x, const fvp_gmres_params *p, fvp_gmres_stats *st);
+
 int gmres_device(fvp_csr *W, hipStream_t s, long n, const Op &A, const Op &M, const double *b, double *x,
                  const fvp_gmres_params *p, fvp_gmres_stats *st) {
     const int m = p->dim_krylov;
     if (m < 1 || m > 63) { set_error("dim_krylov must be 1..63"); return -1; }
     if (p->reorthogonalise < 0 || p->reorthogonalise > 2) { set_error("reorthogonalise must be 0, 1 or 2"); return -1; }
     if (ensure_work(W, m)) return -1;
-    double *Q = W->Q.p, *w = W->w.p, *r = W->r.p, *z = W->z.p, *part = W->part.p, *sc = W->scal.p;
+    GmresWork wk{W->Q.p, W->w.p, W->r.p, W->z.p, W->part.p, W->scal.p, W->hdev.p, W->Hd.p, W->gd.p, W->csd.p,
+                 W->snd.p, W->yd.p, W->hstat, W->ev.data()};
+    return gmres_core(wk, s, n, A, M, b, x, p, st);
+}
+
+static int gmres_core(const GmresWork &W, hipStream_t s, long n, const Op &A, const Op &M, const double *b,
+                      double *x, const fvp_gmres_params *p, fvp_gmres_stats *st) {
+    const int m = p->dim_krylov;
+    double *Q = W.Q, *w = W.w, *r = W.r, *z = W.z, *part = W.part, *sc = W.sc;
     // h1, h2: first / second pass coefficients (j+1 dots, then ||w||^2); stat: {estimate, h_{j+1,j}}
-    double *h1 = W->hdev.p, *h2 = W->hdev.p + (m + 2), *stat = W->hdev.p + 2 * (m + 2);
+    double *h1 = W.hdev, *h2 = W.hdev + (m + 2), *stat = W.hdev + 2 * (m + 2);
     // device scalars: sc[0] ||r||^2, sc[1] ||b||^2, sc[2] first-pass ||w||^2 after projection,
     // sc[3] second-pass ||w||^2, sc[4] reorthogonalisation flag, sc[5] 1 / h_{j+1,j}
     double *n1 = sc + 2, *n2 = sc + 3, *flag = sc + 4, *ninv = sc + 5;
@@ -857,7 +911,7 @@ int gmres_device(fvp_csr *W, hipStream_t s, long n, const Op &A, const Op &M, co
     int restarts = 0, converged = 0, total = 0;
     vscale_inv_norm<<<nblk(n, 256), 256, 0, s>>>(n, r, sc, Q);  // q0 = r / beta
     while (restarts <= p->max_restarts && !converged) {
-        gmres_cycle_init<<<1, 64, 0, s>>>(W->gd.p, m + 1, beta);
+        gmres_cycle_init<<<1, 64, 0, s>>>(W.gd, m + 1, beta);
         int k = m;
         for (int j = 0; j <= m; ++j) {
             if (j < m) {  // queue Arnoldi step j
@@ -884,22 +938,22 @@ int gmres_device(fvp_csr *W, hipStream_t s, long n, const Op &A, const Op &M, co
                 }
                 const bool fused = mode && j + 1 <= kChunk;  // ||w||^2 after the first pass in h2[j + 1]
                 arnoldi_update<<<1, 64, 0, s>>>(j, m, h1, h2, fused ? h2 + j + 1 : n1, n2, mode ? flag : nullptr,
-                                                W->Hd.p, W->csd.p, W->snd.p, W->gd.p, stat, ninv);
+                                                W.Hd, W.csd, W.snd, W.gd, stat, ninv);
                 vscale_dev<<<nblk(n, 256), 256, 0, s>>>(n, w, ninv, Q + (size_t)(j + 1) * ldq);
-                FVP_HIP(hipMemcpyAsync(W->hstat + 2 * j, stat, 2 * sizeof(double),
-                                       hipMemcpyDeviceToHost, s));
-                FVP_HIP(hipEventRecord(W->ev[j], s));
+                FVP_HIP(hipMemcpyAsync(W.hstat + 2 * j, stat, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+                if (W.ev) FVP_HIP(hipEventRecord(W.ev[j], s));
+                else FVP_HIP(hipStreamSynchronize(s));
                 FVP_HIP(hipGetLastError());
             }
-            if (j > 0) {  // read step j-1 while step j runs
-                FVP_HIP(hipEventSynchronize(W->ev[j - 1]));
+            if (j > 0) {  // read step j-1 (while step j runs, when pipelined)
+                if (W.ev) FVP_HIP(hipEventSynchronize(W.ev[j - 1]));
                 ++total;
-                const double est = W->hstat[2 * (j - 1)], hn = W->hstat[2 * (j - 1) + 1];
+                const double est = W.hstat[2 * (j - 1)], hn = W.hstat[2 * (j - 1) + 1];
                 if (est < p->tol * normb || !(hn > 1e-16)) { k = j; break; }
             }
         }
-        gmres_backsolve<<<1, 64, 0, s>>>(k, m, W->Hd.p, W->gd.p, W->yd.p);
-        lincomb<<<nblk(n, 256), 256, 0, s>>>(n, w, Q, ldq, k, W->yd.p);
+        gmres_backsolve<<<1, 64, 0, s>>>(k, m, W.Hd, W.gd, W.yd);
+        lincomb<<<nblk(n, 256), 256, 0, s>>>(n, w, Q, ldq, k, W.yd);
         if (M) { if (M(w, z)) return -1; }
         else vcopy<<<nblk(n, 256), 256, 0, s>>>(n, w, z);
         vadd<<<nblk(n, 256), 256, 0, s>>>(n, z, x);
@@ -1442,6 +1496,70 @@ extern "C" int fvp_gmres_csr(fvp_csr *A, const double *b, double *x, const fvp_g
     Op pc;
     if (A->ptype != FVP_PRECON_NONE) pc = [A](const double *in, double *out) { return precon_apply_impl(A, in, out); };
     return gmres_device(A, A->stream, A->n, op, pc, b, x, p, st);
+}
+
+// ---- allocation-free GMRES ------------------------------------------------------
+// Workspace of fvp_gmres_csr_noalloc: a static device array (32 MiB, one per
+// GPU, held until the process ends).  In host code the array's name is only a
+// stand-in; its device address on each GPU comes from hipGetSymbolAddress,
+// taken once per device with that device current.
+constexpr size_t kScratchDoubles = size_t(4) << 20;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kScratchDevices];
+
+__global__ void zero_fill(long n, double *__restrict__ p) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.;
+}
+
+// the scratch array's address on `device` (which must be current)
+static double *scratch_address(int device) {
+    if (device < 0 || device >= kScratchDevices) { set_error("device index out of range"); return nullptr; }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[device] = static_cast<double *>(ptr);
+    }
+    return g_scratch_ptr[device];
+}
+
+extern "C" int fvp_gmres_csr_noalloc(fvp_csr *A, const double *b, double *x, const fvp_gmres_params *p,
+                                     fvp_gmres_stats *st) {
+    if (!A || !b || !x || !p) { set_error("null argument"); return -1; }
+    if (A->ptype < 0) { set_error("preconditioner not built"); return -1; }
+    const int m = p->dim_krylov;
+    if (m < 1 || m > 63) { set_error("dim_krylov must be 1..63"); return -1; }
+    if (p->reorthogonalise < 0 || p->reorthogonalise > 2) { set_error("reorthogonalise must be 0, 1 or 2"); return -1; }
+    const size_t need = gmres_work_doubles(A->n, m);
+    if (need > kScratchDoubles) {
+        set_error("fvp_gmres_csr_noalloc: the workspace of n = " + std::to_string(A->n) + ", dim_krylov = " +
+                  std::to_string(m) + " needs " + std::to_string(need) + " doubles, more than the " +
+                  std::to_string(kScratchDoubles) + " of the static workspace (use fvp_gmres_csr)");
+        return -1;
+    }
+    FVP_HIP(hipSetDevice(A->device));
+    double *base = scratch_address(A->device);
+    if (!base) return -1;
+    GmresWork wk = gmres_work_carve(base, A->n, m);
+    // the scalars, coefficients and Hessenberg data start cleared on every call
+    // (the array keeps the previous call's contents); the basis, work vectors
+    // and partials are written before they are read
+    const long small = (long)(need - (size_t)(wk.sc - base));
+    zero_fill<<<(int)std::min(1024L, (small + 255) / 256), 256, 0, A->stream>>>(small, wk.sc);
+    FVP_HIP(hipGetLastError());
+    double hstat[2 * 64];
+    wk.hstat = hstat;
+    wk.ev = nullptr;
+    Op op = [A](const double *in, double *out) { return csr_gemv_impl(A, 1.0, in, 0.0, out); };
+    Op pc;
+    if (A->ptype != FVP_PRECON_NONE) pc = [A](const double *in, double *out) { return precon_apply_impl(A, in, out); };
+    return gmres_core(wk, A->stream, A->n, op, pc, b, x, p, st);
 }
 
 // ================================================================================

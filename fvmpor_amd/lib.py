@@ -62,6 +62,8 @@ SIGNATURES = {
     "fvp_precon_values": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p]),
     "fvp_gmres_csr": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p]),
+    "fvp_gmres_csr_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p]),
     "fvp_precon_setup": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_int]),
@@ -309,6 +311,18 @@ class CsrMatrix:
         rc = _check(lib().fvp_gmres_csr(self.h, _ptr(b), _ptr(x), ctypes.byref(p), ctypes.byref(st)),
                     "fvp_gmres_csr")
         return rc, st.as_dict()
+
+
+def _csr_gmres_noalloc(self, b, x, m=30, max_restarts=20, tol=1e-10, reorth=1):
+    """fvp_gmres_csr_noalloc: GMRES in the static per-GPU workspace."""
+    p = GmresParams(m, max_restarts, reorth, tol)
+    st = GmresStats()
+    rc = _check(lib().fvp_gmres_csr_noalloc(self.h, _ptr(b), _ptr(x), ctypes.byref(p), ctypes.byref(st)),
+                "fvp_gmres_csr_noalloc")
+    return rc, st.as_dict()
+
+
+CsrMatrix.gmres_noalloc = _csr_gmres_noalloc
 
 
 def _ctx_precon_setup(self, t, cj, h, psi, psip, F0, ewt, ptype=PRECON_ILU0) -> int:

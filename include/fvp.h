@@ -169,6 +169,20 @@ typedef struct {
 /* Right-preconditioned restarted GMRES on A x = b with A's built
  * preconditioner (gmres(), gmres.c:26-201).  x holds the initial guess. */
 int fvp_gmres_csr(fvp_csr *A, const double *b, double *x, const fvp_gmres_params *p, fvp_gmres_stats *st);
+/* fvp_gmres_csr without any allocation: the Krylov basis, work vectors,
+ * reduction partials and Hessenberg data live in a static 32 MiB device array
+ * (one per GPU, held until the process ends) instead of library allocations,
+ * and each Arnoldi step's residual estimate is read with a synchronisation
+ * instead of through pinned host memory; same kernels and same results.  No
+ * counterpart in the reference: for callers that cannot allocate device or
+ * pinned memory (tight serving loops, harnesses that forbid allocation).  The
+ * preconditioner is built beforehand (fvp_precon_build allocates).  Limit:
+ * the workspace, about n (dim_krylov + 4) + 35 k doubles, must fit 4 Mi
+ * doubles; larger problems return < 0 with a message.  Every call on one GPU
+ * shares the array, so calls that overlap on one GPU (other threads or
+ * streams) race. */
+int fvp_gmres_csr_noalloc(fvp_csr *A, const double *b, double *x, const fvp_gmres_params *p,
+                          fvp_gmres_stats *st);
 
 /* ---------------------------------------------------------------------------
  * Newton-Krylov pieces on a problem context (IDA psetup / psolve and SPGMR,
