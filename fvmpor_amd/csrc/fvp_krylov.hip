@@ -57,17 +57,27 @@ struct fvp_csr {
     DevBuf<unsigned short> b_ja;            // local column
     DevBuf<double> b_lu;
     DevBuf<float> b_lu32;                   // FP32 factor (FVP_PRECON_BJILU0_F32)
-    // solve layout: one slot per (row, triangle) in level order (the order of
-    // b_rows_l / b_rows_u); per slot the first kBjRow entries of the row's L
-    // (U) part as padded columns and values, the row's first entry and count
-    DevBuf<unsigned short> sl_col, su_col;
-    DevBuf<double> sl_val, su_val, su_dinv;
-    DevBuf<float> sl_val32, su_val32, su_dinv32;
-    DevBuf<int> sl_k0, su_k0;
-    DevBuf<unsigned short> sl_kn, su_kn;
     DevBuf<int> b_lev_l, b_lev_u;           // per block: level pointer range [b_lev[b], b_lev[b+1]) into *_lptr
     DevBuf<int> b_lptr_l, b_lptr_u;         // level -> rows range in *_rows
     DevBuf<unsigned short> b_rows_l, b_rows_u;  // local rows grouped by level
+    // the solve's levels: the factor's levels cut into pieces of at most 64 rows
+    DevBuf<int> s_lev_l, s_lev_u, s_lptr_l, s_lptr_u;
+    int s_levmax_l = 0, s_levmax_u = 0;
+    // packed solve layout per triangle (see bj_solve): per slot its meta word
+    // pair, per block its entry base, per entry its source in b_lu (-1:
+    // padding) and local column, the values of the current factor; per U slot
+    // the source of its diagonal and the inverted diagonal
+    DevBuf<unsigned> meta_l, meta_u;
+    DevBuf<long> ebase_l, ebase_u;
+    DevBuf<int> esrc_l, esrc_u, dsrc;
+    DevBuf<unsigned short> col_l, col_u;
+    DevBuf<double> val_l, val_u, dinv;
+    DevBuf<float> val32_l, val32_u, dinv32;
+    // per block its overflow entries (row parts past kBjRow): packed position and triangle
+    DevBuf<int> ov_ptr;
+    DevBuf<long> ov_pos;
+    DevBuf<uint8_t> ov_tri;
+    int ov_max = 0;
     // workspace
     int m_alloc = -1;
     DevBuf<double> Q, w, r, z, part, scal, hdev;
@@ -599,13 +609,20 @@ __global__ void __launch_bounds__(kBjFactorThreads) bj_factor(
         for (int k = threadIdx.x; k < nzb; k += blockDim.x) lu[z0 + k] = val[k];
 }
 
-// Solve: one wavefront per block.  Only the block's solution lives in LDS (so
-// that every block of the matrix is resident at once and the CU interleaves
-// their dependency chains); the factor streams from a level-ordered copy (one
-// slot per row and triangle, its first kBjRow entries padded), whose addresses
-// are known in advance, so the slots of the next levels are loaded ahead and
-// only the LDS reads of the solution and the FMA chain sit on the critical
-// path.  The entries of a row are subtracted in column order, as the
+// Solve: one wavefront per block.  Only the block's solution, its overflow
+// entries and its level pointers live in LDS (so that every block of the
+// matrix is resident at once and the CU interleaves their dependency chains).
+// The factor streams from a level-ordered packed copy per triangle: the row
+// parts one after another (each padded to an even length), so the bytes read
+// are the factor's own.  Per row part (a "slot", level order) a meta word pair
+// gives its start and its overflow offset, its row and its length; lane t of
+// a level reads the fixed window of kBjRow entries starting at its slot's
+// start (past the row's length the window runs into the following rows, which
+// the level uses anyway, and is masked), the entries past kBjRow come from
+// LDS.  Two prefetch stages: the meta words of level l + 2 kBjAhead and the
+// windows of level l + kBjAhead (whose meta words arrived kBjAhead levels
+// earlier) are loaded while level l is solved, so the level chain waits only
+// on LDS.  The entries of a row are subtracted in column order, as the
 // sequential solve does.  T is the factor's arithmetic type (double, or float
 // for the FP32 preconditioner); x and y stay FP64.
 constexpr int kBjRow = 12;
@@ -616,147 +633,256 @@ constexpr int kBjRow = 12;
 #define BJ_AHEAD_F64 4
 #endif
 
-// After a factorisation: copy the first kBjRow entries of every slot's row part
-// (zero beyond the row's count) and, for U, the row's inverted diagonal into
-// the level-ordered solve layout.
+// After a factorisation: the packed values (src < 0: padding, zero) and, for
+// U, every slot's inverted diagonal.
 template <typename T>
-__global__ void bj_pack(long S, const int *__restrict__ k0, const unsigned short *__restrict__ kn,
-                        const T *__restrict__ lu, T *__restrict__ val, T *__restrict__ dinv) {
-    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (s >= S) return;
-    const int a = k0[s], n = kn[s];
-#pragma unroll
-    for (int u = 0; u < kBjRow; ++u) val[s * kBjRow + u] = u < n ? lu[a + u] : T(0);
-    if (dinv) dinv[s] = lu[a - 1];  // U part starts right after the diagonal
+__global__ void bj_pack_vals(long n, const int *__restrict__ src, const T *__restrict__ lu, T *__restrict__ val) {
+    long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int q = src[k];
+    val[k] = q >= 0 ? lu[q] : T(0);
 }
 
 template <typename T>
-struct BjSlot {
-    int r, k0, kn;
+struct BjWin {
     T a[kBjRow];
-    unsigned short c[kBjRow];
-    T d;
+    unsigned c[kBjRow / 2];  // local columns, two per word
+    unsigned m0, m1;         // meta words
+    T d;                     // inverted diagonal (U)
+    bool have;
+};
+struct BjMeta {
+    unsigned m0, m1;  // start (low 16) | overflow offset (high 16); row (low 16) | length (high 16)
+    int s;            // slot
+    bool have;
 };
 
 template <typename T, bool UPPER>
-__device__ __forceinline__ void bj_slot_load(BjSlot<T> &w, long s, const unsigned short *__restrict__ rows,
-                                             const unsigned short *__restrict__ col, const T *__restrict__ val,
-                                             const int *__restrict__ k0, const unsigned short *__restrict__ kn,
-                                             const T *__restrict__ dinv) {
-    w.r = rows[s];
-    w.k0 = k0[s];
-    w.kn = kn[s];
+__device__ __forceinline__ void bj_win_load(BjWin<T> &w, const BjMeta &m, const T *__restrict__ val,
+                                            const unsigned short *__restrict__ col, const T *__restrict__ dinv) {
+    const long k = (long)(m.m0 & 0xffffu);  // even: the window starts 2-entry aligned
+    // values: 8-byte (FP32) or 16-byte (FP64) aligned pairs
+    if (sizeof(T) == 4) {
+        const uint2 *p = reinterpret_cast<const uint2 *>(val + k);
+#pragma unroll
+        for (int i = 0; i < kBjRow / 2; ++i) {
+            const uint2 q = p[i];
+            w.a[2 * i] = (T)__uint_as_float(q.x);
+            w.a[2 * i + 1] = (T)__uint_as_float(q.y);
+        }
+    } else {
+        const double2 *p = reinterpret_cast<const double2 *>(val + k);
+#pragma unroll
+        for (int i = 0; i < kBjRow / 2; ++i) {
+            const double2 q = p[i];
+            w.a[2 * i] = (T)q.x;
+            w.a[2 * i + 1] = (T)q.y;
+        }
+    }
+    const unsigned *pc = reinterpret_cast<const unsigned *>(col + k);
+#pragma unroll
+    for (int i = 0; i < kBjRow / 2; ++i) w.c[i] = pc[i];
+    w.m0 = m.m0;
+    w.m1 = m.m1;
+    w.d = UPPER ? dinv[m.s] : T(0);
+    w.have = m.have;
+}
+
+// The row's entries are subtracted in column order, the overflow ones (from
+// LDS) after the first kBjRow, as the sequential solve does.
+// OVG: the overflow entries are read from the packed arrays in memory (blocks
+// whose overflow lists do not fit the LDS) instead of from LDS
+template <typename T, bool UPPER, bool OVG>
+__device__ __forceinline__ void bj_win_update(const BjWin<T> &w, T *v, const uint4 *ovp, const T *__restrict__ val,
+                                              const unsigned short *__restrict__ col) {
+    const int r = (int)(w.m1 & 0xffffu), kn = (int)(w.m1 >> 16);
+    T s = v[r];
 #pragma unroll
     for (int u = 0; u < kBjRow; ++u) {
-        w.a[u] = val[s * kBjRow + u];
-        w.c[u] = col[s * kBjRow + u];
+        // past the row's length the window holds the following rows' entries
+        // (possibly of another block): masked to 0 * v[r]
+        const unsigned cw = w.c[u / 2];
+        const int c = u < kn ? (int)((u & 1) ? (cw >> 16) : (cw & 0xffffu)) : r;
+        const T a = u < kn ? w.a[u] : T(0);
+        s -= a * v[c];
     }
-    w.d = UPPER ? dinv[s] : T(0);
-}
-
-template <typename T, bool UPPER>
-__device__ __forceinline__ void bj_slot_update(const BjSlot<T> &w, T *v, const unsigned short *__restrict__ ja,
-                                               const T *__restrict__ lu) {
-    T s = v[w.r];
-#pragma unroll
-    for (int u = 0; u < kBjRow; ++u) s -= w.a[u] * v[w.c[u]];
-    for (int k = w.k0 + kBjRow; k < w.k0 + w.kn; ++k) s -= lu[k] * v[ja[k]];
-    if (UPPER) s *= w.d;  // inverted diagonal
-    v[w.r] = s;
-}
-
-// One triangle of one block: thread t takes the t-th slot of each level; the
-// slots of the next kBjAhead levels are loaded ahead (their addresses do not
-// depend on the solution), so the level chain waits only on LDS.
-// levels loaded ahead: deep enough to cover a memory round trip with the short
-// per-level arithmetic; FP64 slots are larger, so fewer of them fit the registers
-template <typename T>
-constexpr int bj_ahead() { return sizeof(T) == 4 ? BJ_AHEAD_F32 : BJ_AHEAD_F64; }
-
-template <typename T, bool UPPER>
-__device__ __forceinline__ void bj_sweep(T *v, int l0, int l1, const int *__restrict__ lptr,
-                                         const unsigned short *__restrict__ rows,
-                                         const unsigned short *__restrict__ col, const T *__restrict__ val,
-                                         const int *__restrict__ k0, const unsigned short *__restrict__ kn,
-                                         const T *__restrict__ dinv, const unsigned short *__restrict__ ja,
-                                         const T *__restrict__ lu) {
-    const int t = threadIdx.x;
-    if (l0 >= l1) return;
-    constexpr int kBjAhead = bj_ahead<T>();
-    BjSlot<T> ring[kBjAhead];
-    bool have[kBjAhead];
-    // Every lane loads a slot for every level, unconditionally (lanes past the
-    // level's width, and levels past the last, load a valid slot they do not
-    // use): with no branch around the loads the compiler can count the
-    // outstanding loads exactly and wait only for the slot it consumes,
-    // instead of for all of them.
-    auto fetch = [&](int l, BjSlot<T> &w) {
-        const int lc = min(l, l1 - 1);
-        const int p0 = lptr[lc], cnt = lptr[lc + 1] - p0;
-        bj_slot_load<T, UPPER>(w, p0 + min(t, cnt - 1), rows, col, val, k0, kn, dinv);
-        return l < l1 && t < cnt;
-    };
-#pragma unroll
-    for (int d = 0; d < kBjAhead; ++d) have[d] = fetch(l0 + d, ring[d]);
-    for (int lb = l0; lb < l1; lb += kBjAhead) {
-#pragma unroll
-        for (int d = 0; d < kBjAhead; ++d) {
-            const int l = lb + d;
-            if (l < l1) {
-                const int p0 = lptr[l], cnt = lptr[l + 1] - p0;
-                if (have[d]) bj_slot_update<T, UPPER>(ring[d], v, ja, lu);
-                for (int tt = t + 64; tt < cnt; tt += 64) {  // levels wider than a wavefront
-                    BjSlot<T> w;
-                    bj_slot_load<T, UPPER>(w, p0 + tt, rows, col, val, k0, kn, dinv);
-                    bj_slot_update<T, UPPER>(w, v, ja, lu);
-                }
-                have[d] = fetch(l + kBjAhead, ring[d]);  // refill this ring slot
-                // one wavefront: the LDS executes its accesses in order, so only
-                // the compiler must keep them in program order
-                wave_lds_order();
+    if (kn > kBjRow) {
+        if (OVG) {  // overflow entries straight after the window in the packed arrays
+            const int k0 = (int)(w.m0 & 0xffffu);
+            for (int k = k0 + kBjRow; k < k0 + kn; ++k) s -= val[k] * v[col[k]];
+        } else {
+            const int oo = (int)(w.m0 >> 16);
+            for (int k = oo; k < oo + kn - kBjRow; ++k) {
+                const uint4 q = ovp[k];
+                T a;
+                if (sizeof(T) == 4) a = (T)__uint_as_float(q.x);
+                else a = (T)__longlong_as_double((long long)(((unsigned long long)q.y << 32) | q.x));
+                s -= a * v[q.z];
             }
         }
     }
+    if (UPPER) s *= w.d;  // inverted diagonal
+    v[r] = s;
+}
+
+// levels loaded ahead: deep enough to cover a memory round trip with the short
+// per-level arithmetic; FP64 windows are larger, so fewer of them fit the registers
+template <typename T>
+constexpr int bj_ahead() { return sizeof(T) == 4 ? BJ_AHEAD_F32 : BJ_AHEAD_F64; }
+
+// One triangle of one block: thread t takes the t-th slot of each level (the
+// solve's levels hold at most 64 slots).  No global load with a
+// data-dependent count is issued inside the level loop, and every pass runs
+// all kBjAhead levels (those past the last have no lane with work), so every
+// path into a level has issued the same loads after that level's own and the
+// compiler's wait counts stay exact.
+template <typename T, bool UPPER, bool OVG>
+__device__ __forceinline__ void bj_sweep(T *v, int nl, const int *lp, const uint2 *__restrict__ meta,
+                                         const T *__restrict__ val, const unsigned short *__restrict__ col,
+                                         const T *__restrict__ dinv, const uint4 *ovp) {
+    const int t = threadIdx.x;
+    if (nl <= 0) return;
+    constexpr int kA = bj_ahead<T>();
+    BjMeta mr[kA];
+    BjWin<T> wr[kA];
+    auto fetch_meta = [&](int l, BjMeta &m) {
+        const int lc = min(l, nl - 1);
+        const int p0 = lp[lc], p1 = lp[lc + 1];
+        const int cnt = p1 - p0;
+        m.s = p0 + min(t, cnt - 1);
+        const uint2 q = meta[m.s];
+        m.m0 = q.x;
+        m.m1 = q.y;
+        m.have = l < nl && t < cnt;
+    };
+    // issued stage by stage and ring slot by ring slot (the barriers keep the
+    // scheduler from moving a slot's loads behind the next slot's)
+#pragma unroll
+    for (int d = 0; d < kA; ++d) {
+        fetch_meta(d, mr[d]);
+        wave_lds_order();
+    }
+#pragma unroll
+    for (int d = 0; d < kA; ++d) {
+        bj_win_load<T, UPPER>(wr[d], mr[d], val, col, dinv);
+        fetch_meta(d + kA, mr[d]);
+        wave_lds_order();
+    }
+    for (int lb = 0; lb < nl; lb += kA) {
+#pragma unroll
+        for (int d = 0; d < kA; ++d) {
+            if (wr[d].have) bj_win_update<T, UPPER, OVG>(wr[d], v, ovp, val, col);
+            bj_win_load<T, UPPER>(wr[d], mr[d], val, col, dinv);  // level lb + d + kA
+            fetch_meta(lb + d + 2 * kA, mr[d]);
+            // one wavefront: the LDS executes its accesses in order, so only
+            // the compiler must keep them in program order
+            wave_lds_order();
+        }
+    }
+}
+
+// LDS bytes of one block's solve: the solution (T x maxrows), the block's
+// overflow entries (16 bytes each; none with OVG), its L and U level pointers
+__host__ __device__ inline size_t bj_solve_lds(size_t tsize, int maxrows, int ovmax, int levl, int levu) {
+    return ((tsize * (size_t)maxrows + 15) & ~(size_t)15) + 16 * (size_t)ovmax + 4 * (size_t)(levl + levu + 2);
 }
 
 // rowid: global row of each local row (nullptr: block b's rows are the
 // contiguous range starting at roff[b]); own: the block's owned local rows,
 // the only ones written back (restricted additive Schwarz).  One wavefront per
-// block with only the solution in LDS, so that every block of the matrix is
-// resident at once.  (Staging the whole block in LDS was measured slower: it
-// limits the resident blocks, and one wavefront stages too slowly.)
-template <typename T>
+// block, so that every block of the matrix is resident at once.  (Staging the
+// whole block in LDS was measured slower: it limits the resident blocks, and
+// one wavefront stages too slowly.)  Per triangle: solve levels blev (per
+// block) into lptr (per level, into the level-ordered slots, at most 64 per
+// level), per slot its meta word pair, the packed values and local columns
+// from the block's entry base ebase, U's inverted diagonals; the block's
+// overflow entries are the packed entries at ov_pos[ov_ptr[b] ..).
+template <typename T, bool OVG>
 __global__ void __launch_bounds__(64) bj_solve(const int *__restrict__ roff, const int *__restrict__ rowid,
-                                               const int *__restrict__ own, const unsigned short *__restrict__ ja,
-                                               const T *__restrict__ lu, const int *__restrict__ blev_l,
-                                               const int *__restrict__ lptr_l, const unsigned short *__restrict__ rows_l,
-                                               const unsigned short *__restrict__ lcol, const T *__restrict__ lval,
-                                               const int *__restrict__ lk0, const unsigned short *__restrict__ lkn,
+                                               const int *__restrict__ own, const int *__restrict__ blev_l,
+                                               const int *__restrict__ lptr_l, const uint2 *__restrict__ meta_l,
+                                               const long *__restrict__ ebase_l, const T *__restrict__ val_l,
+                                               const unsigned short *__restrict__ col_l,
                                                const int *__restrict__ blev_u, const int *__restrict__ lptr_u,
-                                               const unsigned short *__restrict__ rows_u,
-                                               const unsigned short *__restrict__ ucol, const T *__restrict__ uval,
-                                               const int *__restrict__ uk0, const unsigned short *__restrict__ ukn,
-                                               const T *__restrict__ udinv, const double *__restrict__ x,
+                                               const uint2 *__restrict__ meta_u, const long *__restrict__ ebase_u,
+                                               const T *__restrict__ val_u, const unsigned short *__restrict__ col_u,
+                                               const T *__restrict__ dinv, const int *__restrict__ ov_ptr,
+                                               const long *__restrict__ ov_pos, const unsigned char *__restrict__ ov_tri,
+                                               int maxrows, int ovmax, const double *__restrict__ x,
                                                const double *__restrict__ xdiv, double *__restrict__ y,
                                                double *__restrict__ yn2) {
     extern __shared__ double smem[];
     const int b = blockIdx.x;
     const int base = roff[b];
     const int cnt = roff[b + 1] - base;
-    T *v = (T *)smem;
-    for (int r = threadIdx.x; r < cnt; r += 64) {
-        const int g = rowid ? rowid[base + r] : base + r;
-        v[r] = (T)(xdiv ? x[g] / xdiv[g] : x[g]);  // optional row scaling x / xdiv
+    char *lds = reinterpret_cast<char *>(smem);
+    T *v = reinterpret_cast<T *>(lds);
+    uint4 *ovp = reinterpret_cast<uint4 *>(lds + ((sizeof(T) * (size_t)maxrows + 15) & ~(size_t)15));
+    // stage x (optionally scaled by 1 / xdiv) in batches of kBjStage rows per
+    // lane, so that a batch's gathers are in flight together
+    constexpr int kBjStage = 8;
+    for (int r0 = threadIdx.x; r0 < cnt; r0 += 64 * kBjStage) {
+        int g[kBjStage];
+        double xv[kBjStage], dv[kBjStage];
+#pragma unroll
+        for (int k = 0; k < kBjStage; ++k) {
+            const int r = min(r0 + 64 * k, cnt - 1);
+            g[k] = rowid ? rowid[base + r] : base + r;
+        }
+#pragma unroll
+        for (int k = 0; k < kBjStage; ++k) {
+            xv[k] = x[g[k]];
+            dv[k] = xdiv ? xdiv[g[k]] : 1.;
+        }
+#pragma unroll
+        for (int k = 0; k < kBjStage; ++k) {
+            const int r = r0 + 64 * k;
+            if (r < cnt) v[r] = (T)(xdiv ? xv[k] / dv[k] : xv[k]);
+        }
     }
+    // overflow entries: value bits, local column
+    const int ob = ov_ptr[b], on = OVG ? 0 : ov_ptr[b + 1] - ob;
+    if (OVG) ovmax = 0;
+    for (int k = threadIdx.x; k < on; k += 64) {
+        const long e = ov_pos[ob + k];
+        const bool up = ov_tri[ob + k];
+        const T a = up ? val_u[e] : val_l[e];
+        unsigned lo, hi = 0u;
+        if (sizeof(T) == 4) lo = __float_as_uint((float)a);
+        else {
+            const unsigned long long u = (unsigned long long)__double_as_longlong((double)a);
+            lo = (unsigned)u;
+            hi = (unsigned)(u >> 32);
+        }
+        ovp[k] = make_uint4(lo, hi, (unsigned)(up ? col_u[e] : col_l[e]), 0u);
+    }
+    int *lpl = reinterpret_cast<int *>(ovp + ovmax);
+    const int l0 = blev_l[b], nl = blev_l[b + 1] - l0;
+    const int u0 = blev_u[b], nu = blev_u[b + 1] - u0;
+    int *lpu = lpl + nl + 1;
+    for (int k = threadIdx.x; k <= nl; k += 64) lpl[k] = lptr_l[l0 + k];
+    for (int k = threadIdx.x; k <= nu; k += 64) lpu[k] = lptr_u[u0 + k];
     wave_lds_order();
-    bj_sweep<T, false>(v, blev_l[b], blev_l[b + 1], lptr_l, rows_l, lcol, lval, lk0, lkn, nullptr, ja, lu);
-    bj_sweep<T, true>(v, blev_u[b], blev_u[b + 1], lptr_u, rows_u, ucol, uval, uk0, ukn, udinv, ja, lu);
+    bj_sweep<T, false, OVG>(v, nl, lpl, meta_l, val_l + ebase_l[b], col_l + ebase_l[b], nullptr, ovp);
+    bj_sweep<T, true, OVG>(v, nu, lpu, meta_u, val_u + ebase_u[b], col_u + ebase_u[b], dinv, ovp);
     const int olo = own[2 * b], ohi = own[2 * b + 1];
     double q = 0.;
-    for (int r = olo + threadIdx.x; r < ohi; r += 64) {
-        const double yr = (double)v[r];
-        y[rowid ? rowid[base + r] : base + r] = yr;
-        q += yr * yr;
+    for (int r0 = olo + threadIdx.x; r0 < ohi; r0 += 64 * kBjStage) {
+        int g[kBjStage];
+#pragma unroll
+        for (int k = 0; k < kBjStage; ++k) {
+            const int r = min(r0 + 64 * k, ohi - 1);
+            g[k] = rowid ? rowid[base + r] : base + r;
+        }
+#pragma unroll
+        for (int k = 0; k < kBjStage; ++k) {
+            const int r = r0 + 64 * k;
+            if (r < ohi) {
+                const double yr = (double)v[r];
+                y[g[k]] = yr;
+                q += yr * yr;
+            }
+        }
     }
     if (yn2) {  // this block's share of ||y||^2 (owned rows partition the rows)
         q = wave_reduce(q);
@@ -1138,33 +1264,91 @@ static int build_bj(fvp_csr *A) {
     if (ident) A->b_rowid.release();
     A->b_lu32.release();
     {
-        // slot tables of the solve (static part: columns padded with the row itself, first entry, count)
-        const size_t S = (size_t)total;
-        std::vector<unsigned short> lc(S * kBjRow), uc(S * kBjRow), lkn(S), ukn(S);
-        std::vector<int> lk0(S), uk0(S);
+        // packed solve layout (see bj_solve)
+        std::vector<int> ovp(nb + 1, 0);
+        std::vector<long> ovpos;
+        std::vector<uint8_t> ovtri;
+        std::vector<int> dsr;
+        struct Tri {
+            std::vector<unsigned> meta;
+            std::vector<long> ebase;
+            std::vector<int> esrc;
+            std::vector<unsigned short> col;
+        } tr[2];
+        for (int t = 0; t < 2; ++t) tr[t].ebase.assign(nb + 1, 0);
+        int ovmax = 0;
         for (int b = 0; b < nb; ++b) {
-            const int base = roff[b], cnt = roff[b + 1] - base;
-            const int pl = lptr_l[blev_l[b]], pu = lptr_u[blev_u[b]];
-            for (int q = 0; q < cnt; ++q) {
-                for (int tri = 0; tri < 2; ++tri) {
-                    const size_t sl = (size_t)(tri ? pu : pl) + q;
-                    const int r = tri ? rows_u[sl] : rows_l[sl];
+            const int base = roff[b];
+            int nov = 0;
+            for (int tri = 0; tri < 2; ++tri) {
+                Tri &T = tr[tri];
+                const long eb = (long)T.esrc.size();
+                T.ebase[b] = eb;
+                const std::vector<int> &lptr = tri ? lptr_u : lptr_l;
+                const std::vector<int> &blev = tri ? blev_u : blev_l;
+                const std::vector<unsigned short> &rows = tri ? rows_u : rows_l;
+                for (int sl = lptr[blev[b]]; sl < lptr[blev[b + 1]]; ++sl) {
+                    const int r = rows[sl];
                     const int i = base + r;
                     const int k0 = tri ? bdiag[i] + 1 : bia[i], k1 = tri ? bia[i + 1] : bdiag[i];
-                    if (k1 - k0 > 65535) { set_error("block row too long"); return -1; }
-                    (tri ? uk0 : lk0)[sl] = k0;
-                    (tri ? ukn : lkn)[sl] = (unsigned short)(k1 - k0);
-                    for (int u = 0; u < kBjRow; ++u)
-                        (tri ? uc : lc)[sl * kBjRow + u] = (unsigned short)(k0 + u < k1 ? bja[k0 + u] : r);
+                    const int kn = k1 - k0;
+                    const long kl = (long)T.esrc.size() - eb;
+                    if (kl > 65535 || nov > 65535) {
+                        set_error("block-Jacobi ILU(0): more than 65535 entries in one block triangle");
+                        return -1;
+                    }
+                    T.meta.push_back((unsigned)kl | ((unsigned)nov << 16));
+                    T.meta.push_back((unsigned)r | ((unsigned)kn << 16));
+                    for (int k = k0; k < k1; ++k) {
+                        if (k - k0 >= kBjRow) {
+                            ovpos.push_back((long)T.esrc.size());
+                            ovtri.push_back((uint8_t)tri);
+                            ++nov;
+                        }
+                        T.esrc.push_back(k);
+                        T.col.push_back(bja[k]);
+                    }
+                    if (kn & 1) { T.esrc.push_back(-1); T.col.push_back((unsigned short)r); }
+                    if (tri) dsr.push_back(bdiag[i]);
                 }
             }
+            ovp[b + 1] = ovp[b] + nov;
+            ovmax = std::max(ovmax, nov);
         }
-        if (A->sl_col.upload(lc.data(), lc.size()) || A->su_col.upload(uc.data(), uc.size()) ||
-            A->sl_k0.upload(lk0.data(), S) || A->su_k0.upload(uk0.data(), S) || A->sl_kn.upload(lkn.data(), S) ||
-            A->su_kn.upload(ukn.data(), S))
+        for (int t = 0; t < 2; ++t) {  // the last windows read up to kBjRow entries past the end
+            tr[t].ebase[nb] = (long)tr[t].esrc.size();
+            for (int k = 0; k < kBjRow; ++k) { tr[t].esrc.push_back(-1); tr[t].col.push_back(0); }
+        }
+        // the solve's levels: each factor level cut into pieces of at most 64 slots
+        auto split = [&](const std::vector<int> &blev, const std::vector<int> &lptr, std::vector<int> &sb,
+                         std::vector<int> &sp, int &levmax) {
+            sb.assign(nb + 1, 0);
+            sp.assign(1, 0);
+            levmax = 0;
+            for (int b = 0; b < nb; ++b) {
+                for (int l = blev[b]; l < blev[b + 1]; ++l)
+                    for (int p = lptr[l]; p < lptr[l + 1]; p += 64) sp.push_back(std::min(p + 64, lptr[l + 1]));
+                sb[b + 1] = (int)sp.size() - 1;
+                levmax = std::max(levmax, sb[b + 1] - sb[b]);
+            }
+        };
+        std::vector<int> sbl, spl, sbu, spu;
+        split(blev_l, lptr_l, sbl, spl, A->s_levmax_l);
+        split(blev_u, lptr_u, sbu, spu, A->s_levmax_u);
+        if (A->meta_l.upload(tr[0].meta.data(), tr[0].meta.size()) ||
+            A->meta_u.upload(tr[1].meta.data(), tr[1].meta.size()) ||
+            A->ebase_l.upload(tr[0].ebase.data(), nb + 1) || A->ebase_u.upload(tr[1].ebase.data(), nb + 1) ||
+            A->esrc_l.upload(tr[0].esrc.data(), tr[0].esrc.size()) ||
+            A->esrc_u.upload(tr[1].esrc.data(), tr[1].esrc.size()) ||
+            A->col_l.upload(tr[0].col.data(), tr[0].col.size()) || A->col_u.upload(tr[1].col.data(), tr[1].col.size()) ||
+            A->dsrc.upload(dsr.data(), dsr.size()) || A->ov_ptr.upload(ovp.data(), nb + 1) ||
+            A->ov_pos.upload(ovpos.data(), ovpos.size()) || A->ov_tri.upload(ovtri.data(), ovtri.size()) ||
+            A->s_lev_l.upload(sbl.data(), nb + 1) || A->s_lev_u.upload(sbu.data(), nb + 1) ||
+            A->s_lptr_l.upload(spl.data(), spl.size()) || A->s_lptr_u.upload(spu.data(), spu.size()))
             return -1;
-        A->sl_val.release(); A->su_val.release(); A->su_dinv.release();
-        A->sl_val32.release(); A->su_val32.release(); A->su_dinv32.release();
+        A->ov_max = ovmax;
+        A->val_l.release(); A->val_u.release(); A->dinv.release();
+        A->val32_l.release(); A->val32_u.release(); A->dinv32.release();
     }
     A->b_maxnz = maxnz;
     A->b_maxrows = maxrows;
@@ -1174,8 +1358,9 @@ static int build_bj(fvp_csr *A) {
     // per-kernel attributes (shared by every matrix): allow the device maximum;
     // each build checks its own block against it
     for (const void *f : {(const void *)bj_factor<double, true>, (const void *)bj_factor<double, false>,
-                          (const void *)bj_solve<double>, (const void *)bj_factor<float, true>,
-                          (const void *)bj_factor<float, false>, (const void *)bj_solve<float>})
+                          (const void *)bj_solve<double, false>, (const void *)bj_solve<double, true>,
+                          (const void *)bj_factor<float, true>, (const void *)bj_factor<float, false>,
+                          (const void *)bj_solve<float, false>, (const void *)bj_solve<float, true>})
         (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, A->lds_max);
     A->nblk = nb;
     A->bj = true;
@@ -1227,22 +1412,18 @@ static int precon_build_impl(fvp_csr *A, int type) {
                     A->b_roff.p, A->b_maxrows, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p,
                     A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p, A->b_lu32.p, A->flag.p);
         }
-        // level-ordered solve layout of the new factor
-        const long S = (long)A->sl_k0.n;
+        // packed solve values of the new factor
+        const long nl = (long)A->esrc_l.n, nu = (long)A->esrc_u.n, nd = (long)A->dsrc.n;
         if (type == FVP_PRECON_BJILU0) {
-            if ((!A->sl_val.p && (A->sl_val.alloc(S * kBjRow) || A->su_val.alloc(S * kBjRow) ||
-                                  A->su_dinv.alloc(S))))
-                return -1;
-            bj_pack<double><<<nblk(S, 256), 256, 0, s>>>(S, A->sl_k0.p, A->sl_kn.p, A->b_lu.p, A->sl_val.p, nullptr);
-            bj_pack<double><<<nblk(S, 256), 256, 0, s>>>(S, A->su_k0.p, A->su_kn.p, A->b_lu.p, A->su_val.p,
-                                                         A->su_dinv.p);
+            if (!A->val_l.p && (A->val_l.alloc(nl) || A->val_u.alloc(nu) || A->dinv.alloc(nd))) return -1;
+            bj_pack_vals<double><<<nblk(nl, 256), 256, 0, s>>>(nl, A->esrc_l.p, A->b_lu.p, A->val_l.p);
+            bj_pack_vals<double><<<nblk(nu, 256), 256, 0, s>>>(nu, A->esrc_u.p, A->b_lu.p, A->val_u.p);
+            bj_pack_vals<double><<<nblk(nd, 256), 256, 0, s>>>(nd, A->dsrc.p, A->b_lu.p, A->dinv.p);
         } else {
-            if ((!A->sl_val32.p && (A->sl_val32.alloc(S * kBjRow) || A->su_val32.alloc(S * kBjRow) ||
-                                    A->su_dinv32.alloc(S))))
-                return -1;
-            bj_pack<float><<<nblk(S, 256), 256, 0, s>>>(S, A->sl_k0.p, A->sl_kn.p, A->b_lu32.p, A->sl_val32.p, nullptr);
-            bj_pack<float><<<nblk(S, 256), 256, 0, s>>>(S, A->su_k0.p, A->su_kn.p, A->b_lu32.p, A->su_val32.p,
-                                                        A->su_dinv32.p);
+            if (!A->val32_l.p && (A->val32_l.alloc(nl) || A->val32_u.alloc(nu) || A->dinv32.alloc(nd))) return -1;
+            bj_pack_vals<float><<<nblk(nl, 256), 256, 0, s>>>(nl, A->esrc_l.p, A->b_lu32.p, A->val32_l.p);
+            bj_pack_vals<float><<<nblk(nu, 256), 256, 0, s>>>(nu, A->esrc_u.p, A->b_lu32.p, A->val32_u.p);
+            bj_pack_vals<float><<<nblk(nd, 256), 256, 0, s>>>(nd, A->dsrc.p, A->b_lu32.p, A->dinv32.p);
         }
     } else if (type == FVP_PRECON_JACOBI) {
         if (A->jac.n != (size_t)n && A->jac.alloc(n)) return -1;
@@ -1296,17 +1477,32 @@ static int precon_apply_impl(fvp_csr *A, const double *x, double *y, const doubl
         vmul<<<nblk(n, 256), 256, 0, s>>>(n, x, A->jac.p, y);
         break;
     case FVP_PRECON_BJILU0:
-        bj_solve<double><<<A->nblk, 64, sizeof(double) * A->b_maxrows, s>>>(
-            A->b_roff.p, A->b_rowid.p, A->b_own.p, A->b_ja.p, A->b_lu.p, A->b_lev_l.p, A->b_lptr_l.p,
-            A->b_rows_l.p, A->sl_col.p, A->sl_val.p, A->sl_k0.p, A->sl_kn.p, A->b_lev_u.p, A->b_lptr_u.p,
-            A->b_rows_u.p, A->su_col.p, A->su_val.p, A->su_k0.p, A->su_kn.p, A->su_dinv.p, x, xdiv, y, yn2);
+    case FVP_PRECON_BJILU0_F32: {
+        const bool f64 = A->ptype == FVP_PRECON_BJILU0;
+        const size_t ts = f64 ? sizeof(double) : sizeof(float);
+        size_t lds = bj_solve_lds(ts, A->b_maxrows, A->ov_max, A->s_levmax_l, A->s_levmax_u);
+        const bool ovg = lds > (size_t)A->lds_max;  // overflow lists too long for the LDS: read them in memory
+        if (ovg) lds = bj_solve_lds(ts, A->b_maxrows, 0, A->s_levmax_l, A->s_levmax_u);
+        if (lds > (size_t)A->lds_max) {
+            set_error("block-Jacobi ILU(0): a block's solution does not fit the LDS");
+            return -1;
+        }
+        const uint2 *ml = reinterpret_cast<const uint2 *>(A->meta_l.p), *mu = reinterpret_cast<const uint2 *>(A->meta_u.p);
+#define FVP_BJ_SOLVE(TT, OV, VL, VU, DI)                                                                          \
+    bj_solve<TT, OV><<<A->nblk, 64, lds, s>>>(A->b_roff.p, A->b_rowid.p, A->b_own.p, A->s_lev_l.p, A->s_lptr_l.p, \
+                                              ml, A->ebase_l.p, VL, A->col_l.p, A->s_lev_u.p, A->s_lptr_u.p, mu, \
+                                              A->ebase_u.p, VU, A->col_u.p, DI, A->ov_ptr.p, A->ov_pos.p,         \
+                                              A->ov_tri.p, A->b_maxrows, A->ov_max, x, xdiv, y, yn2)
+        if (f64) {
+            if (ovg) FVP_BJ_SOLVE(double, true, A->val_l.p, A->val_u.p, A->dinv.p);
+            else FVP_BJ_SOLVE(double, false, A->val_l.p, A->val_u.p, A->dinv.p);
+        } else {
+            if (ovg) FVP_BJ_SOLVE(float, true, A->val32_l.p, A->val32_u.p, A->dinv32.p);
+            else FVP_BJ_SOLVE(float, false, A->val32_l.p, A->val32_u.p, A->dinv32.p);
+        }
+#undef FVP_BJ_SOLVE
         break;
-    case FVP_PRECON_BJILU0_F32:
-        bj_solve<float><<<A->nblk, 64, sizeof(float) * A->b_maxrows, s>>>(
-            A->b_roff.p, A->b_rowid.p, A->b_own.p, A->b_ja.p, A->b_lu32.p, A->b_lev_l.p, A->b_lptr_l.p,
-            A->b_rows_l.p, A->sl_col.p, A->sl_val32.p, A->sl_k0.p, A->sl_kn.p, A->b_lev_u.p, A->b_lptr_u.p,
-            A->b_rows_u.p, A->su_col.p, A->su_val32.p, A->su_k0.p, A->su_kn.p, A->su_dinv32.p, x, xdiv, y, yn2);
-        break;
+    }
     case FVP_PRECON_ILU0: {
         const int L = 8;
         for (size_t l = 0; l + 1 < A->lptr_h.size(); ++l) {
@@ -1373,11 +1569,13 @@ extern "C" int fvp_csr_destroy(fvp_csr *A) {
     for (hipEvent_t e : A->ev) (void)hipEventDestroy(e);
     A->b_ia.release(); A->b_src.release(); A->b_diag.release(); A->b_ja.release(); A->b_lu.release(); A->b_lu32.release();
     A->b_roff.release(); A->b_rowid.release(); A->b_own.release(); A->b_zn2.release();
-    A->sl_col.release(); A->su_col.release(); A->sl_val.release(); A->su_val.release(); A->su_dinv.release();
-    A->sl_val32.release(); A->su_val32.release(); A->su_dinv32.release(); A->sl_k0.release(); A->su_k0.release();
-    A->sl_kn.release(); A->su_kn.release();
+    A->meta_l.release(); A->meta_u.release(); A->ebase_l.release(); A->ebase_u.release(); A->esrc_l.release();
+    A->esrc_u.release(); A->dsrc.release(); A->col_l.release(); A->col_u.release(); A->val_l.release();
+    A->val_u.release(); A->dinv.release(); A->val32_l.release(); A->val32_u.release(); A->dinv32.release();
     A->b_lev_l.release(); A->b_lev_u.release(); A->b_lptr_l.release(); A->b_lptr_u.release();
     A->b_rows_l.release(); A->b_rows_u.release();
+    A->s_lev_l.release(); A->s_lev_u.release(); A->s_lptr_l.release(); A->s_lptr_u.release();
+    A->ov_ptr.release(); A->ov_pos.release(); A->ov_tri.release();
     if (A->own_stream) (void)hipStreamDestroy(A->own_stream);
     delete A;
     return 0;

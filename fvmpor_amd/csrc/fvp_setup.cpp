@@ -60,6 +60,7 @@ template struct DevBuf<double>;
 template struct DevBuf<unsigned short>;
 template struct DevBuf<float>;
 template struct DevBuf<long>;
+template struct DevBuf<unsigned>;
 
 static const int TRI_EDGE[3][2] = {{0, 1}, {1, 2}, {2, 0}};
 static const int TET_EDGE[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};

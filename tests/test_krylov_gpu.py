@@ -358,3 +358,74 @@ def test_overlapping_blocks_are_restricted_additive_schwarz(box_mid, layers):
     rc, st = A.gmres(_dev(torch, b), xs, m=30, max_restarts=40, tol=1e-12, reorth=1)
     assert rc == 0 and st["converged"]
     assert np.abs(xs.cpu().numpy() - 1.0).max() < 1e-9
+
+
+def _wide_pattern_matrix(mesh, rings, seed):
+    """Diagonally dominant values on the pattern of (adjacency + I)^rings: rows
+    with many more in-block entries than the usual stencil (the solve's
+    overflow lists carry everything past its first 12 entries per row part)."""
+    from fvmpor_amd.learn import node_graph
+    ia0, ja0 = node_graph(mesh)
+    n = len(ia0) - 1
+    G = sp.csr_matrix((np.ones(len(ja0)), ja0, ia0), shape=(n, n)) + sp.identity(n, format="csr")
+    P = G.copy()
+    for _ in range(rings - 1):
+        P = (P @ G).tocsr()
+    P.sort_indices()
+    ia, ja = P.indptr.astype(np.int32), P.indices.astype(np.int32)
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(-1, 0, len(ja))
+    rows = np.repeat(np.arange(n), np.diff(ia))
+    diag = rows == ja
+    a[diag] = 1.2 * np.bincount(rows, weights=np.abs(a) * ~diag, minlength=n)[rows[diag]]
+    return ia, ja, a
+
+
+@pytest.mark.parametrize("ptype", ["BJILU0", "BJILU0_F32"])
+def test_block_solve_long_rows_and_wide_levels(ptype):
+    """Two structures the block solve treats specially, against the oracle's
+    blockwise ILU(0): rows whose L or U part has more than 12 block entries
+    (a 3-ring stencil on the 2D slice), and levels wider than a wavefront (a
+    matrix whose blocks are diagonal, with couplings only between blocks, so
+    every block is one level of 1024 rows)."""
+    import torch
+    from fvmpor_amd import lib as L
+    from fvmpor_amd.problems import slice2d_problem
+    pt = getattr(L, "PRECON_" + ptype)
+    tol = 1e-11 if ptype == "BJILU0" else 2e-5
+    mesh = slice2d_problem().mesh
+    ia, ja, a = _wide_pattern_matrix(mesh, 3, seed=4)
+    n = len(ia) - 1
+    bs = 1024
+    # couplings across blocks only: row i also couples to i +- bs
+    r = np.arange(n)
+    cols = [np.stack([r, r]).T]
+    for off in (-bs, bs):
+        ok = (r + off >= 0) & (r + off < n)
+        cols.append(np.stack([r[ok], r[ok] + off]).T)
+    pairs = np.concatenate(cols)
+    W = sp.csr_matrix((np.ones(len(pairs)), (pairs[:, 0], pairs[:, 1])), shape=(n, n))
+    W.sort_indices()
+    wv = np.where(W.indices == np.repeat(r, np.diff(W.indptr)), 3.0, -0.5)
+    cases = [("long rows", ia, ja, a), ("wide levels", W.indptr.astype(np.int32), W.indices.astype(np.int32), wv)]
+    for name, cia, cja, ca in cases:
+        rows = np.repeat(np.arange(n), np.diff(cia))
+        inb = (rows // bs) == (cja // bs)
+        lo_cnt = np.bincount(rows[inb & (cja < rows)], minlength=n)
+        A = L.CsrMatrix(cia, cja)
+        A.precon_block_rows(bs)
+        A.set_values(_dev(torch, ca))
+        assert A.precon_build(pt) == 0
+        bia, bja, ba, keep = _block_diagonal(cia, cja, ca, bs)
+        lu_o, dp = O.ilu0(bia, bja, ba)
+        x = np.random.default_rng(9).normal(size=n)
+        y = torch.empty(n, dtype=torch.float64, device=device())
+        A.precon_apply(_dev(torch, x), y)
+        torch.cuda.synchronize()
+        ref = O.ilu0_apply(bia, bja, lu_o, dp, x)
+        err = np.abs(y.cpu().numpy() - ref).max() / np.abs(ref).max()
+        print(f"{ptype} {name}: max L entries per row {lo_cnt.max()}, rel err {err:.2e}")
+        assert err <= tol
+        if name == "long rows":
+            assert lo_cnt.max() > 12
+        A.close()
