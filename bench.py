@@ -79,6 +79,8 @@ def parse():
     ap.add_argument("--no-fp32", action="store_true",
                     help="headline run with the FP64 preconditioner (no FP32 variant)")
     ap.add_argument("--no-full", action="store_true", help="skip the full runs to t = 1 day")
+    ap.add_argument("--order", default="tile", choices=["tile", "rcm", "lattice"],
+                    help="node numbering of the box: compact tiles (default), reverse Cuthill-McKee, generator order")
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU oracle's Newton-GMRES sample")
     ap.add_argument("--no-learned", action="store_true", help="headline run without the learned initial iterate")
     ap.add_argument("--train-nz", type=int, default=23, help="training box lattice 4nz x 4nz x nz")
@@ -301,7 +303,7 @@ def main():
 
     params, train_info = (None, None) if args.no_learned else train_network(args)
     t0 = time.perf_counter()
-    prob = box3d_problem(nx=args.nx, ny=args.ny, nz=args.nz)
+    prob = box3d_problem(nx=args.nx, ny=args.ny, nz=args.nz, order=args.order)
     log(f"mesh N={prob.mesh.n_nodes} T={prob.mesh.n_elements} built in {time.perf_counter() - t0:.1f}s")
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
@@ -396,7 +398,7 @@ def main():
                         f"Newton-GMRES, block-Jacobi ILU(0) ({args.block_rows or BLOCK_ROWS}-row blocks, "
                         f"{args.overlap} overlap layer(s)), rtol=atol={args.rtol:g}; headline: {head_name}, "
                         f"timed steps {args.warmup + 1}..{args.warmup + args.steps} from t=0",
-            "nodes": N, "elements": T, "edges": NE, "precon_plain": args.precon,
+            "nodes": N, "elements": T, "edges": NE, "precon_plain": args.precon, "node_order": args.order,
             "block_rows": args.block_rows or BLOCK_ROWS, "overlap": args.overlap,
             "sim_days_timed": days, "t_start": head["t0"], "t_end": head["t1"],
             "newton_iters": head["d"]["newton_iters"], "gmres_iters": head["d"]["gmres_iters"],

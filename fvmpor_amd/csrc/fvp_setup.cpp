@@ -636,3 +636,112 @@ extern "C" int fvp_set_bc(fvp_ctx *c, const double *bc) {
     c->bc.assign(bc, bc + (size_t)c->ntags * FVP_BC_FIELDS);
     return upload_bc_values(c) || upload_dirichlet_values(c) ? -1 : 0;
 }
+
+// ---------------------------------------------------------------------------
+// Reverse Cuthill-McKee renumbering of the node graph of a simplex mesh
+// (Mesh::reorder_nodes_edges and find_RCM_from_edges, mesh.cpp:118-209,
+// 1571-1615, which call Boost's cuthill_mckee_ordering and reverse it).  Per
+// connected component: a pseudo-peripheral start node (George and Liu: from
+// the component's lowest-degree node, breadth-first level structures, each
+// time restarting from the lowest-degree node of the last level while the
+// eccentricity grows), then a breadth-first sweep that appends each node's
+// unvisited neighbours in increasing degree (ties in adjacency order); the
+// concatenated order is reversed.  Degrees count the self loop the reference
+// adds to every node (mesh.cpp:127-128).
+namespace {
+
+void node_graph(int N, int T, int npe, const int *conn, std::vector<int> &ia, std::vector<int> &ja) {
+    std::vector<std::pair<int, int>> pr;
+    pr.reserve((size_t)T * npe * (npe - 1));
+    for (int e = 0; e < T; ++e)
+        for (int a = 0; a < npe; ++a)
+            for (int b = 0; b < npe; ++b)
+                if (a != b) pr.emplace_back(conn[(size_t)e * npe + a], conn[(size_t)e * npe + b]);
+    std::sort(pr.begin(), pr.end());
+    pr.erase(std::unique(pr.begin(), pr.end()), pr.end());
+    ia.assign(N + 1, 0);
+    ja.resize(pr.size());
+    for (size_t k = 0; k < pr.size(); ++k) {
+        ia[pr[k].first + 1]++;
+        ja[k] = pr[k].second;
+    }
+    for (int i = 0; i < N; ++i) ia[i + 1] += ia[i];
+}
+
+// breadth-first level structure from s over unvisited nodes: returns the
+// eccentricity; last = the last level's nodes
+int bfs_levels(int s, const std::vector<int> &ia, const std::vector<int> &ja, const std::vector<char> &done,
+               std::vector<int> &stamp, int mark, std::vector<int> &last) {
+    std::vector<int> cur{s}, next;
+    stamp[s] = mark;
+    int ecc = 0;
+    for (;;) {
+        next.clear();
+        for (int v : cur)
+            for (int k = ia[v]; k < ia[v + 1]; ++k) {
+                const int w = ja[k];
+                if (!done[w] && stamp[w] != mark) { stamp[w] = mark; next.push_back(w); }
+            }
+        if (next.empty()) break;
+        cur.swap(next);
+        ++ecc;
+    }
+    last = cur;
+    return ecc;
+}
+
+}  // namespace
+
+extern "C" int fvp_rcm(int n_nodes, int n_elements, int nodes_per_element, const int *conn, int *perm) {
+    if (n_nodes < 0 || n_elements < 0 || nodes_per_element < 2 || nodes_per_element > 4 || !perm ||
+        (n_elements > 0 && !conn)) {
+        set_error("fvp_rcm: invalid arguments");
+        return -1;
+    }
+    const int N = n_nodes;
+    for (long k = 0; k < (long)n_elements * nodes_per_element; ++k)
+        if (conn[k] < 0 || conn[k] >= N) { set_error("fvp_rcm: node index out of range"); return -1; }
+    std::vector<int> ia, ja;
+    node_graph(N, n_elements, nodes_per_element, conn, ia, ja);
+    std::vector<int> deg(N);
+    for (int i = 0; i < N; ++i) deg[i] = ia[i + 1] - ia[i] + 1;  // self loop
+    std::vector<char> done(N, 0);
+    std::vector<int> stamp(N, -1), order, last, nbr;
+    order.reserve(N);
+    // nodes by increasing degree (ties by index): the first unvisited one seeds a component
+    std::vector<int> bydeg(N);
+    std::iota(bydeg.begin(), bydeg.end(), 0);
+    std::stable_sort(bydeg.begin(), bydeg.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+    int mark = 0;
+    for (int seed : bydeg) {
+        if (done[seed]) continue;
+        // pseudo-peripheral node
+        int s = seed;
+        int ecc = bfs_levels(s, ia, ja, done, stamp, mark++, last);
+        for (;;) {
+            int best = last[0];
+            for (int v : last)
+                if (deg[v] < deg[best] || (deg[v] == deg[best] && v < best)) best = v;
+            const std::vector<int> prev = last;
+            const int e2 = bfs_levels(best, ia, ja, done, stamp, mark++, last);
+            if (e2 <= ecc) { last = prev; break; }
+            s = best;
+            ecc = e2;
+        }
+        // Cuthill-McKee sweep of the component
+        size_t head = order.size();
+        order.push_back(s);
+        done[s] = 1;
+        while (head < order.size()) {
+            const int v = order[head++];
+            nbr.clear();
+            for (int k = ia[v]; k < ia[v + 1]; ++k)
+                if (!done[ja[k]]) { done[ja[k]] = 1; nbr.push_back(ja[k]); }
+            std::stable_sort(nbr.begin(), nbr.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+            order.insert(order.end(), nbr.begin(), nbr.end());
+        }
+    }
+    // reversed: new node i is old node perm[i]
+    for (int i = 0; i < N; ++i) perm[i] = order[N - 1 - i];
+    return 0;
+}

@@ -26,6 +26,7 @@ _c_dbl_p = ctypes.POINTER(ctypes.c_double)
 # name -> (restype, argtypes) for every symbol in include/fvp.h
 SIGNATURES = {
     "fvp_last_error": (ctypes.c_char_p, []),
+    "fvp_rcm": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p]),
     "fvp_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     "fvp_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "fvp_sizes": (ctypes.c_int, [ctypes.c_void_p, _c_int_p]),
@@ -161,6 +162,16 @@ def _ptr(x):
     if isinstance(x, int):
         return ctypes.c_void_p(x)
     return ctypes.c_void_p(x.data_ptr())
+
+
+def rcm_order(mesh) -> np.ndarray:
+    """Reverse Cuthill-McKee order of the mesh's nodes (fvp_rcm): node i of the
+    renumbered mesh is node perm[i] of this one (meshgen.renumber_nodes)."""
+    conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+    perm = np.empty(mesh.n_nodes, dtype=np.int32)
+    _check(lib().fvp_rcm(mesh.n_nodes, conn.shape[0], conn.shape[1], conn.ctypes.data_as(_c_int_p),
+                         perm.ctypes.data_as(_c_int_p)), "fvp_rcm")
+    return perm
 
 
 class Context:

@@ -82,16 +82,27 @@ def slice2d_problem(nx=101, ny=41, seed=1) -> Problem:
 
 
 def box3d_problem(nx=160, ny=160, nz=40, seed_mesh=2, seed_field=3, seed_layers=4,
-                  sigma_lnK=1.0, mean_lnK=float(np.log(0.5)), rain=0.02, mesh=None, tiled=True) -> Problem:
+                  sigma_lnK=1.0, mean_lnK=float(np.log(0.5)), rain=0.02, mesh=None, tiled=True,
+                  order=None) -> Problem:
     """3D box 100 x 100 x 20 m, log-normal Ks (exponential covariance, 10 m / 10 m /
     2 m correlation lengths), four van Genuchten classes in random horizontal
     layers, water table at z = 5 m, rain on the whole top face, to t = 1 day.
-    ``tiled`` renumbers the generated nodes with ``tile_order``."""
+    Node order of a generated mesh: ``order`` = "tile" (``tile_order``: compact
+    ~1024-node tiles, the default), "rcm" (reverse Cuthill-McKee, as the
+    reference renumbers at mesh load) or "lattice" (the generator's own);
+    ``tiled=False`` is "lattice"."""
     lx, ly, lz = 100.0, 100.0, 20.0
+    if order is None:
+        order = "tile" if tiled else "lattice"
     if mesh is None:
         mesh = box_mesh_3d(nx, ny, nz, lx, ly, lz, seed=seed_mesh)
-        if tiled:  # node ids grouped into compact ~1024-node tiles (locality of the residual / block precon)
+        if order == "tile":  # node ids grouped into compact tiles (locality of the residual / block precon)
             mesh = renumber_nodes(mesh, tile_order(mesh.xyz, 3))
+        elif order == "rcm":
+            from .lib import rcm_order
+            mesh = renumber_nodes(mesh, rcm_order(mesh))
+        elif order != "lattice":
+            raise ValueError(f"unknown node order {order!r}")
     p = mesh.xyz[mesh.conn]
     cen = p.mean(axis=1)
     gshape = (nx, ny, nz)

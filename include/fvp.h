@@ -52,6 +52,15 @@ typedef struct {
 
 const char *fvp_last_error(void);
 
+/* Reverse Cuthill-McKee node order of a simplex mesh (conn: n_elements x
+ * nodes_per_element node ids): perm[i] = the old id of new node i.  Replaces
+ * Mesh::reorder_nodes_edges / find_RCM_from_edges (mesh.cpp:118-209,
+ * 1571-1615; Boost's cuthill_mckee_ordering with a pseudo-peripheral start
+ * per component, reversed).  The caller renumbers its arrays with it before
+ * fvp_create (edges are then numbered by sorted node ids, as mesh.cpp:157-166
+ * does).  Host only. */
+int fvp_rcm(int n_nodes, int n_elements, int nodes_per_element, const int *conn, int *perm);
+
 /* Build the device-side problem: replaces mesh::Mesh construction
  * (mesh.cpp:70-80) plus VarSatPhysicsImpl::initialise_vectors and
  * initialise_shape_functions (fvmpor.h:346-644, 947-1124). */

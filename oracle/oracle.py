@@ -83,6 +83,8 @@ def lib():
         L.orc_integrate_replay_opts.argtypes = [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
                                                 ctypes.c_double, _dp, ctypes.c_int, _dp, _ip, _dp, ctypes.c_int,
                                                 _dp, _dp]
+        L.orc_rcm.restype = ctypes.c_int
+        L.orc_rcm.argtypes = [ctypes.c_int, _ip, _ip, ctypes.c_int, _ip]
         L.orc_initial_derivative.argtypes = [_vp, ctypes.c_int, ctypes.c_double, _dp, _dp]
         _LIB = L
     return _LIB
@@ -334,6 +336,15 @@ def integrate_replay_opts(oprob, psi0, hs, ks, wfs=None, t0=0.0, rtol=1e-8, atol
     if n < 0:
         raise RuntimeError("oracle replay: Newton failed")
     return out, dict(newton_iters=int(st[0]), gmres_iters=int(st[1]), jac_setups=int(st[2]), residuals=int(st[3]))
+
+
+def rcm(ia, ja, seed=-1):
+    """Reverse Cuthill-McKee order of a CSR graph (no self loops): perm[i] = old
+    id of new node i; seed < 0 starts each component at a pseudo-peripheral node."""
+    n = len(ia) - 1
+    perm = np.zeros(n, dtype=np.int32)
+    lib().orc_rcm(n, _i32(ia), _i32(ja), int(seed), perm)
+    return perm
 
 
 def initial_derivative(oprob, t0, psi0):

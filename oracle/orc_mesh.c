@@ -519,3 +519,104 @@ void orc_mesh_cvfaces(const OrcMesh *m, int *ints, double *dbl) {
 void orc_mesh_gradients(const OrcMesh *m, double *g) {
     memcpy(g, m->grad, sizeof(double) * (size_t)m->T * m->npe * 3);
 }
+
+/* ---------------------------------------------------------------------------
+ * Reverse Cuthill-McKee order of a graph given as CSR (ia, ja; no self loops,
+ * sorted columns), as Mesh::reorder_nodes_edges / find_RCM_from_edges use it
+ * (mesh.cpp:118-209, 1571-1615, Boost's cuthill_mckee_ordering, reversed).
+ * Degrees count the self loop the reference adds (mesh.cpp:127-128).  seed >= 0
+ * starts the first component's sweep at that node (the other components at
+ * their lowest-degree node); seed < 0 starts every component at a
+ * pseudo-peripheral node: from its lowest-degree node (lowest index among
+ * equal degrees), repeated breadth-first level structures restarting at the
+ * lowest-degree node of the last level while the eccentricity grows (George
+ * and Liu).  The sweep appends each node's unvisited neighbours in
+ * increasing degree, ties in adjacency order.  perm[i] = old id of new node i.
+ */
+static int rcm_levels(int s, const int *ia, const int *ja, const char *done, int *stamp, int mark, int *buf,
+                      int *nlast, int **lastp) {
+    /* buf holds the BFS in order; returns the eccentricity, the last level is buf[*lastp .. + *nlast) */
+    int head = 0, tail = 0, lev_start = 0, ecc = 0;
+    buf[tail++] = s;
+    stamp[s] = mark;
+    int lev_end = tail;
+    for (;;) {
+        int nstart = tail;
+        for (head = lev_start; head < lev_end; ++head) {
+            int v = buf[head];
+            for (int k = ia[v]; k < ia[v + 1]; ++k) {
+                int w = ja[k];
+                if (!done[w] && stamp[w] != mark) { stamp[w] = mark; buf[tail++] = w; }
+            }
+        }
+        if (tail == nstart) break;
+        lev_start = nstart;
+        lev_end = tail;
+        ++ecc;
+    }
+    *lastp = buf + lev_start;
+    *nlast = lev_end - lev_start;
+    return ecc;
+}
+
+int orc_rcm(int N, const int *ia, const int *ja, int seed, int *perm) {
+    int *deg = (int *)malloc(sizeof(int) * (N ? N : 1));
+    char *done = (char *)calloc(N ? N : 1, 1);
+    int *stamp = (int *)malloc(sizeof(int) * (N ? N : 1));
+    int *order = (int *)malloc(sizeof(int) * (N ? N : 1));
+    int *buf = (int *)malloc(sizeof(int) * (N ? N : 1));
+    int *bydeg = (int *)malloc(sizeof(int) * (N ? N : 1));
+    for (int i = 0; i < N; ++i) { deg[i] = ia[i + 1] - ia[i] + 1; stamp[i] = -1; }
+    /* nodes by increasing degree, ties by index (counting sort: stable) */
+    int maxd = 0;
+    for (int i = 0; i < N; ++i) if (deg[i] > maxd) maxd = deg[i];
+    int *cnt = (int *)calloc(maxd + 2, sizeof(int));
+    for (int i = 0; i < N; ++i) cnt[deg[i] + 1]++;
+    for (int d = 0; d <= maxd; ++d) cnt[d + 1] += cnt[d];
+    for (int i = 0; i < N; ++i) bydeg[cnt[deg[i]]++] = i;
+    free(cnt);
+    int n = 0, mark = 0;
+    for (int q = -1; q < N; ++q) {
+        int s;
+        if (q < 0) {
+            if (seed < 0 || seed >= N) continue;
+            s = seed;
+        } else {
+            s = bydeg[q];
+            if (done[s]) continue;
+            if (seed < 0) {
+                int nlast, *last;
+                int ecc = rcm_levels(s, ia, ja, done, stamp, mark++, buf, &nlast, &last);
+                for (;;) {
+                    int best = last[0];
+                    for (int k = 1; k < nlast; ++k) {
+                        int v = last[k];
+                        if (deg[v] < deg[best] || (deg[v] == deg[best] && v < best)) best = v;
+                    }
+                    int e2 = rcm_levels(best, ia, ja, done, stamp, mark++, buf, &nlast, &last);
+                    if (e2 <= ecc) break;
+                    s = best;
+                    ecc = e2;
+                }
+            }
+        }
+        /* Cuthill-McKee sweep: neighbours of each node in increasing degree (insertion sort, stable) */
+        int head = n;
+        order[n++] = s;
+        done[s] = 1;
+        while (head < n) {
+            int v = order[head++];
+            int n0 = n;
+            for (int k = ia[v]; k < ia[v + 1]; ++k)
+                if (!done[ja[k]]) { done[ja[k]] = 1; order[n++] = ja[k]; }
+            for (int a = n0 + 1; a < n; ++a) {
+                int x = order[a], b = a;
+                while (b > n0 && deg[order[b - 1]] > deg[x]) { order[b] = order[b - 1]; --b; }
+                order[b] = x;
+            }
+        }
+    }
+    for (int i = 0; i < N; ++i) perm[i] = order[N - 1 - i];
+    free(deg); free(done); free(stamp); free(order); free(buf); free(bydeg);
+    return 0;
+}
