@@ -18,13 +18,16 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <complex>
 #include <functional>
 #include <mutex>
+#include <numeric>
 #include <set>
 #include <utility>
 #include <vector>
 
 #include "fvp_blas.h"
+#include "fvp_dense.h"
 #include "fvp_internal.h"
 
 using fvp::DevBuf;
@@ -83,6 +86,7 @@ struct fvp_csr {
     DevBuf<double> Q, w, r, z, part, scal, hdev;
     DevBuf<double> Hd, gd, csd, snd, yd;    // Hessenberg (column-major, m+1 rows), Givens pairs, rhs, solution
     double *hstat = nullptr;                // pinned host: per Arnoldi step {residual estimate, h_{j+1,j}}
+    DevBuf<double> Qk;                      // deflated GMRES: the kept harmonic Ritz vectors
     std::vector<hipEvent_t> ev;             // per Arnoldi step: its status copy has landed
 };
 
@@ -506,6 +510,13 @@ __global__ void lincomb(long n, double *__restrict__ out, const double *__restri
     double s = 0.;
     for (int c = 0; c < nc; ++c) s += Q[c * ldq + i] * y[c];
     out[i] = s;
+}
+
+// y = a x + b y  (host scalars)
+__global__ void vaxpby(long n, double a, const double *__restrict__ x, double b, double *__restrict__ y) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    y[i] = b == 0. ? a * x[i] : a * x[i] + b * y[i];
 }
 
 // ---- block-Jacobi ILU(0): one workgroup per block of bs consecutive rows
@@ -1104,6 +1115,205 @@ static int gmres_core(const GmresWork &W, hipStream_t s, long n, const Op &A, co
     return converged ? 0 : FVP_ERR_NOCONV;
 }
 
+
+// Restarted GMRES with deflated restarts (gmresFH, gmres.c:212-405, with
+// arnoldi, :412-511, and deflate, :527-849): each cycle runs the Arnoldi
+// process for A M from column K to m (no early exit within a cycle, as there),
+// solves the least-squares problem min || Q^T r - Hbar y || on the host, sets
+// x += M Q y and recomputes the true residual; before the next cycle the k
+// harmonic Ritz vectors of smallest magnitude (real and imaginary parts of a
+// complex pair) of H = H_m + h_{m+1,m}^2 f e_m^T, f = H_m^-T e_m, are kept at
+// the front of the basis, with the matching Hbar block, and the next cycle
+// continues from column K (the number kept).  The Gram-Schmidt passes, the
+// products and the basis updates run on the device; the small dense problems
+// on the host.  k = 0 is plain restarted GMRES with convergence checked per
+// cycle.
+int gmres_deflated(fvp_csr *Wc, hipStream_t s, long n, const Op &A, const Op &M, const double *b, double *x,
+                   const fvp_gmres_params *p, int k, fvp_gmres_stats *st) {
+    const int m = p->dim_krylov;
+    if (m < 2 || m > 63) { set_error("dim_krylov must be 2..63"); return -1; }
+    if (k < 0 || 2 * k >= m) { set_error("deflation count k must be 0 <= 2k < dim_krylov"); return -1; }
+    if (p->reorthogonalise < 0 || p->reorthogonalise > 2) { set_error("reorthogonalise must be 0, 1 or 2"); return -1; }
+    if (ensure_work(Wc, m)) return -1;
+    const long ldq = gs_ldq(n);
+    if (k && Wc->Qk.n < (size_t)ldq * (2 * k + 1) && Wc->Qk.alloc((size_t)ldq * (2 * k + 1))) return -1;
+    double *Q = Wc->Q.p, *w = Wc->w.p, *r = Wc->r.p, *z = Wc->z.p, *part = Wc->part.p, *sc = Wc->scal.p;
+    double *h1 = Wc->hdev.p, *h2 = Wc->hdev.p + (m + 2), *coef = Wc->hdev.p;
+    double *n1 = sc + 2, *n2 = sc + 3;
+    const int nb = nblk(n, 256);
+    auto host = [&](const double *d, double *hp, int cnt) -> int {
+        FVP_HIP(hipMemcpyAsync(hp, d, sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
+        FVP_HIP(hipStreamSynchronize(s));
+        return 0;
+    };
+    auto norm = [&](const double *v, double &out) -> int {
+        if (dot_async(s, n, v, v, part, sc)) return -1;
+        double q;
+        if (host(sc, &q, 1)) return -1;
+        out = std::sqrt(q);
+        return 0;
+    };
+    fvp_gmres_stats stats{0, 0, 0, 0., 0.};
+    // r = b - A x
+    if (A(x, r)) return -1;
+    vrsub<<<nb, 256, 0, s>>>(n, b, r);
+    double beta, normb;
+    if (norm(r, beta) || norm(b, normb)) return -1;
+    double error = normb > 0 ? beta / normb : 0.;
+    if (beta == 0. || error < p->tol) {
+        stats.converged = 1; stats.residual = beta; stats.error = error;
+        if (st) *st = stats;
+        return 0;
+    }
+    vaxpby<<<nb, 256, 0, s>>>(n, 1. / beta, r, 0., Q);
+    std::vector<double> H((size_t)(m + 1) * m, 0.);  // row-major (m+1) x m
+    auto Hr = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
+    std::vector<double> hh(2 * (m + 2));
+    int K = 0, restarts = 0, total = 0, converged = 0;
+    while (restarts <= p->max_restarts) {
+        // Arnoldi from column K
+        int j = K;
+        while (j < m) {
+            const double *qj = Q + (size_t)j * ldq;
+            if (M) { if (M(qj, z)) return -1; if (A(z, w)) return -1; }
+            else if (A(qj, w)) return -1;
+            multidot(s, n, w, Q, ldq, j + 1, part, h1, nullptr);
+            multiaxpy(s, n, w, Q, ldq, j + 1, h1, part, n1, nullptr);
+            if (p->reorthogonalise) {
+                multidot(s, n, w, Q, ldq, j + 1, part, h2, nullptr);
+                multiaxpy(s, n, w, Q, ldq, j + 1, h2, part, n2, nullptr);
+            }
+            if (host(Wc->hdev.p, hh.data(), 2 * (m + 2)) || host(sc + 2, hh.data() + 2 * (m + 2) - 2, 2)) return -1;
+            for (int i = 0; i <= j; ++i) Hr(i, j) = hh[i] + (p->reorthogonalise ? hh[m + 2 + i] : 0.);
+            const double ww = hh[2 * (m + 2) - 2 + (p->reorthogonalise ? 1 : 0)];
+            const double hn = std::sqrt(std::max(ww, 0.));
+            Hr(j + 1, j) = hn;
+            ++total;
+            if (hn < 1e-16) break;  // lucky breakdown: column j + 1 is not formed (gmres.c:485)
+            vaxpby<<<nb, 256, 0, s>>>(n, 1. / hn, w, 0., Q + (size_t)(j + 1) * ldq);
+            ++j;
+        }
+        // least squares on the j computed columns, right-hand side Q^T r
+        multidot(s, n, r, Q, ldq, j + 1, part, h1, nullptr);
+        std::vector<double> c(j + 1), y, Hs((size_t)(j + 1) * j);
+        if (host(h1, c.data(), j + 1)) return -1;
+        for (int a = 0; a <= j; ++a)
+            for (int bcol = 0; bcol < j; ++bcol) Hs[(size_t)a * j + bcol] = Hr(a, bcol);
+        if (j == 0 || !dense::lstsq(j + 1, j, Hs, c, y)) { set_error("GMRES: singular Hessenberg matrix"); return -1; }
+        FVP_HIP(hipMemcpyAsync(coef, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, s));
+        lincomb<<<nb, 256, 0, s>>>(n, w, Q, ldq, j, coef);
+        if (M) { if (M(w, z)) return -1; }
+        else vcopy<<<nb, 256, 0, s>>>(n, w, z);
+        vadd<<<nb, 256, 0, s>>>(n, z, x);
+        if (A(x, r)) return -1;
+        vrsub<<<nb, 256, 0, s>>>(n, b, r);
+        if (norm(r, beta)) return -1;
+        error = beta / normb;
+        if (error < p->tol) { converged = 1; break; }
+        if (++restarts > p->max_restarts) break;
+        int Knew = 0;
+        if (k && j == m) {
+            // harmonic Ritz vectors of H = H_m + hb^2 f e_m^T (deflate, gmres.c:600-760)
+            const double hb = Hr(m, m - 1);
+            std::vector<double> Hm((size_t)m * m), HmT((size_t)m * m), f(m, 0.);
+            for (int a = 0; a < m; ++a)
+                for (int bcol = 0; bcol < m; ++bcol) {
+                    Hm[(size_t)a * m + bcol] = Hr(a, bcol);
+                    HmT[(size_t)bcol * m + a] = Hr(a, bcol);
+                }
+            f[m - 1] = 1.;
+            bool ok = dense::lu_solve(m, HmT, f);
+            std::vector<double> Hh = Hm, wr, wi;
+            if (ok)
+                for (int a = 0; a < m; ++a) Hh[(size_t)a * m + m - 1] += hb * hb * f[a];
+            ok = ok && dense::eig(m, Hh, wr, wi);
+            std::vector<int> ord(m);
+            std::iota(ord.begin(), ord.end(), 0);
+            if (ok)
+                std::stable_sort(ord.begin(), ord.end(), [&](int a, int bcol) {
+                    return wr[a] * wr[a] + wi[a] * wi[a] < wr[bcol] * wr[bcol] + wi[bcol] * wi[bcol];
+                });
+            std::vector<double> X;  // column-major m x jj
+            for (int i = 0, q = 0; ok && i < k; ++i) {
+                const int pos = ord[i + q];
+                std::vector<std::complex<double>> v;
+                if (wi[pos] != 0.) {
+                    // the member of the pair with positive imaginary part: real and imaginary parts
+                    const std::complex<double> lam(wr[pos], std::fabs(wi[pos]));
+                    ok = dense::eigvec(m, Hh, lam, v);
+                    for (int a = 0; a < m; ++a) X.push_back(v[a].real());
+                    for (int a = 0; a < m; ++a) X.push_back(v[a].imag());
+                    ++q;
+                } else {
+                    ok = dense::eigvec(m, Hh, std::complex<double>(wr[pos], 0.), v);
+                    for (int a = 0; a < m; ++a) X.push_back(v[a].real());
+                }
+            }
+            const int jj = (int)(X.size() / m);
+            ok = ok && jj > 0 && jj + 1 <= m && dense::orthonormalise(m, jj, X);
+            if (ok) {
+                // Qk = Q_m X on the device; p = hb q_{m+1} - hb^2 Q_m f, orthogonalised against Qk
+                double *Qk = Wc->Qk.p;
+                for (int cc = 0; cc < jj; ++cc) {
+                    FVP_HIP(hipMemcpyAsync(coef, &X[(size_t)cc * m], sizeof(double) * m, hipMemcpyHostToDevice, s));
+                    lincomb<<<nb, 256, 0, s>>>(n, Qk + (size_t)cc * ldq, Q, ldq, m, coef);
+                    FVP_HIP(hipStreamSynchronize(s));  // coef is reused by the next column
+                }
+                std::vector<double> pc(m + 1);
+                for (int a = 0; a < m; ++a) pc[a] = -hb * hb * f[a];
+                pc[m] = hb;
+                FVP_HIP(hipMemcpyAsync(coef, pc.data(), sizeof(double) * (m + 1), hipMemcpyHostToDevice, s));
+                double *pm = Qk + (size_t)jj * ldq;
+                lincomb<<<nb, 256, 0, s>>>(n, pm, Q, ldq, m + 1, coef);
+                for (int cc = 0; cc < jj; ++cc) {  // modified Gram-Schmidt
+                    double dot;
+                    if (dot_async(s, n, pm, Qk + (size_t)cc * ldq, part, sc) || host(sc, &dot, 1)) return -1;
+                    vaxpby<<<nb, 256, 0, s>>>(n, -dot, Qk + (size_t)cc * ldq, 1., pm);
+                }
+                double betak;
+                if (norm(pm, betak)) return -1;
+                ok = betak > 0.;
+                if (ok) {
+                    vaxpby<<<nb, 256, 0, s>>>(n, 1. / betak, pm, 0., pm);
+                    FVP_HIP(hipMemcpyAsync(Q, Qk, sizeof(double) * ldq * (jj + 1), hipMemcpyDeviceToDevice, s));
+                    // the new Hbar: Omega = X^T H X - hb^2 (X^T f) s^T, row jj = betak s^T, s = X's last row
+                    std::vector<double> HX((size_t)m * jj, 0.), Xf(jj, 0.);
+                    for (int a = 0; a < m; ++a)
+                        for (int cc = 0; cc < jj; ++cc) {
+                            double acc = 0.;
+                            for (int q2 = 0; q2 < m; ++q2) acc += Hh[(size_t)a * m + q2] * X[(size_t)cc * m + q2];
+                            HX[(size_t)a * jj + cc] = acc;
+                        }
+                    for (int cc = 0; cc < jj; ++cc)
+                        for (int a = 0; a < m; ++a) Xf[cc] += X[(size_t)cc * m + a] * f[a];
+                    std::fill(H.begin(), H.end(), 0.);
+                    for (int r1 = 0; r1 < jj; ++r1)
+                        for (int cc = 0; cc < jj; ++cc) {
+                            double om = 0.;
+                            for (int a = 0; a < m; ++a) om += X[(size_t)r1 * m + a] * HX[(size_t)a * jj + cc];
+                            Hr(r1, cc) = om - hb * hb * Xf[r1] * X[(size_t)cc * m + m - 1];
+                        }
+                    for (int cc = 0; cc < jj; ++cc) Hr(jj, cc) = betak * X[(size_t)cc * m + m - 1];
+                    Knew = jj;
+                }
+            }
+        }
+        if (!Knew) {  // plain restart from the residual
+            std::fill(H.begin(), H.end(), 0.);
+            vaxpby<<<nb, 256, 0, s>>>(n, 1. / beta, r, 0., Q);
+        }
+        K = Knew;
+        FVP_HIP(hipGetLastError());
+    }
+    stats.restarts = converged ? restarts : restarts - 1;
+    stats.iterations = total;
+    stats.converged = converged;
+    stats.residual = beta;
+    stats.error = error;
+    if (st) *st = stats;
+    return converged ? 0 : FVP_ERR_NOCONV;
+}
+
 }  // namespace fvp
 
 // ================================================================================
@@ -1564,7 +1774,7 @@ extern "C" int fvp_csr_destroy(fvp_csr *A) {
     A->ia.release(); A->ja.release(); A->a.release(); A->lu.release(); A->jac.release(); A->diag.release();
     A->lrows.release(); A->urows.release(); A->flag.release(); A->Q.release(); A->w.release(); A->r.release();
     A->z.release(); A->part.release(); A->scal.release(); A->hdev.release();
-    A->Hd.release(); A->gd.release(); A->csd.release(); A->snd.release(); A->yd.release();
+    A->Hd.release(); A->gd.release(); A->csd.release(); A->snd.release(); A->yd.release(); A->Qk.release();
     if (A->hstat) (void)hipHostFree(A->hstat);
     for (hipEvent_t e : A->ev) (void)hipEventDestroy(e);
     A->b_ia.release(); A->b_src.release(); A->b_diag.release(); A->b_ja.release(); A->b_lu.release(); A->b_lu32.release();
@@ -1694,6 +1904,17 @@ extern "C" int fvp_gmres_csr(fvp_csr *A, const double *b, double *x, const fvp_g
     Op pc;
     if (A->ptype != FVP_PRECON_NONE) pc = [A](const double *in, double *out) { return precon_apply_impl(A, in, out); };
     return gmres_device(A, A->stream, A->n, op, pc, b, x, p, st);
+}
+
+extern "C" int fvp_gmres_csr_deflated(fvp_csr *A, const double *b, double *x, const fvp_gmres_params *p, int k,
+                                      fvp_gmres_stats *st) {
+    if (!A || !b || !x || !p) { set_error("null argument"); return -1; }
+    if (A->ptype < 0) { set_error("preconditioner not built"); return -1; }
+    FVP_HIP(hipSetDevice(A->device));
+    Op op = [A](const double *in, double *out) { return csr_gemv_impl(A, 1.0, in, 0.0, out); };
+    Op pc;
+    if (A->ptype != FVP_PRECON_NONE) pc = [A](const double *in, double *out) { return precon_apply_impl(A, in, out); };
+    return gmres_deflated(A, A->stream, A->n, op, pc, b, x, p, k, st);
 }
 
 // ---- allocation-free GMRES ------------------------------------------------------

@@ -63,6 +63,8 @@ SIGNATURES = {
     "fvp_precon_values": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p]),
     "fvp_gmres_csr": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p]),
+    "fvp_gmres_csr_deflated": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_int, ctypes.c_void_p]),
     "fvp_gmres_csr_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                              ctypes.c_void_p]),
     "fvp_precon_setup": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -334,6 +336,18 @@ def _csr_gmres_noalloc(self, b, x, m=30, max_restarts=20, tol=1e-10, reorth=1):
 
 
 CsrMatrix.gmres_noalloc = _csr_gmres_noalloc
+
+
+def _csr_gmres_deflated(self, b, x, k=4, m=30, max_restarts=20, tol=1e-10, reorth=1):
+    """fvp_gmres_csr_deflated: GMRES keeping k harmonic Ritz vectors at each restart."""
+    p = GmresParams(m, max_restarts, reorth, tol)
+    st = GmresStats()
+    rc = _check(lib().fvp_gmres_csr_deflated(self.h, _ptr(b), _ptr(x), ctypes.byref(p), int(k), ctypes.byref(st)),
+                "fvp_gmres_csr_deflated")
+    return rc, st.as_dict()
+
+
+CsrMatrix.gmres_deflated = _csr_gmres_deflated
 
 
 def _ctx_precon_setup(self, t, cj, h, psi, psip, F0, ewt, ptype=PRECON_ILU0) -> int:

@@ -178,6 +178,15 @@ typedef struct {
 /* Right-preconditioned restarted GMRES on A x = b with A's built
  * preconditioner (gmres(), gmres.c:26-201).  x holds the initial guess. */
 int fvp_gmres_csr(fvp_csr *A, const double *b, double *x, const fvp_gmres_params *p, fvp_gmres_stats *st);
+/* Restarted GMRES with deflated restarts (gmresFH, gmres.c:212-405; deflate,
+ * gmres.c:527-849): before each restart the k harmonic Ritz vectors of
+ * smallest magnitude (a complex pair contributes its real and imaginary
+ * parts) are kept at the front of the Krylov basis, with the matching block of
+ * the Hessenberg matrix.  Each cycle runs the Arnoldi process to dim_krylov
+ * columns and convergence is checked on the true residual after each cycle,
+ * as there.  0 <= 2k < dim_krylov; k = 0 is plain restarted GMRES. */
+int fvp_gmres_csr_deflated(fvp_csr *A, const double *b, double *x, const fvp_gmres_params *p, int k,
+                           fvp_gmres_stats *st);
 /* fvp_gmres_csr without any allocation: the Krylov basis, work vectors,
  * reduction partials and Hessenberg data live in a static 32 MiB device array
  * (one per GPU, held until the process ends) instead of library allocations,
