@@ -33,6 +33,8 @@
 using fvp::DevBuf;
 using fvp::set_error;
 
+struct fvp_schur;
+
 struct fvp_csr {
     int device = 0;
     hipStream_t stream = nullptr, own_stream = nullptr;
@@ -51,6 +53,7 @@ struct fvp_csr {
     // block-Jacobi ILU(0): per-block local CSR (columns relative to the block start)
     bool bj = false;
     int bs = FVP_BLOCK_ROWS;                // owned rows per block
+    std::vector<int> bstart;                // explicit block boundaries (nb + 1), empty: every bs rows
     int overlap = 0;                        // layers of neighbouring rows added to each block (RAS)
     int nblk = 0, b_maxnz = 0, b_maxrows = 0, lds_max = 0;
     DevBuf<int> b_roff, b_rowid, b_own;     // per block: first local row; global row of each local row; owned range
@@ -88,6 +91,27 @@ struct fvp_csr {
     double *hstat = nullptr;                // pinned host: per Arnoldi step {residual estimate, h_{j+1,j}}
     DevBuf<double> Qk;                      // deflated GMRES: the kept harmonic Ritz vectors
     std::vector<hipEvent_t> ev;             // per Arnoldi step: its status copy has landed
+    fvp_schur *sc = nullptr;                // Schur-complement preconditioner (FVP_PRECON_SCHUR)
+};
+
+// One-level Schur-complement preconditioner over subdomains of `bs`
+// consecutive rows.  Interior unknowns (all couplings inside their subdomain)
+// are numbered first, subdomain by subdomain, then the boundary unknowns.
+struct fvp_schur {
+    int nI = 0, nG = 0, nsub = 0, maxg = 0;
+    fvp_csr *B = nullptr;                   // interior block: block-Jacobi ILU(0), one block per subdomain
+    fvp::DevBuf<int> irow, grow;            // original row of each interior / boundary unknown
+    fvp::DevBuf<int> bsrc;                  // source entry in A of each B entry
+    fvp::DevBuf<int> e_ia, e_ja, e_loc, e_src;  // E (interior x boundary): column = boundary unknown, its
+                                            // index inside the subdomain, source entry in A
+    fvp::DevBuf<int> f_ia, f_ja, f_src;     // F (boundary x interior): column = interior unknown
+    fvp::DevBuf<double> ev, fv;             // current values of E and F
+    fvp::DevBuf<int> c_src;                 // C_ii entries: source in A and position in the dense blocks
+    fvp::DevBuf<long> c_pos;
+    fvp::DevBuf<int> goff, gsub, glocal;    // first boundary unknown per subdomain; subdomain and local index
+    fvp::DevBuf<long> soff;                 // per subdomain: offset of its dense block (column major)
+    fvp::DevBuf<double> S;                  // S_i, inverted in place by the build
+    fvp::DevBuf<double> t1, t2, yG;         // work vectors (nI, nI, nG)
 };
 
 namespace fvp {
@@ -903,6 +927,173 @@ __global__ void __launch_bounds__(64) bj_solve(const int *__restrict__ roff, con
 
 static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
+// ---- Schur-complement preconditioner (schur.c:887-981 form, :1583-1626 apply)
+// Largest boundary set of one subdomain: its dense block is inverted in LDS.
+constexpr int kSchurMax = 128;
+
+__global__ void vgather(long n, const int *__restrict__ src, const double *__restrict__ in, double *__restrict__ out) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[src[i]];
+}
+
+__global__ void vscatter(long n, const int *__restrict__ dst, const double *__restrict__ in, double *__restrict__ out) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) out[dst[i]] = in[i];
+}
+
+// column c of every subdomain's E at once (schur_form_local's dense E column)
+__global__ void schur_col_rhs(int nI, const int *__restrict__ e_ia, const int *__restrict__ e_loc,
+                              const double *__restrict__ ev, int c, double *__restrict__ t) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nI) return;
+    double v = 0.;
+    for (int k = e_ia[i]; k < e_ia[i + 1]; ++k)
+        if (e_loc[k] == c) v = ev[k];
+    t[i] = v;
+}
+
+// S_b(:, c) = -F_b y for every subdomain with more than c boundary unknowns
+__global__ void schur_col_store(int nG, const int *__restrict__ f_ia, const int *__restrict__ f_ja,
+                                const double *__restrict__ fv, const double *__restrict__ y,
+                                const int *__restrict__ gsub, const int *__restrict__ glocal,
+                                const int *__restrict__ goff, const long *__restrict__ soff, int c,
+                                double *__restrict__ S) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nG) return;
+    const int b = gsub[j], n = goff[b + 1] - goff[b];
+    if (c >= n) return;
+    double s = 0.;
+    for (int k = f_ia[j]; k < f_ia[j + 1]; ++k) s += fv[k] * y[f_ja[k]];
+    S[soff[b] + (long)c * n + glocal[j]] = -s;
+}
+
+__global__ void schur_add_c(long nc, const long *__restrict__ pos, const int *__restrict__ src,
+                            const double *__restrict__ a, double *__restrict__ S) {
+    long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k < nc) S[pos[k]] += a[src[k]];
+}
+
+// In-place Gauss-Jordan inversion of each subdomain's dense S_b (column major,
+// n <= kSchurMax) in LDS, with partial pivoting: at step k the first row of
+// largest |a(r, k)|, r >= k, is swapped into row k; the interchanges are undone
+// as column swaps in reverse order at the end.  Waves own columns, lanes own
+// rows (two rows per lane), so every LDS access of a wave is one contiguous
+// column segment.  A zero pivot sets *flag.
+__global__ void __launch_bounds__(256) schur_invert(const int *__restrict__ goff, const long *__restrict__ soff,
+                                                    double *__restrict__ S, int *__restrict__ flag) {
+    extern __shared__ double sm[];  // n x n block, pivot row, pivot column, n pivots, 2 flags
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int n = goff[b + 1] - goff[b];
+    if (n == 0) return;
+    double *a = sm, *rk = sm + n * n, *fk = rk + n;
+    int *perm = reinterpret_cast<int *>(fk + n);
+    int &s_piv = perm[n], &s_bad = perm[n + 1];
+    double *g = S + soff[b];
+    for (int e = tid; e < n * n; e += 256) a[e] = g[e];
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        if (wv == 0) {
+            double best = -1.;
+            int bi = n;
+            for (int r = k + lane; r < n; r += 64) {
+                const double v = fabs(a[k * n + r]);
+                if (v > best) { best = v; bi = r; }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob = __shfl_xor(best, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            }
+            if (lane == 0) {
+                s_piv = bi;
+                perm[k] = bi;
+                if (!(best > 0.)) s_bad = 1;
+            }
+        }
+        __syncthreads();
+        if (s_bad) break;
+        const int p = s_piv;
+        if (p != k)
+            for (int c = tid; c < n; c += 256) {
+                const double t = a[c * n + k];
+                a[c * n + k] = a[c * n + p];
+                a[c * n + p] = t;
+            }
+        __syncthreads();
+        const double inv = 1.0 / a[k * n + k];
+        for (int c = tid; c < n; c += 256) {
+            rk[c] = c == k ? inv : a[c * n + k] * inv;
+            fk[c] = a[k * n + c];
+        }
+        __syncthreads();
+        const double f0 = lane < n ? fk[lane] : 0., f1 = lane + 64 < n ? fk[lane + 64] : 0.;
+        for (int c = wv; c < n; c += 4) {
+            const double rc = rk[c];
+            double *col = a + c * n;
+            if (lane < n)
+                col[lane] = lane == k ? rc : (c == k ? -f0 * inv : col[lane] - f0 * rc);
+            if (lane + 64 < n)
+                col[lane + 64] = lane + 64 == k ? rc : (c == k ? -f1 * inv : col[lane + 64] - f1 * rc);
+        }
+        __syncthreads();
+    }
+    if (s_bad) {
+        if (tid == 0) atomicOr(flag, 1);
+        return;
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        const int p = perm[k];
+        if (p != k)
+            for (int r = tid; r < n; r += 256) {
+                const double t = a[k * n + r];
+                a[k * n + r] = a[p * n + r];
+                a[p * n + r] = t;
+            }
+        __syncthreads();
+    }
+    for (int e = tid; e < n * n; e += 256) g[e] = a[e];
+}
+
+// Middle of the apply, one workgroup per subdomain: v = x_G - F u (u = M_B^-1 x_I),
+// y_G = S_b^-1 v, written to y (original rows) and to yG (boundary numbering).
+__global__ void __launch_bounds__(kSchurMax) schur_mid(const int *__restrict__ goff, const long *__restrict__ soff,
+                                                       const double *__restrict__ Sinv, const int *__restrict__ grow,
+                                                       const int *__restrict__ f_ia, const int *__restrict__ f_ja,
+                                                       const double *__restrict__ fv, const double *__restrict__ x,
+                                                       const double *__restrict__ u, double *__restrict__ y,
+                                                       double *__restrict__ yG) {
+    __shared__ double v[kSchurMax];
+    const int b = blockIdx.x, r = threadIdx.x;
+    const int g0 = goff[b], n = goff[b + 1] - g0;
+    if (r < n) {
+        const int j = g0 + r;
+        double s = x[grow[j]];
+        for (int k = f_ia[j]; k < f_ia[j + 1]; ++k) s -= fv[k] * u[f_ja[k]];
+        v[r] = s;
+    }
+    __syncthreads();
+    if (r < n) {
+        const double *Sb = Sinv + soff[b];
+        double s = 0.;
+        for (int c = 0; c < n; ++c) s += Sb[(long)c * n + r] * v[c];
+        yG[g0 + r] = s;
+        y[grow[g0 + r]] = s;
+    }
+}
+
+// t_I = x_I - E y_G
+__global__ void schur_back(int nI, const int *__restrict__ irow, const int *__restrict__ e_ia,
+                           const int *__restrict__ e_ja, const double *__restrict__ ev, const double *__restrict__ x,
+                           const double *__restrict__ yG, double *__restrict__ t) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nI) return;
+    double s = x[irow[i]];
+    for (int k = e_ia[i]; k < e_ia[i + 1]; ++k) s -= ev[k] * yG[e_ja[k]];
+    t[i] = s;
+}
+
 // ---- host helpers ---------------------------------------------------------------
 static int ensure_work(fvp_csr *A, int m) {
     if (A->m_alloc >= m) return 0;
@@ -1377,12 +1568,13 @@ static size_t bj_lds(int maxnz, size_t value_bytes, int bs) {
 // solve are kept (restricted additive Schwarz).
 static int build_bj(fvp_csr *A) {
     const int n = A->n, BS = A->bs;
-    const int nb = (n + BS - 1) / BS;
+    const bool given = !A->bstart.empty();
+    const int nb = given ? (int)A->bstart.size() - 1 : (n + BS - 1) / BS;
     std::vector<int> roff(nb + 1, 0), rowid, own(2 * (size_t)nb), bia(1, 0), bsrc, bdiag;
     std::vector<unsigned short> bja;
     std::vector<int> stamp(n, -1), loc(n, 0), ext, frontier, next;
     for (int b = 0; b < nb; ++b) {
-        const int lo = b * BS, hi = std::min(n, lo + BS);
+        const int lo = given ? A->bstart[b] : b * BS, hi = given ? A->bstart[b + 1] : std::min(n, lo + BS);
         ext.clear();
         frontier.clear();
         for (int i = lo; i < hi; ++i) { stamp[i] = b; ext.push_back(i); frontier.push_back(i); }
@@ -1577,18 +1769,200 @@ static int build_bj(fvp_csr *A) {
     return 0;
 }
 
+int csr_create_impl(int device, int n, const int *ia, const int *ja, fvp_csr **out);
+static int precon_build_impl(fvp_csr *A, int type);
+static int precon_apply_impl(fvp_csr *A, const double *x, double *y, const double *xdiv = nullptr,
+                             double *tmp = nullptr, double *yn2 = nullptr);
+
+static void schur_free(fvp_csr *A) {
+    fvp_schur *P = A->sc;
+    if (!P) return;
+    if (P->B) fvp_csr_destroy(P->B);
+    P->irow.release(); P->grow.release(); P->bsrc.release(); P->e_ia.release(); P->e_ja.release();
+    P->e_loc.release(); P->e_src.release(); P->f_ia.release(); P->f_ja.release(); P->f_src.release();
+    P->ev.release(); P->fv.release(); P->c_src.release(); P->c_pos.release(); P->goff.release();
+    P->gsub.release(); P->glocal.release(); P->soff.release(); P->S.release(); P->t1.release();
+    P->t2.release(); P->yG.release();
+    delete P;
+    A->sc = nullptr;
+}
+
+// Split of mtx_CRS_dist_split (schur.c:82-600) for subdomains of bs consecutive
+// rows: a row is a boundary row when it has an entry in another subdomain or
+// another subdomain has an entry in its column.  Interior unknowns keep their
+// row order inside each subdomain, and so do the boundary unknowns.
+static int build_schur(fvp_csr *A) {
+    const int n = A->n, bs = A->bs, nsub = (n + bs - 1) / bs;
+    std::vector<char> bnd(n, 0);
+    for (int i = 0; i < n; ++i)
+        for (int k = A->ia_h[i]; k < A->ia_h[i + 1]; ++k) {
+            const int j = A->ja_h[k];
+            if (i / bs != j / bs) bnd[i] = bnd[j] = 1;
+        }
+    std::vector<int> irow, grow, idx(n), goff(nsub + 1, 0), bstart(1, 0), gsub, glocal;
+    for (int b = 0; b < nsub; ++b) {
+        const int lo = b * bs, hi = std::min(n, lo + bs);
+        for (int i = lo; i < hi; ++i)
+            if (!bnd[i]) { idx[i] = (int)irow.size(); irow.push_back(i); }
+        if ((int)irow.size() > bstart.back()) bstart.push_back((int)irow.size());  // empty interiors are skipped
+        for (int i = lo; i < hi; ++i)
+            if (bnd[i]) {
+                idx[i] = (int)grow.size();
+                glocal.push_back((int)grow.size() - goff[b]);
+                grow.push_back(i);
+                gsub.push_back(b);
+            }
+        goff[b + 1] = (int)grow.size();
+        if (goff[b + 1] - goff[b] > kSchurMax) {
+            set_error("Schur preconditioner: subdomain " + std::to_string(b) + " has " +
+                      std::to_string(goff[b + 1] - goff[b]) + " boundary rows, at most " +
+                      std::to_string(kSchurMax) + " are supported (use smaller subdomains)");
+            return -1;
+        }
+    }
+    const int nI = (int)irow.size(), nG = (int)grow.size();
+    std::vector<int> bia(1, 0), bja, bsrc, eia(1, 0), eja, eloc, esrc, fia(1, 0), fja, fsrc, csrc;
+    std::vector<long> cpos, soff(nsub + 1, 0);
+    for (int b = 0; b < nsub; ++b) {
+        const long m = goff[b + 1] - goff[b];
+        soff[b + 1] = soff[b] + m * m;
+    }
+    for (int i : irow) {
+        for (int k = A->ia_h[i]; k < A->ia_h[i + 1]; ++k) {
+            const int j = A->ja_h[k];
+            if (bnd[j]) { eja.push_back(idx[j]); eloc.push_back(idx[j] - goff[j / bs]); esrc.push_back(k); }
+            else { bja.push_back(idx[j]); bsrc.push_back(k); }
+        }
+        bia.push_back((int)bja.size());
+        eia.push_back((int)eja.size());
+    }
+    for (int i : grow) {
+        const int b = i / bs;
+        for (int k = A->ia_h[i]; k < A->ia_h[i + 1]; ++k) {
+            const int j = A->ja_h[k];
+            if (!bnd[j]) { fja.push_back(idx[j]); fsrc.push_back(k); }
+            else if (j / bs == b) {  // C_ii; couplings to other subdomains' boundary rows are dropped
+                const long m = goff[b + 1] - goff[b];
+                cpos.push_back(soff[b] + (long)(idx[j] - goff[b]) * m + (idx[i] - goff[b]));
+                csrc.push_back(k);
+            }
+        }
+        fia.push_back((int)fja.size());
+    }
+    schur_free(A);
+    fvp_schur *P = new fvp_schur();
+    A->sc = P;
+    P->nI = nI; P->nG = nG; P->nsub = nsub;
+    P->maxg = 0;
+    for (int b = 0; b < nsub; ++b) P->maxg = std::max(P->maxg, goff[b + 1] - goff[b]);
+    if (nI > 0) {
+        if (csr_create_impl(A->device, nI, bia.data(), bja.data(), &P->B)) { schur_free(A); return -1; }
+        P->B->bstart = bstart;
+        if (P->irow.upload(irow.data(), nI) || P->bsrc.upload(bsrc.data(), bsrc.size()) ||
+            P->e_ia.upload(eia.data(), nI + 1) || P->t1.alloc(nI) || P->t2.alloc(nI) ||
+            (!eja.empty() && (P->e_ja.upload(eja.data(), eja.size()) || P->e_loc.upload(eloc.data(), eloc.size()) ||
+                              P->e_src.upload(esrc.data(), esrc.size()) || P->ev.alloc(eja.size())))) {
+            schur_free(A);
+            return -1;
+        }
+    }
+    if (nG > 0) {
+        if (P->grow.upload(grow.data(), nG) || P->f_ia.upload(fia.data(), nG + 1) || P->yG.alloc(nG) ||
+            P->gsub.upload(gsub.data(), nG) || P->glocal.upload(glocal.data(), nG) ||
+            P->goff.upload(goff.data(), nsub + 1) || P->soff.upload(soff.data(), nsub + 1) ||
+            P->S.alloc(soff[nsub]) || P->c_src.upload(csrc.data(), csrc.size()) ||
+            P->c_pos.upload(cpos.data(), cpos.size()) ||
+            (!fja.empty() && (P->f_ja.upload(fja.data(), fja.size()) || P->f_src.upload(fsrc.data(), fsrc.size()) ||
+                              P->fv.alloc(fja.size())))) {
+            schur_free(A);
+            return -1;
+        }
+    }
+    return 0;
+}
+
+// precon_schur (schur.c:1247-1323) with schur_form_local (schur.c:887-981):
+// ILU(0) of the interior blocks, then S_i = C_ii - F_i M_B^-1 E_i one column
+// index at a time for every subdomain at once, then each S_i inverted.
+static int schur_factor(fvp_csr *A) {
+    hipStream_t s = A->stream;
+    if (!A->lds_max && hipDeviceGetAttribute(&A->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, A->device))
+        A->lds_max = 64 * 1024;
+    if (!A->sc && build_schur(A)) return -1;
+    fvp_schur *P = A->sc;
+    const size_t lds = (size_t)P->maxg * (P->maxg + 2) * sizeof(double) + (size_t)(P->maxg + 2) * sizeof(int);
+    if (lds > (size_t)A->lds_max) { set_error("Schur preconditioner: a dense block does not fit the LDS"); return -1; }
+    if (lds > 65536 && hipFuncSetAttribute((const void *)schur_invert, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("Schur preconditioner: the block inversion cannot have " + std::to_string(lds) + " bytes of LDS");
+        return -1;
+    }
+    if (P->B) {
+        fvp_csr *B = P->B;
+        fvp_csr_set_stream(B, s);
+        vgather<<<nblk(B->nnz, 256), 256, 0, s>>>(B->nnz, P->bsrc.p, A->a.p, B->a.p);
+        int rc = precon_build_impl(B, FVP_PRECON_BJILU0);
+        if (rc) return rc;
+        if (P->ev.n) vgather<<<nblk(P->ev.n, 256), 256, 0, s>>>(P->ev.n, P->e_src.p, A->a.p, P->ev.p);
+    }
+    if (P->nG == 0) return 0;
+    if (P->fv.n) vgather<<<nblk(P->fv.n, 256), 256, 0, s>>>(P->fv.n, P->f_src.p, A->a.p, P->fv.p);
+    for (int c = 0; c < P->maxg; ++c) {
+        if (P->B) {
+            schur_col_rhs<<<nblk(P->nI, 256), 256, 0, s>>>(P->nI, P->e_ia.p, P->e_loc.p, P->ev.p, c, P->t1.p);
+            int rc = precon_apply_impl(P->B, P->t1.p, P->t2.p);
+            if (rc) return rc;
+        }
+        schur_col_store<<<nblk(P->nG, 256), 256, 0, s>>>(P->nG, P->f_ia.p, P->f_ja.p, P->fv.p, P->t2.p, P->gsub.p,
+                                                        P->glocal.p, P->goff.p, P->soff.p, c, P->S.p);
+    }
+    if (P->c_src.n)
+        schur_add_c<<<nblk(P->c_src.n, 256), 256, 0, s>>>(P->c_src.n, P->c_pos.p, P->c_src.p, A->a.p, P->S.p);
+    schur_invert<<<P->nsub, 256, lds, s>>>(P->goff.p, P->soff.p, P->S.p, A->flag.p);
+    FVP_HIP(hipGetLastError());
+    return 0;
+}
+
+// precon_schur_apply (schur.c:1583-1626): u = M_B^-1 x_I; y_G = S^-1 (x_G - F u);
+// y_I = M_B^-1 (x_I - E y_G)
+static int schur_apply(fvp_csr *A, const double *x, double *y) {
+    hipStream_t s = A->stream;
+    fvp_schur *P = A->sc;
+    if (P->B) {
+        fvp_csr_set_stream(P->B, s);
+        vgather<<<nblk(P->nI, 256), 256, 0, s>>>(P->nI, P->irow.p, x, P->t1.p);
+        int rc = precon_apply_impl(P->B, P->t1.p, P->t2.p);
+        if (rc) return rc;
+    }
+    if (P->nG)
+        schur_mid<<<P->nsub, kSchurMax, 0, s>>>(P->goff.p, P->soff.p, P->S.p, P->grow.p, P->f_ia.p, P->f_ja.p,
+                                                 P->fv.p, x, P->t2.p, y, P->yG.p);
+    if (P->B) {
+        schur_back<<<nblk(P->nI, 256), 256, 0, s>>>(P->nI, P->irow.p, P->e_ia.p, P->e_ja.p, P->ev.p, x, P->yG.p,
+                                                   P->t1.p);
+        int rc = precon_apply_impl(P->B, P->t1.p, P->t2.p);
+        if (rc) return rc;
+        vscatter<<<nblk(P->nI, 256), 256, 0, s>>>(P->nI, P->irow.p, P->t2.p, y);
+    }
+    return 0;
+}
+
 static int precon_build_impl(fvp_csr *A, int type) {
     hipStream_t s = A->stream;
     const int n = A->n;
     if (type == FVP_PRECON_NONE) { A->ptype = type; return 0; }
     if (type != FVP_PRECON_JACOBI && type != FVP_PRECON_ILU0 && type != FVP_PRECON_BJILU0 &&
-        type != FVP_PRECON_BJILU0_F32) {
+        type != FVP_PRECON_BJILU0_F32 && type != FVP_PRECON_SCHUR) {
         set_error("invalid preconditioner type");
         return -1;
     }
     if (!A->flag.p && A->flag.alloc(1)) return -1;
     FVP_HIP(hipMemsetAsync(A->flag.p, 0, sizeof(int), s));
-    if (type == FVP_PRECON_BJILU0 || type == FVP_PRECON_BJILU0_F32) {
+    if (type == FVP_PRECON_SCHUR) {
+        int rc = schur_factor(A);
+        if (rc) { A->ptype = -1; return rc; }
+    } else if (type == FVP_PRECON_BJILU0 || type == FVP_PRECON_BJILU0_F32) {
         if (!A->bj) {
             int rc = build_bj(A);
             if (rc) { A->ptype = -1; return rc; }
@@ -1626,14 +2000,14 @@ static int precon_build_impl(fvp_csr *A, int type) {
         const long nl = (long)A->esrc_l.n, nu = (long)A->esrc_u.n, nd = (long)A->dsrc.n;
         if (type == FVP_PRECON_BJILU0) {
             if (!A->val_l.p && (A->val_l.alloc(nl) || A->val_u.alloc(nu) || A->dinv.alloc(nd))) return -1;
-            bj_pack_vals<double><<<nblk(nl, 256), 256, 0, s>>>(nl, A->esrc_l.p, A->b_lu.p, A->val_l.p);
-            bj_pack_vals<double><<<nblk(nu, 256), 256, 0, s>>>(nu, A->esrc_u.p, A->b_lu.p, A->val_u.p);
-            bj_pack_vals<double><<<nblk(nd, 256), 256, 0, s>>>(nd, A->dsrc.p, A->b_lu.p, A->dinv.p);
+            if (nl) bj_pack_vals<double><<<nblk(nl, 256), 256, 0, s>>>(nl, A->esrc_l.p, A->b_lu.p, A->val_l.p);
+            if (nu) bj_pack_vals<double><<<nblk(nu, 256), 256, 0, s>>>(nu, A->esrc_u.p, A->b_lu.p, A->val_u.p);
+            if (nd) bj_pack_vals<double><<<nblk(nd, 256), 256, 0, s>>>(nd, A->dsrc.p, A->b_lu.p, A->dinv.p);
         } else {
             if (!A->val32_l.p && (A->val32_l.alloc(nl) || A->val32_u.alloc(nu) || A->dinv32.alloc(nd))) return -1;
-            bj_pack_vals<float><<<nblk(nl, 256), 256, 0, s>>>(nl, A->esrc_l.p, A->b_lu32.p, A->val32_l.p);
-            bj_pack_vals<float><<<nblk(nu, 256), 256, 0, s>>>(nu, A->esrc_u.p, A->b_lu32.p, A->val32_u.p);
-            bj_pack_vals<float><<<nblk(nd, 256), 256, 0, s>>>(nd, A->dsrc.p, A->b_lu32.p, A->dinv32.p);
+            if (nl) bj_pack_vals<float><<<nblk(nl, 256), 256, 0, s>>>(nl, A->esrc_l.p, A->b_lu32.p, A->val32_l.p);
+            if (nu) bj_pack_vals<float><<<nblk(nu, 256), 256, 0, s>>>(nu, A->esrc_u.p, A->b_lu32.p, A->val32_u.p);
+            if (nd) bj_pack_vals<float><<<nblk(nd, 256), 256, 0, s>>>(nd, A->dsrc.p, A->b_lu32.p, A->dinv32.p);
         }
     } else if (type == FVP_PRECON_JACOBI) {
         if (A->jac.n != (size_t)n && A->jac.alloc(n)) return -1;
@@ -1669,8 +2043,8 @@ static int precon_build_impl(fvp_csr *A, int type) {
 
 // y = M^-1 (x / xdiv) with xdiv optional (the block solves divide as they load);
 // yn2 (optional, at least nblk doubles): the block solves' partials of ||y||^2
-static int precon_apply_impl(fvp_csr *A, const double *x, double *y, const double *xdiv = nullptr,
-                             double *tmp = nullptr, double *yn2 = nullptr) {
+static int precon_apply_impl(fvp_csr *A, const double *x, double *y, const double *xdiv, double *tmp,
+                             double *yn2) {
     hipStream_t s = A->stream;
     const int n = A->n;
     if (xdiv && A->ptype != FVP_PRECON_BJILU0 && A->ptype != FVP_PRECON_BJILU0_F32) {
@@ -1711,6 +2085,11 @@ static int precon_apply_impl(fvp_csr *A, const double *x, double *y, const doubl
             else FVP_BJ_SOLVE(float, false, A->val32_l.p, A->val32_u.p, A->dinv32.p);
         }
 #undef FVP_BJ_SOLVE
+        break;
+    }
+    case FVP_PRECON_SCHUR: {
+        int rc = schur_apply(A, x, y);
+        if (rc) return rc;
         break;
     }
     case FVP_PRECON_ILU0: {
@@ -1786,6 +2165,7 @@ extern "C" int fvp_csr_destroy(fvp_csr *A) {
     A->b_rows_l.release(); A->b_rows_u.release();
     A->s_lev_l.release(); A->s_lev_u.release(); A->s_lptr_l.release(); A->s_lptr_u.release();
     A->ov_ptr.release(); A->ov_pos.release(); A->ov_tri.release();
+    schur_free(A);
     if (A->own_stream) (void)hipStreamDestroy(A->own_stream);
     delete A;
     return 0;
@@ -1827,7 +2207,9 @@ static int set_block_rows(fvp_csr *A, int rows) {
     if (rows != A->bs) {
         A->bs = rows;
         A->bj = false;  // block tables are rebuilt by the next block-Jacobi build
-        if (A->ptype == FVP_PRECON_BJILU0 || A->ptype == FVP_PRECON_BJILU0_F32) A->ptype = -1;
+        schur_free(A);  // and the subdomains by the next Schur build
+        if (A->ptype == FVP_PRECON_BJILU0 || A->ptype == FVP_PRECON_BJILU0_F32 || A->ptype == FVP_PRECON_SCHUR)
+            A->ptype = -1;
     }
     return 0;
 }

@@ -104,7 +104,7 @@ SIGNATURES = {
                                                     ctypes.c_void_p]),
 }
 
-PRECON_NONE, PRECON_JACOBI, PRECON_ILU0, PRECON_BJILU0, PRECON_BJILU0_F32 = 0, 2, 5, 7, 8
+PRECON_NONE, PRECON_SCHUR, PRECON_JACOBI, PRECON_ILU0, PRECON_BJILU0, PRECON_BJILU0_F32 = 0, 1, 2, 5, 7, 8
 JACOBIAN_ELEMENT, JACOBIAN_COLOURED = 0, 1
 RESIDUAL_OWNER, RESIDUAL_WINDOW = 0, 1
 BLOCK_ROWS = 512

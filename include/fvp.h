@@ -111,6 +111,16 @@ int fvp_jv(fvp_ctx *ctx, double t, const double *psi, const double *psip, const 
  * precon.h:8-13.  decomp's build calls return 1 on success (precon.c:151-195);
  * here 0 = success and a missing / zero pivot returns FVP_ERR_PIVOT (> 0). */
 #define FVP_PRECON_NONE 0
+/* One-level Schur complement (PRECON_SCHUR, precon.h:9; precon_schur,
+ * schur.c:1247-1323, schur_form_local :887-981, precon_schur_apply
+ * :1583-1626).  Subdomains are the blocks of FVP_BLOCK_ROWS consecutive rows
+ * (fvp_precon_block_rows); an interior row couples only inside its subdomain,
+ * the others are boundary rows.  B (interior) gets block ILU(0), one block per
+ * subdomain; each local S_i = C_ii - F_i M_B^-1 E_i is formed dense and
+ * inverted (Gauss-Jordan, partial pivoting); S's couplings between subdomains
+ * are dropped (block-Jacobi on S at level 0).  At most 128 boundary rows per
+ * subdomain: the build fails (< 0) beyond that. */
+#define FVP_PRECON_SCHUR 1
 #define FVP_PRECON_JACOBI 2
 #define FVP_PRECON_ILU0 5
 /* Block-Jacobi ILU(0): ILU(0) of the diagonal blocks of consecutive
