@@ -67,7 +67,7 @@ def parse():
     ap.add_argument("--nz", type=int, default=40)
     ap.add_argument("--rtol", type=float, default=1e-6)
     ap.add_argument("--atol", type=float, default=1e-6)
-    ap.add_argument("--precon", default="bjilu0", choices=["bjilu0", "ilu0", "jacobi"],
+    ap.add_argument("--precon", default="bjilu0", choices=["bjilu0", "ilu0", "jacobi", "schur"],
                     help="preconditioner of the plain FP64 run")
     ap.add_argument("--block-rows", type=int, default=1024,
                     help="block-Jacobi rows per block (0 = library default; 1024 measured fastest on this box)")
@@ -297,7 +297,8 @@ def main():
     import torch
     from fvmpor_amd.diagnostics import inflow_rate, total_mass
     from fvmpor_amd.learn import node_graph, static_features
-    from fvmpor_amd.lib import (BLOCK_ROWS, PRECON_BJILU0, PRECON_BJILU0_F32, PRECON_ILU0, PRECON_JACOBI, Context,
+    from fvmpor_amd.lib import (BLOCK_ROWS, PRECON_BJILU0, PRECON_BJILU0_F32, PRECON_ILU0, PRECON_JACOBI,
+                                PRECON_SCHUR, Context,
                                 Mpnn)
     from fvmpor_amd.problems import box3d_problem
 
@@ -311,7 +312,8 @@ def main():
     stream = torch.cuda.current_stream()
     ctx.set_stream(stream.cuda_stream)
     psi0 = torch.tensor(prob.psi0, device=dev)
-    pc64 = {"bjilu0": PRECON_BJILU0, "ilu0": PRECON_ILU0, "jacobi": PRECON_JACOBI}[args.precon]
+    pc64 = {"bjilu0": PRECON_BJILU0, "ilu0": PRECON_ILU0, "jacobi": PRECON_JACOBI,
+            "schur": PRECON_SCHUR}[args.precon]
     ncol, _ = ctx.colours()
     if args.launch == "auto":
         launch = ctx.autotune(0.0, psi0, torch.zeros_like(psi0))

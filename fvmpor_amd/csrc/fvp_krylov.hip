@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <array>
 #include <cmath>
 #include <complex>
@@ -56,9 +57,9 @@ struct fvp_csr {
     std::vector<int> bstart;                // explicit block boundaries (nb + 1), empty: every bs rows
     int overlap = 0;                        // layers of neighbouring rows added to each block (RAS)
     int nblk = 0, b_maxnz = 0, b_maxrows = 0, lds_max = 0;
+    int factor_mode = -1;                   // staging form of the last block factorisation (bj_factor)
     DevBuf<int> b_roff, b_rowid, b_own;     // per block: first local row; global row of each local row; owned range
     DevBuf<double> b_zn2;                   // per block: its share of ||z||^2 of the last scaled solve
-    size_t b_lds = 0;
     DevBuf<int> b_ia, b_src, b_diag;        // row pointers into the block arrays, source entry, diagonal position
     DevBuf<unsigned short> b_ja;            // local column
     DevBuf<double> b_lu;
@@ -562,11 +563,24 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// STAGED: the block's values, columns, row pointers and diagonal offsets are
-// staged in LDS (when they fit); otherwise the factorisation works in place on
-// the block's values in memory (one block's values stay in its CU's caches) and
-// LDS holds only the column maps.
-template <typename T, bool STAGED>
+// MODE 2: the block's values, local columns, row pointers and diagonal
+// offsets are staged in LDS; MODE 1: all but the columns (read from memory,
+// each one loaded a step ahead of its use); MODE 0: the factorisation works in
+// place on the block's values in memory and LDS holds only the column maps.
+// Per wavefront a byte map column -> position in its current row (-1: not in
+// the row; rows have at most 64 entries).  Within a row the L entries are
+// taken in column order; the columns of row k's U part for the next L entry
+// are loaded while the current one is applied.
+template <int MODE>
+__host__ __device__ inline size_t bj_factor_lds(int max_nnz, size_t value_bytes, int max_rows) {
+    const size_t maps = (size_t)(kBjFactorThreads / 64) * max_rows;
+    if (MODE == 0) return maps;
+    size_t s = (size_t)max_nnz * value_bytes + (size_t)(2 * max_rows + 1) * sizeof(unsigned short) + maps;
+    if (MODE == 2) s += (size_t)max_nnz * sizeof(unsigned short);
+    return s;
+}
+
+template <typename T, int MODE>
 __global__ void __launch_bounds__(kBjFactorThreads) bj_factor(
     const int *__restrict__ roff, int max_rows, int max_nnz, const int *__restrict__ ia,
     const unsigned short *__restrict__ ja,
@@ -581,66 +595,81 @@ __global__ void __launch_bounds__(kBjFactorThreads) bj_factor(
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
     T *val;
     const unsigned short *col;
-    short *pmap;  // per wavefront: position of column j in its current row, -1 if absent
+    signed char *pmap;
     unsigned short *rps = nullptr, *dgs = nullptr;
-    if (STAGED) {
+    if (MODE > 0) {
         val = (T *)smem;
-        unsigned short *cs = (unsigned short *)(val + max_nnz);
-        rps = cs + max_nnz;
+        unsigned short *cs = (unsigned short *)(val + max_nnz);  // MODE 2: the staged columns
+        col = MODE == 2 ? cs : ja + z0;
+        rps = MODE == 2 ? cs + max_nnz : cs;
         dgs = rps + (max_rows + 1);
-        pmap = (short *)(dgs + max_rows);
+        pmap = (signed char *)(dgs + max_rows);
         for (int k = threadIdx.x; k < nzb; k += blockDim.x) {
             val[k] = (T)a[src[z0 + k]];
-            cs[k] = ja[z0 + k];
+            if (MODE == 2) cs[k] = ja[z0 + k];
         }
         for (int r = threadIdx.x; r < cnt; r += blockDim.x) {
             rps[r] = (unsigned short)(ia[base + r] - z0);
             dgs[r] = (unsigned short)(diag[base + r] - z0);
         }
         if (threadIdx.x == 0) rps[cnt] = (unsigned short)nzb;
-        col = cs;
     } else {
         val = lu + z0;
         col = ja + z0;
-        pmap = (short *)smem;
+        pmap = (signed char *)smem;
         for (int k = threadIdx.x; k < nzb; k += blockDim.x) val[k] = (T)a[src[z0 + k]];
         __threadfence_block();
     }
-    auto rp = [&](int r) -> int { return STAGED ? (int)rps[r] : ia[base + r] - z0; };
-    auto dg = [&](int r) -> int { return STAGED ? (int)dgs[r] : diag[base + r] - z0; };
+    auto rp = [&](int r) -> int { return MODE > 0 ? (int)rps[r] : ia[base + r] - z0; };
+    auto dg = [&](int r) -> int { return MODE > 0 ? (int)dgs[r] : diag[base + r] - z0; };
     for (int k = threadIdx.x; k < nwaves * max_rows; k += blockDim.x) pmap[k] = -1;
     __syncthreads();
-    short *map = pmap + wave * max_rows;
+    signed char *map = pmap + wave * max_rows;
     for (int l = blev[b]; l < blev[b + 1]; ++l) {
         for (int t = lptr[l] + wave; t < lptr[l + 1]; t += nwaves) {
             const int r = rows[t];
             const int s0 = rp(r), len = rp(r + 1) - s0, d = dg(r) - s0;
-            if (lane < len) map[col[s0 + lane]] = (short)lane;
+            const int cl = lane < len ? (int)col[s0 + lane] : 0;  // row r's columns, one per lane
+            if (lane < len) map[cl] = (signed char)lane;
+            // row k = column of L entry kk: its U part [ks, ke) and this lane's column there
+            int k = 0, ks = 0, ke = 0, cm = -1;
+            if (d > 0) {
+                k = __builtin_amdgcn_readlane(cl, 0);
+                ks = dg(k) + 1;
+                ke = rp(k + 1);
+                cm = ks + lane < ke ? (int)col[ks + lane] : -1;
+            }
             wave_lds_sync();
             for (int kk = 0; kk < d; ++kk) {
-                const int k = col[s0 + kk];
+                int k2 = 0, ks2 = 0, ke2 = 0, cm2 = -1;
+                if (kk + 1 < d) {
+                    k2 = __builtin_amdgcn_readlane(cl, kk + 1);
+                    ks2 = dg(k2) + 1;
+                    ke2 = rp(k2 + 1);
+                    cm2 = ks2 + lane < ke2 ? (int)col[ks2 + lane] : -1;
+                }
                 const T lik = val[s0 + kk] * val[dg(k)];  // u_kk is stored inverted
-                const int ks = dg(k) + 1, ke = rp(k + 1);
-                for (int m = ks + lane; m < ke; m += 64) {
-                    const int p = map[col[m]];
-                    if (p > kk) val[s0 + p] -= lik * val[m];
+                if (cm >= 0) {
+                    const int p = map[cm];
+                    if (p > kk) val[s0 + p] -= lik * val[ks + lane];
                 }
                 wave_lds_sync();
                 if (lane == 0) val[s0 + kk] = lik;
                 wave_lds_sync();
+                k = k2; ks = ks2; ke = ke2; cm = cm2;
             }
             if (lane == 0) {
                 const T piv = val[s0 + d];
                 if (piv == T(0)) { atomicOr(flag, 1); val[s0 + d] = T(0); }
                 else val[s0 + d] = T(1) / piv;
             }
-            if (lane < len) map[col[s0 + lane]] = -1;
+            if (lane < len) map[cl] = -1;
             wave_lds_sync();
         }
-        if (!STAGED) __threadfence_block();
+        if (MODE == 0) __threadfence_block();
         __syncthreads();
     }
-    if (STAGED)
+    if (MODE > 0)
         for (int k = threadIdx.x; k < nzb; k += blockDim.x) lu[z0 + k] = val[k];
 }
 
@@ -1555,12 +1584,6 @@ static int build_levels(fvp_csr *A, const std::vector<int> &diag) {
     return 0;
 }
 
-// LDS bytes of one block's factor staging: values, local columns, row pointers, diagonal offsets
-static size_t bj_lds(int maxnz, size_t value_bytes, int bs) {
-    return (size_t)maxnz * (value_bytes + 2) + (size_t)(2 * bs + 1) * 2 + 8 +
-           (size_t)(kBjFactorThreads / 64) * bs * sizeof(short);
-}
-
 // Block-local CSR and the per-block level sets of L and U.  Block b owns rows
 // [b bs, (b+1) bs); with overlap > 0 its local rows are the owned rows plus
 // `overlap` layers of their graph neighbours (sorted by global row, so the
@@ -1754,14 +1777,14 @@ static int build_bj(fvp_csr *A) {
     }
     A->b_maxnz = maxnz;
     A->b_maxrows = maxrows;
-    A->b_lds = bj_lds(maxnz, sizeof(double), maxrows);
     if (hipDeviceGetAttribute(&A->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, A->device) != hipSuccess)
         A->lds_max = 64 * 1024;
     // per-kernel attributes (shared by every matrix): allow the device maximum;
     // each build checks its own block against it
-    for (const void *f : {(const void *)bj_factor<double, true>, (const void *)bj_factor<double, false>,
-                          (const void *)bj_solve<double, false>, (const void *)bj_solve<double, true>,
-                          (const void *)bj_factor<float, true>, (const void *)bj_factor<float, false>,
+    for (const void *f : {(const void *)bj_factor<double, 2>, (const void *)bj_factor<double, 1>,
+                          (const void *)bj_factor<double, 0>, (const void *)bj_solve<double, false>,
+                          (const void *)bj_solve<double, true>, (const void *)bj_factor<float, 2>,
+                          (const void *)bj_factor<float, 1>, (const void *)bj_factor<float, 0>,
                           (const void *)bj_solve<float, false>, (const void *)bj_solve<float, true>})
         (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, A->lds_max);
     A->nblk = nb;
@@ -1968,34 +1991,42 @@ static int precon_build_impl(fvp_csr *A, int type) {
             if (rc) { A->ptype = -1; return rc; }
         }
         const size_t vb = type == FVP_PRECON_BJILU0 ? sizeof(double) : sizeof(float);
-        const size_t staged = bj_lds(A->b_maxnz, vb, A->b_maxrows);
-        const size_t maps = (size_t)(kBjFactorThreads / 64) * A->b_maxrows * sizeof(short);
-        if (maps > (size_t)A->lds_max) {
+        const size_t lds[3] = {bj_factor_lds<0>(A->b_maxnz, vb, A->b_maxrows),
+                               bj_factor_lds<1>(A->b_maxnz, vb, A->b_maxrows),
+                               bj_factor_lds<2>(A->b_maxnz, vb, A->b_maxrows)};
+        if (lds[0] > (size_t)A->lds_max) {
             set_error("block-Jacobi ILU(0): blocks of " + std::to_string(A->b_maxrows) + " rows exceed the LDS");
             A->ptype = -1;
             return -1;
         }
-        const bool stage = staged <= (size_t)A->lds_max;
+        // the most staged form of which two workgroups fit a CU's LDS (with one per CU,
+        // a quarter of the blocks run at a time and the in-place form was measured
+        // faster); FVP_BJ_FACTOR_MODE caps the form, and forces one that fits the
+        // workgroup's LDS (a tuning and test knob)
+        int mode = 2;
+        size_t room = (size_t)A->lds_max / 2;
+        if (const char *e = getenv("FVP_BJ_FACTOR_MODE")) {
+            mode = std::max(0, std::min(2, atoi(e)));
+            room = (size_t)A->lds_max;
+        }
+        while (mode > 0 && lds[mode] > room) --mode;
+        A->factor_mode = mode;
+#define FVP_BJ_FACTOR(TT, M, LU)                                                                             \
+    bj_factor<TT, M><<<A->nblk, kBjFactorThreads, lds[M], s>>>(A->b_roff.p, A->b_maxrows, A->b_maxnz, A->b_ia.p, \
+                                                             A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p,        \
+                                                             A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p, LU,    \
+                                                             A->flag.p)
         if (type == FVP_PRECON_BJILU0) {
-            if (stage)
-                bj_factor<double, true><<<A->nblk, kBjFactorThreads, staged, s>>>(
-                    A->b_roff.p, A->b_maxrows, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p,
-                    A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p, A->b_lu.p, A->flag.p);
-            else
-                bj_factor<double, false><<<A->nblk, kBjFactorThreads, maps, s>>>(
-                    A->b_roff.p, A->b_maxrows, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p,
-                    A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p, A->b_lu.p, A->flag.p);
+            if (mode == 2) FVP_BJ_FACTOR(double, 2, A->b_lu.p);
+            else if (mode == 1) FVP_BJ_FACTOR(double, 1, A->b_lu.p);
+            else FVP_BJ_FACTOR(double, 0, A->b_lu.p);
         } else {
             if (!A->b_lu32.p && A->b_lu32.alloc(A->b_lu.n)) return -1;
-            if (stage)
-                bj_factor<float, true><<<A->nblk, kBjFactorThreads, staged, s>>>(
-                    A->b_roff.p, A->b_maxrows, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p,
-                    A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p, A->b_lu32.p, A->flag.p);
-            else
-                bj_factor<float, false><<<A->nblk, kBjFactorThreads, maps, s>>>(
-                    A->b_roff.p, A->b_maxrows, A->b_maxnz, A->b_ia.p, A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p,
-                    A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p, A->b_lu32.p, A->flag.p);
+            if (mode == 2) FVP_BJ_FACTOR(float, 2, A->b_lu32.p);
+            else if (mode == 1) FVP_BJ_FACTOR(float, 1, A->b_lu32.p);
+            else FVP_BJ_FACTOR(float, 0, A->b_lu32.p);
         }
+#undef FVP_BJ_FACTOR
         // packed solve values of the new factor
         const long nl = (long)A->esrc_l.n, nu = (long)A->esrc_u.n, nd = (long)A->dsrc.n;
         if (type == FVP_PRECON_BJILU0) {

@@ -292,6 +292,38 @@ def test_block_jacobi_ilu0_fp32(box_mid):
     assert np.abs(xs.cpu().numpy() - 1.0).max() < 1e-9
 
 
+@pytest.mark.parametrize("layers", [0, 1])
+def test_block_factor_staging_modes_agree(box_mid, monkeypatch, layers):
+    """The block factorisation's three staging forms (everything in LDS; values
+    and pointers in LDS with the columns read from memory; in place in memory)
+    run the same arithmetic: their solves agree bit for bit, in FP64 and FP32,
+    and the FP64 one is the oracle's blockwise ILU(0)."""
+    import torch
+    from fvmpor_amd.lib import CsrMatrix, PRECON_BJILU0, PRECON_BJILU0_F32, BLOCK_ROWS
+    ia, ja, a = _matrix(box_mid.mesh, seed=4, shift=1.2)
+    n = len(ia) - 1
+    x = np.random.default_rng(9).normal(size=n)
+    xd = _dev(torch, x)
+    for ptype in (PRECON_BJILU0, PRECON_BJILU0_F32):
+        ys = []
+        for cap in (2, 1, 0):
+            monkeypatch.setenv("FVP_BJ_FACTOR_MODE", str(cap))
+            A = CsrMatrix(ia, ja)
+            A.precon_overlap(layers)
+            A.set_values(_dev(torch, a))
+            assert A.precon_build(ptype) == 0
+            y = torch.empty(n, dtype=torch.float64, device=device())
+            A.precon_apply(xd, y)
+            torch.cuda.synchronize()
+            ys.append(y.cpu().numpy())
+        np.testing.assert_array_equal(ys[0], ys[1])
+        np.testing.assert_array_equal(ys[0], ys[2])
+        if ptype == PRECON_BJILU0 and layers == 0:
+            bia, bja, ba, _ = _block_diagonal(ia, ja, a, BLOCK_ROWS)
+            lu_o, dp = O.ilu0(bia, bja, ba)
+            np.testing.assert_allclose(ys[0], O.ilu0_apply(bia, bja, lu_o, dp, x), rtol=1e-11, atol=1e-13)
+
+
 def test_block_rows_rejects_invalid_sizes(box_mid):
     from fvmpor_amd.lib import CsrMatrix, FvpError
     ia, ja, a = _matrix(box_mid.mesh, seed=3, shift=1.2)
