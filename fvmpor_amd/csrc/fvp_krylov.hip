@@ -67,6 +67,8 @@ struct fvp_csr {
     DevBuf<int> b_lev_l, b_lev_u;           // per block: level pointer range [b_lev[b], b_lev[b+1]) into *_lptr
     DevBuf<int> b_lptr_l, b_lptr_u;         // level -> rows range in *_rows
     DevBuf<unsigned short> b_rows_l, b_rows_u;  // local rows grouped by level
+    DevBuf<unsigned> fac_rec, fac_pair;     // factorisation: per entry its pair record (2 words), the pairs
+    DevBuf<long> fac_pbase;                 // per block: its first pair
     // the solve's levels: the factor's levels cut into pieces of at most 64 rows
     DevBuf<int> s_lev_l, s_lev_u, s_lptr_l, s_lptr_u;
     int s_levmax_l = 0, s_levmax_u = 0;
@@ -545,12 +547,7 @@ __global__ void vaxpby(long n, double a, const double *__restrict__ x, double b,
 }
 
 // ---- block-Jacobi ILU(0): one workgroup per block of bs consecutive rows
-// Factorisation: the whole block (values, local columns, row pointers, diagonal
-// offsets) is staged in LDS; one wavefront factors one row (IKJ order): for
-// each L entry k of row i in column order, l_ik = a_ik / u_kk, then the lanes
-// walk row k's U part and subtract l_ik u_kj from a_ij where (i, j) is in the
-// pattern, found through a per-wavefront column -> position map of row i.
-// Rows of one level run on different wavefronts.
+// (factorisation: bj_factor; the rows of one level run concurrently)
 constexpr int kBjFactorThreads = 512;
 
 // compiler-only ordering of one wavefront's LDS accesses (the LDS executes a
@@ -563,113 +560,82 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// MODE 2: the block's values, local columns, row pointers and diagonal
-// offsets are staged in LDS; MODE 1: all but the columns (read from memory,
-// each one loaded a step ahead of its use); MODE 0: the factorisation works in
-// place on the block's values in memory and LDS holds only the column maps.
-// Per wavefront a byte map column -> position in its current row (-1: not in
-// the row; rows have at most 64 entries).  Within a row the L entries are
-// taken in column order; the columns of row k's U part for the next L entry
-// are loaded while the current one is applied.
-template <int MODE>
-__host__ __device__ inline size_t bj_factor_lds(int max_nnz, size_t value_bytes, int max_rows) {
-    const size_t maps = (size_t)(kBjFactorThreads / 64) * max_rows;
-    if (MODE == 0) return maps;
-    size_t s = (size_t)max_nnz * value_bytes + (size_t)(2 * max_rows + 1) * sizeof(unsigned short) + maps;
-    if (MODE == 2) s += (size_t)max_nnz * sizeof(unsigned short);
-    return s;
+// The update pairs of the factorisation are precomputed from the pattern: for
+// the L entry e = (i, k) of a block, rec[e] = {first pair, count | pos(u_kk) << 16}
+// and each pair packs the block positions of u_kj (low half) and a_ij (high
+// half) for the columns j > k that rows i and k share.  A group of kBjGroup
+// lanes factors one row (IKJ order: for each L entry in column order,
+// l_ik = a_ik u_kk^-1, then a_ij -= l_ik u_kj over the entry's pairs, one pair
+// per lane); kBjFactorThreads / kBjGroup rows of a level are in flight per
+// workgroup.  The records of a row are held in the group's registers and the
+// pairs of the next L entry are loaded while the current one is applied.
+// STAGED: the block's values are staged in LDS; otherwise the factorisation
+// works in place on the block's values in memory.
+constexpr int kBjGroup = 16;
+
+__device__ __forceinline__ uint2 group_shfl(uint2 v, int src) {
+    return make_uint2((unsigned)__shfl((int)v.x, src, kBjGroup), (unsigned)__shfl((int)v.y, src, kBjGroup));
 }
 
-template <typename T, int MODE>
+template <typename T, bool STAGED>
 __global__ void __launch_bounds__(kBjFactorThreads) bj_factor(
-    const int *__restrict__ roff, int max_rows, int max_nnz, const int *__restrict__ ia,
-    const unsigned short *__restrict__ ja,
-    const int *__restrict__ diag, const int *__restrict__ src, const double *__restrict__ a,
-    const int *__restrict__ blev, const int *__restrict__ lptr, const unsigned short *__restrict__ rows,
-    T *__restrict__ lu, int *__restrict__ flag) {
+    const int *__restrict__ roff, const int *__restrict__ ia, const int *__restrict__ diag,
+    const int *__restrict__ src, const double *__restrict__ a, const uint2 *__restrict__ rec,
+    const long *__restrict__ pbase, const unsigned *__restrict__ pairs, const int *__restrict__ blev,
+    const int *__restrict__ lptr, const unsigned short *__restrict__ rows, T *__restrict__ lu,
+    int *__restrict__ flag) {
     extern __shared__ double smem[];
     const int b = blockIdx.x;
     const int base = roff[b];
     const int cnt = roff[b + 1] - base;
     const int z0 = ia[base], nzb = ia[base + cnt] - z0;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-    T *val;
-    const unsigned short *col;
-    signed char *pmap;
-    unsigned short *rps = nullptr, *dgs = nullptr;
-    if (MODE > 0) {
-        val = (T *)smem;
-        unsigned short *cs = (unsigned short *)(val + max_nnz);  // MODE 2: the staged columns
-        col = MODE == 2 ? cs : ja + z0;
-        rps = MODE == 2 ? cs + max_nnz : cs;
-        dgs = rps + (max_rows + 1);
-        pmap = (signed char *)(dgs + max_rows);
-        for (int k = threadIdx.x; k < nzb; k += blockDim.x) {
-            val[k] = (T)a[src[z0 + k]];
-            if (MODE == 2) cs[k] = ja[z0 + k];
-        }
-        for (int r = threadIdx.x; r < cnt; r += blockDim.x) {
-            rps[r] = (unsigned short)(ia[base + r] - z0);
-            dgs[r] = (unsigned short)(diag[base + r] - z0);
-        }
-        if (threadIdx.x == 0) rps[cnt] = (unsigned short)nzb;
-    } else {
-        val = lu + z0;
-        col = ja + z0;
-        pmap = (signed char *)smem;
-        for (int k = threadIdx.x; k < nzb; k += blockDim.x) val[k] = (T)a[src[z0 + k]];
-        __threadfence_block();
-    }
-    auto rp = [&](int r) -> int { return MODE > 0 ? (int)rps[r] : ia[base + r] - z0; };
-    auto dg = [&](int r) -> int { return MODE > 0 ? (int)dgs[r] : diag[base + r] - z0; };
-    for (int k = threadIdx.x; k < nwaves * max_rows; k += blockDim.x) pmap[k] = -1;
+    T *val = STAGED ? (T *)smem : lu + z0;
+    for (int k = threadIdx.x; k < nzb; k += blockDim.x) val[k] = (T)a[src[z0 + k]];
+    if (!STAGED) __threadfence_block();
     __syncthreads();
-    signed char *map = pmap + wave * max_rows;
+    const int gl = threadIdx.x & (kBjGroup - 1), g = threadIdx.x / kBjGroup, ng = blockDim.x / kBjGroup;
+    const unsigned *pr = pairs + pbase[b];
+    const uint2 *rb = rec + z0;
     for (int l = blev[b]; l < blev[b + 1]; ++l) {
-        for (int t = lptr[l] + wave; t < lptr[l + 1]; t += nwaves) {
+        for (int t = lptr[l] + g; t < lptr[l + 1]; t += ng) {
             const int r = rows[t];
-            const int s0 = rp(r), len = rp(r + 1) - s0, d = dg(r) - s0;
-            const int cl = lane < len ? (int)col[s0 + lane] : 0;  // row r's columns, one per lane
-            if (lane < len) map[cl] = (signed char)lane;
-            // row k = column of L entry kk: its U part [ks, ke) and this lane's column there
-            int k = 0, ks = 0, ke = 0, cm = -1;
-            if (d > 0) {
-                k = __builtin_amdgcn_readlane(cl, 0);
-                ks = dg(k) + 1;
-                ke = rp(k + 1);
-                cm = ks + lane < ke ? (int)col[ks + lane] : -1;
-            }
-            wave_lds_sync();
+            const int s0 = ia[base + r] - z0, d = diag[base + r] - z0 - s0;
+            uint2 rc = gl < d ? rb[s0 + gl] : make_uint2(0u, 0u);  // records kk = gl (mod kBjGroup)
+            uint2 cur = group_shfl(rc, 0);
+            unsigned q = gl < (cur.y & 0xffffu) ? pr[cur.x + gl] : 0u;
             for (int kk = 0; kk < d; ++kk) {
-                int k2 = 0, ks2 = 0, ke2 = 0, cm2 = -1;
-                if (kk + 1 < d) {
-                    k2 = __builtin_amdgcn_readlane(cl, kk + 1);
-                    ks2 = dg(k2) + 1;
-                    ke2 = rp(k2 + 1);
-                    cm2 = ks2 + lane < ke2 ? (int)col[ks2 + lane] : -1;
+                const int k1 = kk + 1;
+                if (k1 < d && (k1 & (kBjGroup - 1)) == 0) rc = k1 + gl < d ? rb[s0 + k1 + gl] : make_uint2(0u, 0u);
+                uint2 nxt = make_uint2(0u, 0u);
+                unsigned qn = 0u;
+                if (k1 < d) {
+                    nxt = group_shfl(rc, k1 & (kBjGroup - 1));
+                    qn = gl < (nxt.y & 0xffffu) ? pr[nxt.x + gl] : 0u;
                 }
-                const T lik = val[s0 + kk] * val[dg(k)];  // u_kk is stored inverted
-                if (cm >= 0) {
-                    const int p = map[cm];
-                    if (p > kk) val[s0 + p] -= lik * val[ks + lane];
+                const int np = (int)(cur.y & 0xffffu);
+                const T lik = val[s0 + kk] * val[cur.y >> 16];  // u_kk is stored inverted
+                if (gl < np) val[q >> 16] -= lik * val[q & 0xffffu];
+                for (int j = gl + kBjGroup; j < np; j += kBjGroup) {
+                    const unsigned q2 = pr[cur.x + j];
+                    val[q2 >> 16] -= lik * val[q2 & 0xffffu];
                 }
                 wave_lds_sync();
-                if (lane == 0) val[s0 + kk] = lik;
+                if (gl == 0) val[s0 + kk] = lik;
                 wave_lds_sync();
-                k = k2; ks = ks2; ke = ke2; cm = cm2;
+                cur = nxt;
+                q = qn;
             }
-            if (lane == 0) {
+            if (gl == 0) {
                 const T piv = val[s0 + d];
                 if (piv == T(0)) { atomicOr(flag, 1); val[s0 + d] = T(0); }
                 else val[s0 + d] = T(1) / piv;
             }
-            if (lane < len) map[cl] = -1;
             wave_lds_sync();
         }
-        if (MODE == 0) __threadfence_block();
+        if (!STAGED) __threadfence_block();
         __syncthreads();
     }
-    if (MODE > 0)
+    if (STAGED)
         for (int k = threadIdx.x; k < nzb; k += blockDim.x) lu[z0 + k] = val[k];
 }
 
@@ -1677,6 +1643,37 @@ static int build_bj(fvp_csr *A) {
     }
     if (maxrow > 64) { set_error("block-Jacobi ILU(0) supports rows of at most 64 block entries"); return -1; }
     if (maxnz > 65535) { set_error("block-Jacobi ILU(0): more than 65535 entries in one block"); return -1; }
+    // the factorisation's update pairs (see bj_factor)
+    std::vector<unsigned> frec(2 * bsrc.size(), 0u), fpair;
+    std::vector<long> fpbase(nb + 1, 0);
+    {
+        std::vector<int> pos(maxrows, -1);
+        for (int b = 0; b < nb; ++b) {
+            const int base = roff[b], cnt = roff[b + 1] - base, z0 = bia[base];
+            fpbase[b] = (long)fpair.size();
+            for (int r = 0; r < cnt; ++r) {
+                const int s0 = bia[base + r], s1 = bia[base + r + 1], dd = bdiag[base + r];
+                for (int q = s0; q < s1; ++q) pos[bja[q]] = q - z0;
+                for (int e = s0; e < dd; ++e) {
+                    const int k = bja[e], kd = bdiag[base + k], ke = bia[base + k + 1];
+                    const long start = (long)fpair.size() - fpbase[b];
+                    unsigned count = 0;
+                    for (int m = kd + 1; m < ke; ++m) {
+                        const int p = pos[bja[m]];
+                        if (p >= 0) { fpair.push_back((unsigned)(m - z0) | ((unsigned)p << 16)); ++count; }
+                    }
+                    frec[2 * (size_t)e] = (unsigned)start;
+                    frec[2 * (size_t)e + 1] = count | ((unsigned)(kd - z0) << 16);
+                }
+                for (int q = s0; q < s1; ++q) pos[bja[q]] = -1;
+            }
+        }
+        fpbase[nb] = (long)fpair.size();
+        if (fpair.empty()) fpair.push_back(0u);  // a block-diagonal matrix has no pairs
+    }
+    if (A->fac_rec.upload(frec.data(), frec.size()) || A->fac_pair.upload(fpair.data(), fpair.size()) ||
+        A->fac_pbase.upload(fpbase.data(), nb + 1))
+        return -1;
     const bool ident = A->overlap == 0;  // rows of block b are [b bs, ...): no row list needed
     if (A->b_ia.upload(bia.data(), total + 1) || A->b_src.upload(bsrc.data(), bsrc.size()) ||
         A->b_diag.upload(bdiag.data(), total) || A->b_ja.upload(bja.data(), bja.size()) ||
@@ -1781,10 +1778,9 @@ static int build_bj(fvp_csr *A) {
         A->lds_max = 64 * 1024;
     // per-kernel attributes (shared by every matrix): allow the device maximum;
     // each build checks its own block against it
-    for (const void *f : {(const void *)bj_factor<double, 2>, (const void *)bj_factor<double, 1>,
-                          (const void *)bj_factor<double, 0>, (const void *)bj_solve<double, false>,
-                          (const void *)bj_solve<double, true>, (const void *)bj_factor<float, 2>,
-                          (const void *)bj_factor<float, 1>, (const void *)bj_factor<float, 0>,
+    for (const void *f : {(const void *)bj_factor<double, true>, (const void *)bj_factor<double, false>,
+                          (const void *)bj_solve<double, false>, (const void *)bj_solve<double, true>,
+                          (const void *)bj_factor<float, true>, (const void *)bj_factor<float, false>,
                           (const void *)bj_solve<float, false>, (const void *)bj_solve<float, true>})
         (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, A->lds_max);
     A->nblk = nb;
@@ -1991,40 +1987,24 @@ static int precon_build_impl(fvp_csr *A, int type) {
             if (rc) { A->ptype = -1; return rc; }
         }
         const size_t vb = type == FVP_PRECON_BJILU0 ? sizeof(double) : sizeof(float);
-        const size_t lds[3] = {bj_factor_lds<0>(A->b_maxnz, vb, A->b_maxrows),
-                               bj_factor_lds<1>(A->b_maxnz, vb, A->b_maxrows),
-                               bj_factor_lds<2>(A->b_maxnz, vb, A->b_maxrows)};
-        if (lds[0] > (size_t)A->lds_max) {
-            set_error("block-Jacobi ILU(0): blocks of " + std::to_string(A->b_maxrows) + " rows exceed the LDS");
-            A->ptype = -1;
-            return -1;
-        }
-        // the most staged form of which two workgroups fit a CU's LDS (with one per CU,
-        // a quarter of the blocks run at a time and the in-place form was measured
-        // faster); FVP_BJ_FACTOR_MODE caps the form, and forces one that fits the
-        // workgroup's LDS (a tuning and test knob)
-        int mode = 2;
-        size_t room = (size_t)A->lds_max / 2;
-        if (const char *e = getenv("FVP_BJ_FACTOR_MODE")) {
-            mode = std::max(0, std::min(2, atoi(e)));
-            room = (size_t)A->lds_max;
-        }
-        while (mode > 0 && lds[mode] > room) --mode;
-        A->factor_mode = mode;
-#define FVP_BJ_FACTOR(TT, M, LU)                                                                             \
-    bj_factor<TT, M><<<A->nblk, kBjFactorThreads, lds[M], s>>>(A->b_roff.p, A->b_maxrows, A->b_maxnz, A->b_ia.p, \
-                                                             A->b_ja.p, A->b_diag.p, A->b_src.p, A->a.p,        \
-                                                             A->b_lev_l.p, A->b_lptr_l.p, A->b_rows_l.p, LU,    \
-                                                             A->flag.p)
+        // values staged in LDS when two workgroups fit a CU (FVP_BJ_FACTOR_MODE 0 / 1
+        // forces in place / staged where it fits the workgroup: a tuning and test knob)
+        const size_t staged = (size_t)A->b_maxnz * vb;
+        bool stage = staged <= (size_t)A->lds_max / 2;
+        if (const char *e = getenv("FVP_BJ_FACTOR_MODE")) stage = atoi(e) > 0 && staged <= (size_t)A->lds_max;
+        A->factor_mode = stage ? 1 : 0;
+        const uint2 *frec = reinterpret_cast<const uint2 *>(A->fac_rec.p);
+#define FVP_BJ_FACTOR(TT, ST, LU)                                                                                 \
+    bj_factor<TT, ST><<<A->nblk, kBjFactorThreads, ST ? staged : 0, s>>>(                                         \
+        A->b_roff.p, A->b_ia.p, A->b_diag.p, A->b_src.p, A->a.p, frec, A->fac_pbase.p, A->fac_pair.p, A->b_lev_l.p, \
+        A->b_lptr_l.p, A->b_rows_l.p, LU, A->flag.p)
         if (type == FVP_PRECON_BJILU0) {
-            if (mode == 2) FVP_BJ_FACTOR(double, 2, A->b_lu.p);
-            else if (mode == 1) FVP_BJ_FACTOR(double, 1, A->b_lu.p);
-            else FVP_BJ_FACTOR(double, 0, A->b_lu.p);
+            if (stage) FVP_BJ_FACTOR(double, true, A->b_lu.p);
+            else FVP_BJ_FACTOR(double, false, A->b_lu.p);
         } else {
             if (!A->b_lu32.p && A->b_lu32.alloc(A->b_lu.n)) return -1;
-            if (mode == 2) FVP_BJ_FACTOR(float, 2, A->b_lu32.p);
-            else if (mode == 1) FVP_BJ_FACTOR(float, 1, A->b_lu32.p);
-            else FVP_BJ_FACTOR(float, 0, A->b_lu32.p);
+            if (stage) FVP_BJ_FACTOR(float, true, A->b_lu32.p);
+            else FVP_BJ_FACTOR(float, false, A->b_lu32.p);
         }
 #undef FVP_BJ_FACTOR
         // packed solve values of the new factor
@@ -2194,6 +2174,7 @@ extern "C" int fvp_csr_destroy(fvp_csr *A) {
     A->val_u.release(); A->dinv.release(); A->val32_l.release(); A->val32_u.release(); A->dinv32.release();
     A->b_lev_l.release(); A->b_lev_u.release(); A->b_lptr_l.release(); A->b_lptr_u.release();
     A->b_rows_l.release(); A->b_rows_u.release();
+    A->fac_rec.release(); A->fac_pair.release(); A->fac_pbase.release();
     A->s_lev_l.release(); A->s_lev_u.release(); A->s_lptr_l.release(); A->s_lptr_u.release();
     A->ov_ptr.release(); A->ov_pos.release(); A->ov_tri.release();
     schur_free(A);

@@ -294,10 +294,9 @@ def test_block_jacobi_ilu0_fp32(box_mid):
 
 @pytest.mark.parametrize("layers", [0, 1])
 def test_block_factor_staging_modes_agree(box_mid, monkeypatch, layers):
-    """The block factorisation's three staging forms (everything in LDS; values
-    and pointers in LDS with the columns read from memory; in place in memory)
-    run the same arithmetic: their solves agree bit for bit, in FP64 and FP32,
-    and the FP64 one is the oracle's blockwise ILU(0)."""
+    """The block factorisation's two forms (values staged in LDS; in place in
+    memory) run the same arithmetic: their solves agree bit for bit, in FP64
+    and FP32, and the FP64 one is the oracle's blockwise ILU(0)."""
     import torch
     from fvmpor_amd.lib import CsrMatrix, PRECON_BJILU0, PRECON_BJILU0_F32, BLOCK_ROWS
     ia, ja, a = _matrix(box_mid.mesh, seed=4, shift=1.2)
@@ -306,7 +305,7 @@ def test_block_factor_staging_modes_agree(box_mid, monkeypatch, layers):
     xd = _dev(torch, x)
     for ptype in (PRECON_BJILU0, PRECON_BJILU0_F32):
         ys = []
-        for cap in (2, 1, 0):
+        for cap in (1, 0):
             monkeypatch.setenv("FVP_BJ_FACTOR_MODE", str(cap))
             A = CsrMatrix(ia, ja)
             A.precon_overlap(layers)
@@ -317,7 +316,6 @@ def test_block_factor_staging_modes_agree(box_mid, monkeypatch, layers):
             torch.cuda.synchronize()
             ys.append(y.cpu().numpy())
         np.testing.assert_array_equal(ys[0], ys[1])
-        np.testing.assert_array_equal(ys[0], ys[2])
         if ptype == PRECON_BJILU0 and layers == 0:
             bia, bja, ba, _ = _block_diagonal(ia, ja, a, BLOCK_ROWS)
             lu_o, dp = O.ilu0(bia, bja, ba)
