@@ -132,6 +132,17 @@ def cpu_baseline(args, prob, plain, gpu_res_us):
         op.residual(plain["t0"], psi, psip)
         times.append(time.perf_counter() - a)
     res_s = float(np.median(times))
+    # the 2D slice (the CPU-runnable configuration): residual median
+    from fvmpor_amd.problems import slice2d_problem
+    sp_ = slice2d_problem()
+    sop = O.OracleProblem(sp_)
+    spsi, spsip = sp_.psi0.copy(), np.zeros(sp_.mesh.n_nodes)
+    sop.begin_step(0.0, spsi, spsip)
+    stimes = []
+    while len(stimes) < 20 or sum(stimes) < 1.0:
+        a = time.perf_counter()
+        sop.residual(0.0, spsi, spsip)
+        stimes.append(time.perf_counter() - a)
     hs = np.asarray(plain["hs"], dtype=np.float64)
     ks = np.asarray(plain["ks"], dtype=np.int32)
     gpu_step_s = np.asarray(plain["step_s"], dtype=np.float64)
@@ -157,11 +168,41 @@ def cpu_baseline(args, prob, plain, gpu_res_us):
                    f"{st_tot['jac_setups']} Jacobian set-up(s), {st_tot['residuals']} residuals in {wall:.1f} s"),
         "residual_median_s": res_s,
         "residual_evals_timed": len(times),
+        "slice_residual_median_s": float(np.median(stimes)),
+        "slice_nodes": sp_.mesh.n_nodes,
         "gpu_residual_us": gpu_res_us,
         "oracle_build_s": build_s,
         "gpu_plain_fp64_same_steps": gpu_same,
         "gpu_over_cpu_same_steps": gpu_same / (days / wall),
     }
+
+
+def slice_residual_gpu():
+    """The 2D slice (the CPU-runnable configuration, 4 141 nodes): median of 50
+    whole residual evaluations (HIP events) at the hydrostatic start."""
+    import torch
+    from fvmpor_amd.lib import Context
+    from fvmpor_amd.problems import slice2d_problem
+    prob = slice2d_problem()
+    ctx = Context(prob)
+    stream = torch.cuda.current_stream()
+    ctx.set_stream(stream.cuda_stream)
+    psi = torch.tensor(prob.psi0, device="cuda:0")
+    psip = torch.zeros_like(psi)
+    F = torch.empty_like(psi)
+    ctx.begin_step(0.0, psi, psip)
+    for _ in range(5):
+        ctx.residual(0.0, psi, psip, F)
+    us = []
+    for _ in range(50):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ctx.residual(0.0, psi, psip, F)
+        e1.record(stream)
+        e1.synchronize()
+        us.append(e0.elapsed_time(e1) * 1e3)
+    ctx.close()
+    return {"nodes": prob.mesh.n_nodes, "elements": prob.mesh.n_elements, "gpu_residual_median_us": float(np.median(us))}
 
 
 def measured_traffic(n_nodes, n_elements, launch):
@@ -469,6 +510,7 @@ def main():
         for _ in range(3):
             b.copy_(a)
         torch.cuda.synchronize()
+    out["slice_residual"] = slice_residual_gpu()
     if not args.no_cpu:
         try:
             log("CPU oracle baseline")
