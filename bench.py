@@ -355,7 +355,6 @@ def main():
     psi0 = torch.tensor(prob.psi0, device=dev)
     pc64 = {"bjilu0": PRECON_BJILU0, "ilu0": PRECON_ILU0, "jacobi": PRECON_JACOBI,
             "schur": PRECON_SCHUR}[args.precon]
-    ncol, _ = ctx.colours()
     if args.launch == "auto":
         launch = ctx.autotune(0.0, psi0, torch.zeros_like(psi0))
     else:
@@ -371,7 +370,7 @@ def main():
         torch.cuda.synchronize()
         learned = (model, stat)
     torch.cuda.synchronize()
-    log(f"setup {time.perf_counter() - t0:.1f}s, {ncol} colours, N_edges={ctx.NE}, residual launch {launch}")
+    log(f"setup {time.perf_counter() - t0:.1f}s, N_edges={ctx.NE}, residual launch {launch}")
 
     head_pc = PRECON_BJILU0_F32 if args.precon == "bjilu0" and not args.no_fp32 else pc64
     head_name = ("FP32-precon" if head_pc == PRECON_BJILU0_F32 else "plain FP64") + \

@@ -70,8 +70,10 @@ namespace fvp {
 
 // learned start: a try is strong when the candidate's preconditioned residual
 // is below this share of the predictor's; weak tries in a row before a back-off
+// (one: a weak try costs a network pass, a residual and two preconditioner
+// solves, and on the 1M-node box no try was strong)
 constexpr double kLearnStrong = 0.5;
-constexpr int kLearnWeakRun = 3;
+constexpr int kLearnWeakRun = 1;
 
 struct Coef8 {
     double v[8];
@@ -394,7 +396,7 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
         FVP_HIP(hipMemsetAsync(I->delta.p, 0, sizeof(double) * N, s));
         double bn = 0.;
         (void)wrms(I, I->G.p, &bn);
-        fvp_gmres_params gp{I->p.dim_krylov, I->p.max_restarts, 1, bn > 0 ? tol_abs / bn : 1.};
+        fvp_gmres_params gp{I->p.dim_krylov, I->p.max_restarts, I->p.reorthogonalise, bn > 0 ? tol_abs / bn : 1.};
         fvp_gmres_stats gs{};
         const double *yv = I->y.p, *ypv = I->ypn.p, *Gv = I->G.p, *ew = I->ewt.p;
         double *tmpv = I->tmp.p;
@@ -556,6 +558,7 @@ extern "C" int fvp_integrator_create(fvp_ctx *c, const fvp_integrator_params *p,
     if (p->max_order < 1 || p->max_order > KMAX) { set_error("max_order must be 1..5"); return -1; }
     if (!(p->rtol >= 0) || !(p->atol > 0)) { set_error("tolerances must be positive"); return -1; }
     if (p->dim_krylov < 1 || p->dim_krylov > 63) { set_error("dim_krylov must be 1..63"); return -1; }
+    if (p->reorthogonalise < 0 || p->reorthogonalise > 2) { set_error("reorthogonalise must be 0, 1 or 2"); return -1; }
     FVP_HIP(hipSetDevice(c->device));
     fvp_integrator *I = new fvp_integrator();
     I->c = c;

@@ -275,6 +275,26 @@ __device__ __forceinline__ void block_partial(double v, double *__restrict__ out
     }
 }
 
+// Block partials of K values at once: every wavefront reduces all K with
+// shuffles, one barrier, then thread c sums the wavefronts' values of column c
+// into part[c * gridDim.x + block].
+template <int K>
+__device__ __forceinline__ void block_partials(const double (&acc)[K], int nk, double *__restrict__ part) {
+    __shared__ double lds[kThreads / 64][K];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+        const double v = wave_reduce(acc[c]);
+        if (lane == 0) lds[wv][c] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nk) {
+        double t = 0.;
+        for (int k = 0; k < kThreads / 64; ++k) t += lds[k][threadIdx.x];
+        part[threadIdx.x * gridDim.x + blockIdx.x] = t;
+    }
+}
+
 // Block partials of w . Q[:, c] for c < NC and of w . w (stored as column NC
 // when ww is set): part[c * gridDim.x + block].  Each thread reads element
 // pairs (16-byte loads; the basis columns are padded to an even length, ldq),
@@ -304,13 +324,7 @@ __global__ void __launch_bounds__(kThreads) mdot_kernel(long n, const double *__
         for (int c = 0; c < NC; ++c) acc[c] += a * Q[c * ldq + n - 1];
         acc[NC] += a * a;
     }
-#pragma unroll
-    for (int c = 0; c <= NC; ++c) {
-        if (c < NC || ww) {
-            block_partial(acc[c], part + c * gridDim.x + blockIdx.x);
-            __syncthreads();
-        }
-    }
+    block_partials<NC + 1>(acc, ww ? NC + 1 : NC, part);
 }
 
 // w -= Q[:, 0..NC) coef, and block partials of the updated w . w into
@@ -387,11 +401,7 @@ __global__ void __launch_bounds__(kThreads) axpy_mdot(long n, double *__restrict
         for (int c = 0; c < NC; ++c) acc[c] += a * qt[c];
         acc[NC] += a * a;
     }
-#pragma unroll
-    for (int c = 0; c <= NC; ++c) {
-        block_partial(acc[c], part + c * gridDim.x + blockIdx.x);
-        __syncthreads();
-    }
+    block_partials<NC + 1>(acc, NC + 1, part);
 }
 
 using MdotLaunch = void (*)(hipStream_t, int, long, const double *, const double *, long, int, double *,
@@ -2380,6 +2390,7 @@ extern "C" int fvp_gmres_csr_noalloc(fvp_csr *A, const double *b, double *x, con
 // ================================================================================
 struct fvp_jacdata {
     fvp_csr *J = nullptr;
+    bool coloured = false;             // colouring built (colour_init)
     int ncol = 0;
     std::vector<int> colour;           // per node
     std::vector<int> cptr;             // colour -> [cptr[c], cptr[c+1]) in cols
@@ -2432,25 +2443,88 @@ static int jac_init(fvp_ctx *c) {
     const int N = c->N, T = c->T, npe = c->npe;
     std::vector<int4> conn(T);
     FVP_HIP(hipMemcpy(conn.data(), c->conn.p, sizeof(int4) * T, hipMemcpyDeviceToHost));
-    // adjacency lists
-    std::vector<std::vector<int>> adj(N);
+    // node adjacency (the Jacobian pattern): every element's node pairs, counted
+    // into per-row segments, then sorted and made unique row by row
+    std::vector<int> off(N + 1, 0), buf;
     for (int e = 0; e < T; ++e) {
-        int n[4] = {conn[e].x, conn[e].y, conn[e].z, conn[e].w};
-        for (int a = 0; a < npe; ++a)
-            for (int b = 0; b < npe; ++b) adj[n[a]].push_back(n[b]);
+        const int n[4] = {conn[e].x, conn[e].y, conn[e].z, conn[e].w};
+        for (int a = 0; a < npe; ++a) off[n[a] + 1] += npe;
+    }
+    for (int i = 0; i < N; ++i) off[i + 1] += off[i];
+    buf.resize(off[N]);
+    {
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        for (int e = 0; e < T; ++e) {
+            const int n[4] = {conn[e].x, conn[e].y, conn[e].z, conn[e].w};
+            for (int a = 0; a < npe; ++a)
+                for (int b = 0; b < npe; ++b) buf[fill[n[a]]++] = n[b];
+        }
     }
     std::vector<int> ia(N + 1, 0), ja;
+    ja.reserve(off[N] / 4);
     for (int i = 0; i < N; ++i) {
-        auto &v = adj[i];
-        std::sort(v.begin(), v.end());
-        v.erase(std::unique(v.begin(), v.end()), v.end());
-        ia[i + 1] = ia[i] + (int)v.size();
+        auto b0 = buf.begin() + off[i], b1 = buf.begin() + off[i + 1];
+        std::sort(b0, b1);
+        ja.insert(ja.end(), b0, std::unique(b0, b1));
+        ia[i + 1] = (int)ja.size();
     }
-    ja.reserve(ia[N]);
-    for (int i = 0; i < N; ++i) ja.insert(ja.end(), adj[i].begin(), adj[i].end());
+    std::vector<int>().swap(buf);
     fvp_jacdata *jd = new fvp_jacdata();
     if (csr_create_impl(c->device, N, ia.data(), ja.data(), &jd->J)) { delete jd; return -1; }
     fvp_csr_set_stream(jd->J, c->stream);
+    {
+        // each entry's contribution slots, counting-sorted by entry (element order within)
+        const long nnz = ia[N];
+        std::vector<int> pos((size_t)npe * npe * T), dpos(N), srank(16 * (size_t)T, 0);
+        std::vector<long> cp(nnz + 1, 0);
+        auto at = [&](int row, int col) {
+            return (int)(std::lower_bound(ja.begin() + ia[row], ja.begin() + ia[row + 1], col) - ja.begin());
+        };
+        for (int e = 0; e < T; ++e) {
+            const int n[4] = {conn[e].x, conn[e].y, conn[e].z, conn[e].w};
+            for (int i = 0; i < npe; ++i)
+                for (int k = 0; k < npe; ++k) {
+                    const int p = at(n[i], n[k]);
+                    pos[((size_t)e * npe + i) * npe + k] = p;
+                    ++cp[p + 1];
+                }
+        }
+        for (long p = 0; p < nnz; ++p) cp[p + 1] += cp[p];
+        {
+            std::vector<long> fill(cp.begin(), cp.end() - 1);
+            for (int e = 0; e < T; ++e)
+                for (int i = 0; i < npe; ++i)
+                    for (int k = 0; k < npe; ++k)
+                        srank[16 * (size_t)e + 4 * i + k] = (int)fill[pos[((size_t)e * npe + i) * npe + k]]++;
+        }
+        for (int i = 0; i < N; ++i) dpos[i] = at(i, i);
+        if ((long)16 * T > INT32_MAX) { set_error("too many elements for the FD Jacobian slots"); fvp_csr_destroy(jd->J); delete jd; return -1; }
+        if (jd->cptr2.upload(cp.data(), cp.size()) || jd->srank.upload(srank.data(), srank.size()) ||
+            jd->dpos.upload(dpos.data(), N) || jd->contrib.alloc((size_t)npe * npe * T) ||
+            jd->kr2.alloc((size_t)c->nz * N) || jd->cap2.alloc(N)) {
+            fvp_csr_destroy(jd->J);
+            delete jd;
+            return -1;
+        }
+    }
+    if (jd->shift.alloc(N) || jd->ys.alloc(N) || jd->yps.alloc(N) || jd->Fs.alloc(N)) {
+        fvp_csr_destroy(jd->J);
+        delete jd;
+        return -1;
+    }
+    c->jd = jd;
+    return 0;
+}
+
+// The reference's greedy distance-2 colouring in node order and, per colour,
+// the entries its columns fill (the coloured FD Jacobian and fvp_colours only;
+// built on first use)
+static int colour_init(fvp_ctx *c) {
+    if (jac_init(c)) return -1;
+    fvp_jacdata *jd = c->jd;
+    if (jd->coloured) return 0;
+    const int N = c->N;
+    const std::vector<int> &ia = jd->J->ia_h, &ja = jd->J->ja_h;
     // greedy distance-2 colouring in node order
     std::vector<int> colour(N, N - 1), mark(N, -1);
     int maxc = 0;
@@ -2491,49 +2565,10 @@ static int jac_init(fvp_ctx *c) {
     }
     jd->cptr = cptr;
     jd->eptr = eptr;
-    {
-        // each entry's contribution slots, counting-sorted by entry (element order within)
-        const long nnz = ia[N];
-        std::vector<int> pos((size_t)npe * npe * T), dpos(N), srank(16 * (size_t)T, 0);
-        std::vector<long> cp(nnz + 1, 0);
-        auto at = [&](int row, int col) {
-            return (int)(std::lower_bound(ja.begin() + ia[row], ja.begin() + ia[row + 1], col) - ja.begin());
-        };
-        for (int e = 0; e < T; ++e) {
-            const int n[4] = {conn[e].x, conn[e].y, conn[e].z, conn[e].w};
-            for (int i = 0; i < npe; ++i)
-                for (int k = 0; k < npe; ++k) {
-                    const int p = at(n[i], n[k]);
-                    pos[((size_t)e * npe + i) * npe + k] = p;
-                    ++cp[p + 1];
-                }
-        }
-        for (long p = 0; p < nnz; ++p) cp[p + 1] += cp[p];
-        {
-            std::vector<long> fill(cp.begin(), cp.end() - 1);
-            for (int e = 0; e < T; ++e)
-                for (int i = 0; i < npe; ++i)
-                    for (int k = 0; k < npe; ++k)
-                        srank[16 * (size_t)e + 4 * i + k] = (int)fill[pos[((size_t)e * npe + i) * npe + k]]++;
-        }
-        for (int i = 0; i < N; ++i) dpos[i] = at(i, i);
-        if ((long)16 * T > INT32_MAX) { set_error("too many elements for the FD Jacobian slots"); fvp_csr_destroy(jd->J); delete jd; return -1; }
-        if (jd->cptr2.upload(cp.data(), cp.size()) || jd->srank.upload(srank.data(), srank.size()) ||
-            jd->dpos.upload(dpos.data(), N) || jd->contrib.alloc((size_t)npe * npe * T) ||
-            jd->kr2.alloc((size_t)c->nz * N) || jd->cap2.alloc(N)) {
-            fvp_csr_destroy(jd->J);
-            delete jd;
-            return -1;
-        }
-    }
     if (jd->cols.upload(cols.data(), N) || jd->erow.upload(erow.data(), erow.size()) ||
-        jd->epos.upload(epos.data(), epos.size()) || jd->ecol.upload(ecol.data(), ecol.size()) ||
-        jd->shift.alloc(N) || jd->ys.alloc(N) || jd->yps.alloc(N) || jd->Fs.alloc(N)) {
-        fvp_csr_destroy(jd->J);
-        delete jd;
+        jd->epos.upload(epos.data(), epos.size()) || jd->ecol.upload(ecol.data(), ecol.size()))
         return -1;
-    }
-    c->jd = jd;
+    jd->coloured = true;
     return 0;
 }
 
@@ -2574,6 +2609,7 @@ extern "C" int fvp_precon_setup(fvp_ctx *c, double t, double c_j, double h, cons
             return -1;
         return precon_build_impl(jd->J, type);
     }
+    if (colour_init(c)) return -1;
     vcopy<<<nblk(N, 256), 256, 0, s>>>(N, psi, jd->ys.p);
     vcopy<<<nblk(N, 256), 256, 0, s>>>(N, psip, jd->yps.p);
     for (int k = 0; k < jd->ncol; ++k) {
@@ -2656,7 +2692,7 @@ extern "C" int fvp_set_overlap(fvp_ctx *c, int layers) {
 extern "C" int fvp_colours(fvp_ctx *c, int *n_colours, int *colour_host) {
     if (!c || !n_colours) { set_error("null argument"); return -1; }
     FVP_HIP(hipSetDevice(c->device));
-    if (jac_init(c)) return -1;
+    if (colour_init(c)) return -1;
     *n_colours = c->jd->ncol;
     if (colour_host) std::copy(c->jd->colour.begin(), c->jd->colour.end(), colour_host);
     return 0;

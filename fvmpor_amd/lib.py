@@ -435,7 +435,7 @@ class IntegratorParams(ctypes.Structure):
                 ("h0", ctypes.c_double), ("h_max", ctypes.c_double), ("precon_type", ctypes.c_int),
                 ("dim_krylov", ctypes.c_int), ("max_restarts", ctypes.c_int), ("lin_factor", ctypes.c_double),
                 ("newton_tol", ctypes.c_double), ("max_newton", ctypes.c_int), ("max_steps", ctypes.c_int),
-                ("block_rows", ctypes.c_int), ("overlap", ctypes.c_int)]
+                ("block_rows", ctypes.c_int), ("overlap", ctypes.c_int), ("reorthogonalise", ctypes.c_int)]
 
 
 class IntegratorStats(ctypes.Structure):
@@ -453,10 +453,11 @@ class Integrator:
 
     def __init__(self, ctx: Context, psi0, t0=0.0, rtol=1e-8, atol=1e-8, max_order=3, h0=0.0, h_max=0.0,
                  precon=PRECON_ILU0, dim_krylov=20, max_restarts=5, lin_factor=0.05, newton_tol=0.33,
-                 max_newton=4, max_steps=1000000, block_rows=0, overlap=-1):
+                 max_newton=4, max_steps=1000000, block_rows=0, overlap=-1, reorthogonalise=0):
         self.ctx = ctx
         self.params = IntegratorParams(rtol, atol, max_order, h0, h_max, precon, dim_krylov, max_restarts,
-                                       lin_factor, newton_tol, max_newton, max_steps, block_rows, overlap)
+                                       lin_factor, newton_tol, max_newton, max_steps, block_rows, overlap,
+                                       reorthogonalise)
         h = ctypes.c_void_p()
         _check(lib().fvp_integrator_create(ctx.h, ctypes.byref(self.params), float(t0), _ptr(psi0),
                                            ctypes.byref(h)), "fvp_integrator_create")

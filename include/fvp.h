@@ -274,6 +274,9 @@ typedef struct {
     int max_steps;        /* per advance call */
     int block_rows;       /* block-Jacobi block size, 0 = FVP_BLOCK_ROWS */
     int overlap;          /* block-Jacobi overlap layers (0..2); < 0 keeps the context's setting */
+    int reorthogonalise;  /* GMRES second Gram-Schmidt pass: 0 never (IDA's SPGMR makes a single
+                             modified Gram-Schmidt pass), 1 when the first cancelled more than half
+                             of ||w||^2, 2 always */
 } fvp_integrator_params;
 
 typedef struct {

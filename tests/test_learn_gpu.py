@@ -41,10 +41,10 @@ def test_a_network_that_proposes_nothing_keeps_the_plain_start(box3d_small):
     n_params = 16 * MPNN_HIDDEN + MPNN_HIDDEN + 3 * (2 * MPNN_HIDDEN * MPNN_HIDDEN + MPNN_HIDDEN) + MPNN_HIDDEN + 1
     hp, sp, _ = run(box3d_small, 0.05)
     hz, sz, _ = run(box3d_small, 0.05, model_params=np.zeros(n_params, dtype=np.float32))
-    # a proposal equal to the predictor never wins, and after three such tries
-    # the network backs off (sits out 4 attempts, then 8, ...)
-    assert sz["learned_used"] == 0 and sz["learned_tried"] >= min(3, sz["steps"])
-    assert sz["steps"] < 8 or sz["learned_tried"] < sz["steps"]
+    # a proposal equal to the predictor never wins, and after such a try the
+    # network backs off (sits out 4 attempts, then 8, ...)
+    assert sz["learned_used"] == 0 and sz["learned_tried"] >= 1
+    assert sz["steps"] < 6 or sz["learned_tried"] < sz["steps"] / 2
     for k in ("steps", "newton_iters", "error_fails"):
         assert sz[k] == sp[k], k
     assert np.abs(hz - hp).max() <= 1e-10 * np.abs(hp).max()
