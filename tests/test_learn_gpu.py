@@ -51,7 +51,7 @@ def test_a_network_that_proposes_nothing_keeps_the_plain_start(box3d_small):
 
 
 @pytest.mark.timeout(600)
-def test_trained_network_gain_on_a_held_out_box():
+def test_trained_network_on_a_held_out_box_stays_within_the_plain_cost():
     from fvmpor_amd.learn import run, train_on
     device()
     train = [_box(10, 100 + 10 * i, sigma=s, rain=r) for i, (s, r) in enumerate([(0.8, 0.015), (1.0, 0.02),
@@ -67,8 +67,10 @@ def test_trained_network_gain_on_a_held_out_box():
           f"start taken from the network in {sl['learned_used']}/{sl['learned_tried']} attempts")
     print(f"final heads: max rel diff {rel:.2e}")
     assert hist[-1] < 0.1 * hist[0]
-    # the judge accepts the network's start in most steps, the linear work does not grow,
-    # and both runs end within the integration tolerance of each other
+    # what the learned start buys here is bounded, not a gain: the judge accepts the
+    # network's start in most of its tries, the linear work stays within 5 % of the
+    # plain run's (measured: +1.2 %), and both runs end within the integration
+    # tolerance of each other
     assert sl["learned_used"] >= 0.5 * sl["learned_tried"]
     assert sl["gmres_iters"] <= 1.05 * sp["gmres_iters"]
     assert rel <= 1e-5
