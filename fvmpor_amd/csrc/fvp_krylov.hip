@@ -237,13 +237,6 @@ __global__ void vscale_inv_norm(long n, const double *__restrict__ src, const do
     if (i < n) dst[i] = src[i] * (1. / sqrt(nsq[0]));
 }
 
-// dst = src * s[0] (s a device scalar: the reciprocal of the new basis vector's norm)
-__global__ void vscale_dev(long n, const double *__restrict__ src, const double *__restrict__ sc,
-                           double *__restrict__ dst) {
-    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[i] * sc[0];
-}
-
 constexpr int kChunk = 16;  // basis columns per Gram-Schmidt dot pass
 
 __device__ __forceinline__ double wave_reduce(double v) {
@@ -465,24 +458,33 @@ __host__ __device__ inline void drotg_dev(double *da, double *db, double *c, dou
     *da = r; *db = z;
 }
 
-// New Hessenberg column j = h1 (+ h2 after a second pass), h_{j+1,j} = ||w||;
-// previous rotations, the new Givens rotation, the rotated right-hand side g,
-// the residual estimate |g_{j+1}| and h_{j+1,j} into stat, 1/h_{j+1,j} into ninv.
-__global__ void arnoldi_update(int j, int m, const double *__restrict__ h1, const double *__restrict__ h2,
-                               const double *__restrict__ n1, const double *__restrict__ n2,
-                               const double *__restrict__ flag, double *__restrict__ H, double *__restrict__ cs,
-                               double *__restrict__ sn, double *__restrict__ g, double *__restrict__ stat,
-                               double *__restrict__ ninv) {
-    if (threadIdx.x != 0) return;
+// End of Arnoldi step j in one launch.  Every block forms 1/h_{j+1,j} from the
+// same norm (||w||^2 after the first pass, or after the second when flag is
+// set) and scales its slice of w into q_{j+1}.  Block 0 also sets the new
+// Hessenberg column j = h1 (+ h2 after a second pass) with h_{j+1,j} = ||w||,
+// applies the previous rotations and the new Givens rotation, rotates the
+// right-hand side g, and writes {|g_{j+1}|, h_{j+1,j}} to stat and, when hstat
+// is given (pinned host memory the device can write), there too.
+__global__ void __launch_bounds__(256) arnoldi_finish(int j, int m, const double *__restrict__ h1,
+                                                      const double *__restrict__ h2, const double *__restrict__ n1,
+                                                      const double *__restrict__ n2, const double *__restrict__ flag,
+                                                      double *__restrict__ H, double *__restrict__ cs,
+                                                      double *__restrict__ sn, double *__restrict__ g,
+                                                      double *__restrict__ stat, double *__restrict__ hstat, long n,
+                                                      const double *__restrict__ w, double *__restrict__ q) {
     const bool two = flag && flag[0] != 0.;
-    double *hc = H + (size_t)j * (m + 1);
-    for (int i = 0; i <= j; ++i) hc[i] = two ? h1[i] + h2[i] : h1[i];
     const double hn = sqrt(two ? n2[0] : n1[0]);
+    const double ninv = hn > 0. ? 1. / hn : 0.;
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) q[i] = w[i] * ninv;
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double *hc = H + (size_t)j * (m + 1);
+    for (int k = 0; k <= j; ++k) hc[k] = two ? h1[k] + h2[k] : h1[k];
     hc[j + 1] = hn;
-    for (int i = 0; i < j; ++i) {
-        const double t1 = cs[i] * hc[i] + sn[i] * hc[i + 1];
-        hc[i + 1] = cs[i] * hc[i + 1] - sn[i] * hc[i];
-        hc[i] = t1;
+    for (int k = 0; k < j; ++k) {
+        const double t1 = cs[k] * hc[k] + sn[k] * hc[k + 1];
+        hc[k + 1] = cs[k] * hc[k + 1] - sn[k] * hc[k];
+        hc[k] = t1;
     }
     double a = hc[j], b = hc[j + 1], c, sv;
     drotg_dev(&a, &b, &c, &sv);
@@ -494,7 +496,11 @@ __global__ void arnoldi_update(int j, int m, const double *__restrict__ h1, cons
     g[j + 1] = -sv * gj;
     stat[0] = fabs(g[j + 1]);
     stat[1] = hn;
-    ninv[0] = hn > 0. ? 1. / hn : 0.;
+    if (hstat) {
+        hstat[0] = stat[0];
+        hstat[1] = hn;
+        __threadfence_system();
+    }
 }
 
 __global__ void gmres_cycle_init(double *__restrict__ g, int len, double beta) {
@@ -1170,6 +1176,7 @@ struct GmresWork {
     double *Q, *w, *r, *z, *part, *sc, *hdev, *Hd, *gd, *csd, *snd, *yd;
     double *hstat;
     const hipEvent_t *ev;
+    bool hmapped;  // hstat is pinned host memory the kernels write directly
 };
 
 // doubles of a GmresWork carved from one device buffer (each piece 256-byte aligned)
@@ -1210,7 +1217,7 @@ int gmres_device(fvp_csr *W, hipStream_t s, long n, const Op &A, const Op &M, co
     if (p->reorthogonalise < 0 || p->reorthogonalise > 2) { set_error("reorthogonalise must be 0, 1 or 2"); return -1; }
     if (ensure_work(W, m)) return -1;
     GmresWork wk{W->Q.p, W->w.p, W->r.p, W->z.p, W->part.p, W->scal.p, W->hdev.p, W->Hd.p, W->gd.p, W->csd.p,
-                 W->snd.p, W->yd.p, W->hstat, W->ev.data()};
+                 W->snd.p, W->yd.p, W->hstat, W->ev.data(), true};
     return gmres_core(wk, s, n, A, M, b, x, p, st);
 }
 
@@ -1221,8 +1228,8 @@ static int gmres_core(const GmresWork &W, hipStream_t s, long n, const Op &A, co
     // h1, h2: first / second pass coefficients (j+1 dots, then ||w||^2); stat: {estimate, h_{j+1,j}}
     double *h1 = W.hdev, *h2 = W.hdev + (m + 2), *stat = W.hdev + 2 * (m + 2);
     // device scalars: sc[0] ||r||^2, sc[1] ||b||^2, sc[2] first-pass ||w||^2 after projection,
-    // sc[3] second-pass ||w||^2, sc[4] reorthogonalisation flag, sc[5] 1 / h_{j+1,j}
-    double *n1 = sc + 2, *n2 = sc + 3, *flag = sc + 4, *ninv = sc + 5;
+    // sc[3] second-pass ||w||^2, sc[4] reorthogonalisation flag
+    double *n1 = sc + 2, *n2 = sc + 3, *flag = sc + 4;
     const long ldq = gs_ldq(n);
     const int mode = p->reorthogonalise;
     fvp_gmres_stats stats{0, 0, 0, 0., 0.};
@@ -1270,10 +1277,12 @@ static int gmres_core(const GmresWork &W, hipStream_t s, long n, const Op &A, co
                     }
                 }
                 const bool fused = mode && j + 1 <= kChunk;  // ||w||^2 after the first pass in h2[j + 1]
-                arnoldi_update<<<1, 64, 0, s>>>(j, m, h1, h2, fused ? h2 + j + 1 : n1, n2, mode ? flag : nullptr,
-                                                W.Hd, W.csd, W.snd, W.gd, stat, ninv);
-                vscale_dev<<<nblk(n, 256), 256, 0, s>>>(n, w, ninv, Q + (size_t)(j + 1) * ldq);
-                FVP_HIP(hipMemcpyAsync(W.hstat + 2 * j, stat, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+                arnoldi_finish<<<nblk(n, 256), 256, 0, s>>>(j, m, h1, h2, fused ? h2 + j + 1 : n1, n2,
+                                                            mode ? flag : nullptr, W.Hd, W.csd, W.snd, W.gd, stat,
+                                                            W.hmapped ? W.hstat + 2 * j : nullptr, n, w,
+                                                            Q + (size_t)(j + 1) * ldq);
+                if (!W.hmapped)
+                    FVP_HIP(hipMemcpyAsync(W.hstat + 2 * j, stat, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
                 if (W.ev) FVP_HIP(hipEventRecord(W.ev[j], s));
                 else FVP_HIP(hipStreamSynchronize(s));
                 FVP_HIP(hipGetLastError());
@@ -2377,8 +2386,9 @@ extern "C" int fvp_gmres_csr_noalloc(fvp_csr *A, const double *b, double *x, con
     zero_fill<<<(int)std::min(1024L, (small + 255) / 256), 256, 0, A->stream>>>(small, wk.sc);
     FVP_HIP(hipGetLastError());
     double hstat[2 * 64];
-    wk.hstat = hstat;
+    wk.hstat = hstat;  // pageable: each step's pair is copied and read synchronously
     wk.ev = nullptr;
+    wk.hmapped = false;
     Op op = [A](const double *in, double *out) { return csr_gemv_impl(A, 1.0, in, 0.0, out); };
     Op pc;
     if (A->ptype != FVP_PRECON_NONE) pc = [A](const double *in, double *out) { return precon_apply_impl(A, in, out); };
