@@ -140,7 +140,7 @@ int jv_impl(fvp_ctx *c, double t, const double *psi, const double *psip, const d
 // element-local FD Jacobian into J (nnz values); the arrays as in fvp_jacdata
 int fd_jacobian_elem(fvp_ctx *c, double cj, const double *psi, const double *psip, const double *shift, double *ys,
                      double *kr2, double *cap2, double *contrib, const long *cptr, const int *srank, const int *ia,
-                     const int *dpos, double *J, long nnz);
+                     const int *dpos, double *J, long nnz, bool fp32 = false);
 // z = M^-1 (r / rdiv) with the context's preconditioner (tmp: a work vector of
 // N).  With zn2 given, the block-Jacobi solves also leave partial sums of
 // ||z||^2: *zn2 points at them and *zn2_count is their number (null and 0 for

@@ -2615,7 +2615,8 @@ extern "C" int fvp_precon_setup(fvp_ctx *c, double t, double c_j, double h, cons
     shift_kernel<<<nblk(N, 256), 256, 0, s>>>(N, eps, h, psi, psip, ewt, jd->shift.p);
     if (!c->averaging && c->jac_mode == FVP_JACOBIAN_ELEMENT) {
         if (fd_jacobian_elem(c, c_j, psi, psip, jd->shift.p, jd->ys.p, jd->kr2.p, jd->cap2.p, jd->contrib.p,
-                             jd->cptr2.p, jd->srank.p, jd->J->ia.p, jd->dpos.p, jd->J->a.p, jd->J->nnz))
+                             jd->cptr2.p, jd->srank.p, jd->J->ia.p, jd->dpos.p, jd->J->a.p, jd->J->nnz,
+                             type == FVP_PRECON_BJILU0_F32))
             return -1;
         return precon_build_impl(jd->J, type);
     }

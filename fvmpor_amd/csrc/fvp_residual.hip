@@ -803,6 +803,120 @@ __global__ void __launch_bounds__(256) fd_elem_kernel(int T, int N, ZoneTable zt
     }
 }
 
+// The same element-local quotients in FP32 arithmetic (the FP32
+// preconditioner's input).  For a fixed k_r a face flux is linear in the
+// heads, so with node k moved by s_k the difference quotient of the face's
+// mass flow M = rho0 k_r flux is, exactly,
+//   (M1 - M0) / s_k = rho0 [ (k_r1 - k_r0) / s_k * flux + k_r1 * dflux/dh_k ],
+// k_r1 differing from k_r0 only when node k is the face's upwind node.  Each
+// term is formed directly (dflux/dh_k from the adjugate, (k_r1 - k_r0) / s_k
+// in FP64 from the two van Genuchten passes), so nothing cancels and FP32
+// carries the quotient to its rounding; the geometry is built once per
+// element, relative to its first node.  Same increments, same upwind bits,
+// same contribution slots as fd_elem_kernel.
+template <int DIM>
+__global__ void __launch_bounds__(256) fd_elem_kernel32(int T, int N, ZoneTable zt, double rho0,
+                                                        const int4 *__restrict__ conn,
+                                                        const uint8_t *__restrict__ ezone,
+                                                        const double *__restrict__ kmul,
+                                                        const uint8_t *__restrict__ emask,
+                                                        const double *__restrict__ nrec, const double *__restrict__ kr,
+                                                        const double *__restrict__ kr2, const double *__restrict__ shift,
+                                                        const double *__restrict__ invV, const int *__restrict__ srank,
+                                                        float *__restrict__ contrib) {
+    constexpr int NPE = DIM + 1;
+    constexpr int NED = DIM == 2 ? 3 : 6;
+    __shared__ float zk[3 * kMaxZones];
+    if (threadIdx.x < 3 * zt.nz) zk[threadIdx.x] = (float)zt.p[threadIdx.x / 3][ZK_X + threadIdx.x % 3];
+    __syncthreads();
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= T) return;
+    const ElemRec rec = load_elem<0>(e, conn, ezone, kmul, emask, nullptr);
+    NodeVals<DIM> nv;
+    load_nodes_rec<DIM>(rec, N, nrec, kr, nv);
+    const int n[4] = {rec.c4.x, rec.c4.y, rec.c4.z, rec.c4.w};
+    float G[NPE][3], det;
+    if (DIM == 3) {
+        const float ux = (float)(nv.px[1] - nv.px[0]), uy = (float)(nv.py[1] - nv.py[0]), uz = (float)(nv.pz[1] - nv.pz[0]);
+        const float vx = (float)(nv.px[2] - nv.px[0]), vy = (float)(nv.py[2] - nv.py[0]), vz = (float)(nv.pz[2] - nv.pz[0]);
+        const float wx = (float)(nv.px[3] - nv.px[0]), wy = (float)(nv.py[3] - nv.py[0]), wz = (float)(nv.pz[3] - nv.pz[0]);
+        G[1][0] = vy * wz - vz * wy; G[1][1] = vz * wx - vx * wz; G[1][2] = vx * wy - vy * wx;
+        G[2][0] = wy * uz - wz * uy; G[2][1] = wz * ux - wx * uz; G[2][2] = wx * uy - wy * ux;
+        G[3][0] = uy * vz - uz * vy; G[3][1] = uz * vx - ux * vz; G[3][2] = ux * vy - uy * vx;
+        det = ux * G[1][0] + uy * G[1][1] + uz * G[1][2];
+    } else {
+        const float ux = (float)(nv.px[1] - nv.px[0]), uy = (float)(nv.py[1] - nv.py[0]);
+        const float vx = (float)(nv.px[2] - nv.px[0]), vy = (float)(nv.py[2] - nv.py[0]);
+        G[1][0] = vy; G[1][1] = -vx; G[1][2] = 0.f;
+        G[2][0] = -uy; G[2][1] = ux; G[2][2] = 0.f;
+        det = ux * vy - uy * vx;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) G[0][d] = -(G[1][d] + G[2][d] + (DIM == 3 ? G[NPE - 1][d] : 0.f));
+    const float inv = 1.f / det;
+    const float km = (float)rec.km;
+    const float Kx = -(zk[3 * rec.z] * km), Ky = -(zk[3 * rec.z + 1] * km), Kz = -(zk[3 * rec.z + 2] * km);
+    // base state: head gradient, Darcy flux without k_r, face fluxes
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll
+    for (int k = 1; k < NPE; ++k) {
+        const float dh = (float)(nv.h[k] - nv.h[0]);
+        gx += dh * G[k][0]; gy += dh * G[k][1]; gz += dh * G[k][2];
+    }
+    gx *= inv; gy *= inv; gz *= inv;
+    float qx = gx * Kx, qy, qz;
+    if (DIM == 2) { qy = (gy + 1.f) * Ky; qz = 0.f; }
+    else { qy = gy * Ky; qz = (gz + 1.f) * Kz; }
+    float g[NPE];
+#pragma unroll
+    for (int k = 0; k < NPE; ++k) g[k] = G[k][0] * qx + G[k][1] * qy + G[k][2] * qz;
+    const float sc = (det > 0.f ? 1.f : -1.f) / (DIM == 3 ? 24.f : 6.f);
+    const float r0 = (float)rho0;
+    const double *k2 = kr2 + (size_t)rec.z * N;
+    float w[NPE], kb[NPE], dkr[NPE];
+#pragma unroll
+    for (int i = 0; i < NPE; ++i) {
+        w[i] = (float)invV[n[i]];
+        kb[i] = (float)nv.k[i];
+    }
+#pragma unroll
+    for (int k = 0; k < NPE; ++k) {
+        const double sk = shift[n[k]], k1 = k2[n[k]];
+        dkr[k] = (float)((k1 - nv.k[k]) / sk);
+        // d(grad h)/dh_k = G[k] / det, so dq/dh_k = K (G[k] / det) (no gravity term)
+        const float dqx = G[k][0] * inv * Kx, dqy = G[k][1] * inv * Ky, dqz = G[k][2] * inv * Kz;
+        float dg[NPE];
+#pragma unroll
+        for (int m = 0; m < NPE; ++m) dg[m] = G[m][0] * dqx + G[m][1] * dqy + G[m][2] * dqz;
+        float d[4] = {0.f, 0.f, 0.f, 0.f};
+        const float kmv = (float)k1;
+#pragma unroll
+        for (int j = 0; j < NED; ++j) {
+            const int a = edge_a(j), b = edge_b(j);
+            const unsigned up_a = (rec.mask >> j) & 1u;  // 1: node a is upwind
+            const int u = up_a ? a : b;
+            const float flux = sc * (g[b] - g[a]);
+            const float dflux = sc * (dg[b] - dg[a]);
+            const float kr1 = u == k ? kmv : (up_a ? kb[a] : kb[b]);
+            const float dM = r0 * ((u == k ? dkr[k] : 0.f) * flux + kr1 * dflux);
+            d[a] += dM;  // (f0 - f1) / s_k at a: f[a] -= M
+            d[b] -= dM;
+        }
+#pragma unroll
+        for (int i = 0; i < NPE; ++i) contrib[srank[16 * (size_t)e + 4 * i + k]] = d[i] * w[i];
+    }
+}
+
+// J[p] = sum of entry p's FP32 contributions (element order), accumulated in FP64
+__global__ void fd_gather32(long nnz, const long *__restrict__ cptr, const float *__restrict__ contrib,
+                            double *__restrict__ J) {
+    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (p >= nnz) return;
+    double s = 0.;
+    for (long q = cptr[p]; q < cptr[p + 1]; ++q) s += (double)contrib[q];
+    J[p] = s;
+}
+
 // J[p] = sum of entry p's contributions contrib[cptr[p] .. cptr[p + 1]) (element order)
 __global__ void fd_gather(long nnz, const long *__restrict__ cptr, const double *__restrict__ contrib,
                           double *__restrict__ J) {
@@ -841,7 +955,7 @@ __global__ void fd_add(long n, const double *__restrict__ a, const double *__res
 
 int fd_jacobian_elem(fvp_ctx *c, double cj, const double *psi, const double *psip, const double *shift, double *ys,
                      double *kr2, double *cap2, double *contrib, const long *cptr, const int *srank, const int *ia,
-                     const int *dpos, double *J, long nnz) {
+                     const int *dpos, double *J, long nnz, bool fp32) {
     hipStream_t s = c->stream;
     const int N = c->N, T = c->T;
     // base state: k_r, capacity and the node records' heads
@@ -851,17 +965,32 @@ int fd_jacobian_elem(fvp_ctx *c, double cj, const double *psi, const double *psi
     fd_add<<<blocks(N, 256), 256, 0, s>>>(N, psi, shift, ys);
     vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, ys, c->zmask.p, c->zw.p, kr2, cap2, nullptr,
                                                      nullptr, nullptr);
-    if (c->dim == 3)
-        fd_elem_kernel<3><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, (const int4 *)c->conn.p,
-                                                          (const uint8_t *)c->ezone.p, c->kmul.p,
-                                                          (const uint8_t *)c->emask.p, c->nrec.p, c->kr.p, kr2, shift,
-                                                          c->invV.p, srank, contrib);
-    else
-        fd_elem_kernel<2><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, (const int4 *)c->conn.p,
-                                                          (const uint8_t *)c->ezone.p, c->kmul.p,
-                                                          (const uint8_t *)c->emask.p, c->nrec.p, c->kr.p, kr2, shift,
-                                                          c->invV.p, srank, contrib);
-    fd_gather<<<blocks(nnz, 256), 256, 0, s>>>(nnz, cptr, contrib, J);
+    if (fp32) {
+        float *c32 = reinterpret_cast<float *>(contrib);
+        if (c->dim == 3)
+            fd_elem_kernel32<3><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, (const int4 *)c->conn.p,
+                                                                (const uint8_t *)c->ezone.p, c->kmul.p,
+                                                                (const uint8_t *)c->emask.p, c->nrec.p, c->kr.p, kr2,
+                                                                shift, c->invV.p, srank, c32);
+        else
+            fd_elem_kernel32<2><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, (const int4 *)c->conn.p,
+                                                                (const uint8_t *)c->ezone.p, c->kmul.p,
+                                                                (const uint8_t *)c->emask.p, c->nrec.p, c->kr.p, kr2,
+                                                                shift, c->invV.p, srank, c32);
+        fd_gather32<<<blocks(nnz, 256), 256, 0, s>>>(nnz, cptr, c32, J);
+    } else {
+        if (c->dim == 3)
+            fd_elem_kernel<3><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, (const int4 *)c->conn.p,
+                                                              (const uint8_t *)c->ezone.p, c->kmul.p,
+                                                              (const uint8_t *)c->emask.p, c->nrec.p, c->kr.p, kr2,
+                                                              shift, c->invV.p, srank, contrib);
+        else
+            fd_elem_kernel<2><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, (const int4 *)c->conn.p,
+                                                              (const uint8_t *)c->ezone.p, c->kmul.p,
+                                                              (const uint8_t *)c->emask.p, c->nrec.p, c->kr.p, kr2,
+                                                              shift, c->invV.p, srank, contrib);
+        fd_gather<<<blocks(nnz, 256), 256, 0, s>>>(nnz, cptr, contrib, J);
+    }
     fd_node_kernel<<<blocks(N, 256), 256, 0, s>>>(N, cj, psip, shift, c->cap.p, cap2, dpos, J);
     if (c->ndir)
         fd_dirichlet<<<blocks(c->ndir, 256), 256, 0, s>>>(c->ndir, c->dnode.p, c->dval.p, psi, shift, ia, dpos, J);

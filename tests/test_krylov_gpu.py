@@ -216,6 +216,56 @@ def test_element_local_jacobian_matches_coloured(which, request):
         ctx.set_jacobian_mode(7)
 
 
+@pytest.mark.parametrize("which", ["slice2d", "box3d_small", "slice2d_dirichlet"])
+def test_fp32_jacobian_of_the_fp32_preconditioner(which, request):
+    """The FP32 preconditioner's Jacobian: the element-local quotients in FP32
+    arithmetic, each formed without cancellation from the linearity of the
+    face flux in the heads.  Against the coloured FD form (the reference's) on
+    the same state and increments it stays within the FD noise bar of the
+    FP64 forms; against the FP64 element-local form (whose own FD noise is the
+    larger part) likewise; Dirichlet rows are equal."""
+    import torch
+    import dataclasses
+    from fvmpor_amd.lib import Context, JACOBIAN_COLOURED, PRECON_BJILU0, PRECON_BJILU0_F32
+    if which == "slice2d_dirichlet":
+        base = request.getfixturevalue("slice2d")
+        bc = base.bc.copy()
+        bc[2, 0] = -1.0
+        prob = dataclasses.replace(base, bc_type=np.array([0, 3, 1], dtype=np.int32), bc=bc)
+    else:
+        prob = request.getfixturevalue(which)
+    ctx = Context(prob)
+    op = O.OracleProblem(prob)
+    rng = np.random.default_rng(11)
+    y = prob.psi0 - 0.3 + rng.uniform(-0.2, 0.2, op.N)
+    yp = rng.normal(scale=0.05, size=op.N)
+    ty, typ = _dev(torch, y), _dev(torch, yp)
+    ctx.begin_step(0.0, ty, typ)
+    F0 = torch.empty_like(ty)
+    ctx.residual(0.0, ty, typ, F0)
+    ewt = _dev(torch, 1.0 / (1e-6 * np.abs(y) + 1e-6))
+    ia, ja = O.jac_pattern(op.omesh)
+    J = ctx.jacobian()
+    assert ctx.precon_setup(0.0, 12.0, 0.05, ty, typ, F0, ewt, PRECON_BJILU0_F32) == 0
+    j32 = J.get_values(len(ja))
+    assert ctx.precon_setup(0.0, 12.0, 0.05, ty, typ, F0, ewt, PRECON_BJILU0) == 0
+    j64 = J.get_values(len(ja))
+    ctx.set_jacobian_mode(JACOBIAN_COLOURED)
+    assert ctx.precon_setup(0.0, 12.0, 0.05, ty, typ, F0, ewt, PRECON_BJILU0) == 0
+    col = J.get_values(len(ja))
+    rowmax = np.repeat(np.maximum.reduceat(np.abs(col), ia[:-1]), np.diff(ia))
+    e_col = (np.abs(j32 - col) / rowmax).max()
+    e_64 = (np.abs(j32 - j64) / rowmax).max()
+    print(f"{which}: FP32 Jacobian vs coloured {e_col:.2e}, vs FP64 element-local {e_64:.2e}")
+    assert e_col < 1e-4 and e_64 < 1e-4
+    if which == "slice2d_dirichlet":
+        rows = np.repeat(np.arange(len(ia) - 1), np.diff(ia))
+        drow = np.array([np.all(col[ia[i]:ia[i + 1]][ja[ia[i]:ia[i + 1]] != i] == 0) for i in range(len(ia) - 1)])
+        assert drow.sum() > 0
+        sel = drow[rows]
+        assert np.array_equal(j32[sel], j64[sel])
+
+
 @pytest.fixture(scope="module")
 def box_mid():
     from fvmpor_amd.problems import box3d_problem
