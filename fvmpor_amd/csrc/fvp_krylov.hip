@@ -72,6 +72,7 @@ struct fvp_csr {
     // the solve's levels: the factor's levels cut into pieces of at most 64 rows
     DevBuf<int> s_lev_l, s_lev_u, s_lptr_l, s_lptr_u;
     int s_levmax_l = 0, s_levmax_u = 0;
+    int s_maxslots = 0;                     // most slots of one block's triangle
     // packed solve layout per triangle (see bj_solve): per slot its meta word
     // pair, per block its entry base, per entry its source in b_lu (-1:
     // padding) and local column, the values of the current factor; per U slot
@@ -777,49 +778,57 @@ template <typename T>
 constexpr int bj_ahead() { return sizeof(T) == 4 ? BJ_AHEAD_F32 : BJ_AHEAD_F64; }
 
 // One triangle of one block: thread t takes the t-th slot of each level (the
-// solve's levels hold at most 64 slots).  No global load with a
-// data-dependent count is issued inside the level loop, and every pass runs
-// all kBjAhead levels (those past the last have no lane with work), so every
-// path into a level has issued the same loads after that level's own and the
-// compiler's wait counts stay exact.
+// solve's levels hold at most 64 slots).  The triangle's meta words are first
+// staged in LDS (ms), so a level's window addresses come from LDS and only the
+// windows themselves are loads in flight: kA levels ahead, one window per lane
+// and level, each consumed by the same unrolled stage that issued it kA
+// stages earlier (no loop-carried register holds a pending load, so the
+// compiler's wait counts keep all kA levels in flight).  No global load with
+// a data-dependent count is issued inside the level loop, and every pass runs
+// all kA levels (those past the last have no lane with work).
 template <typename T, bool UPPER, bool OVG>
-__device__ __forceinline__ void bj_sweep(T *v, int nl, const int *lp, const uint2 *__restrict__ meta,
+__device__ __forceinline__ void bj_sweep(T *v, int nl, const int *lp, const uint2 *__restrict__ meta, uint2 *ms,
                                          const T *__restrict__ val, const unsigned short *__restrict__ col,
                                          const T *__restrict__ dinv, const uint4 *ovp) {
     const int t = threadIdx.x;
     if (nl <= 0) return;
     constexpr int kA = bj_ahead<T>();
-    BjMeta mr[kA];
-    BjWin<T> wr[kA];
+    const int s0 = lp[0], ns = lp[nl] - s0;
+    // stage the meta words of the triangle's slots, 8 loads per lane in flight
+    for (int k0 = t; k0 < ns; k0 += 64 * 8) {
+        uint2 q[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) q[i] = meta[s0 + min(k0 + 64 * i, ns - 1)];
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (k0 + 64 * i < ns) ms[k0 + 64 * i] = q[i];
+    }
+    wave_lds_order();
     auto fetch_meta = [&](int l, BjMeta &m) {
         const int lc = min(l, nl - 1);
         const int p0 = lp[lc], p1 = lp[lc + 1];
         const int cnt = p1 - p0;
         m.s = p0 + min(t, cnt - 1);
-        const uint2 q = meta[m.s];
+        const uint2 q = ms[m.s - s0];
         m.m0 = q.x;
         m.m1 = q.y;
         m.have = l < nl && t < cnt;
     };
-    // issued stage by stage and ring slot by ring slot (the barriers keep the
-    // scheduler from moving a slot's loads behind the next slot's)
+    BjWin<T> wr[kA];
 #pragma unroll
     for (int d = 0; d < kA; ++d) {
-        fetch_meta(d, mr[d]);
-        wave_lds_order();
-    }
-#pragma unroll
-    for (int d = 0; d < kA; ++d) {
-        bj_win_load<T, UPPER>(wr[d], mr[d], val, col, dinv);
-        fetch_meta(d + kA, mr[d]);
+        BjMeta m;
+        fetch_meta(d, m);
+        bj_win_load<T, UPPER>(wr[d], m, val, col, dinv);
         wave_lds_order();
     }
     for (int lb = 0; lb < nl; lb += kA) {
 #pragma unroll
         for (int d = 0; d < kA; ++d) {
             if (wr[d].have) bj_win_update<T, UPPER, OVG>(wr[d], v, ovp, val, col);
-            bj_win_load<T, UPPER>(wr[d], mr[d], val, col, dinv);  // level lb + d + kA
-            fetch_meta(lb + d + 2 * kA, mr[d]);
+            BjMeta m;
+            fetch_meta(lb + d + kA, m);
+            bj_win_load<T, UPPER>(wr[d], m, val, col, dinv);  // level lb + d + kA
             // one wavefront: the LDS executes its accesses in order, so only
             // the compiler must keep them in program order
             wave_lds_order();
@@ -828,9 +837,12 @@ __device__ __forceinline__ void bj_sweep(T *v, int nl, const int *lp, const uint
 }
 
 // LDS bytes of one block's solve: the solution (T x maxrows), the block's
-// overflow entries (16 bytes each; none with OVG), its L and U level pointers
-__host__ __device__ inline size_t bj_solve_lds(size_t tsize, int maxrows, int ovmax, int levl, int levu) {
-    return ((tsize * (size_t)maxrows + 15) & ~(size_t)15) + 16 * (size_t)ovmax + 4 * (size_t)(levl + levu + 2);
+// overflow entries (16 bytes each; none with OVG), its L and U level pointers,
+// the meta words of one triangle's slots
+__host__ __device__ inline size_t bj_solve_lds(size_t tsize, int maxrows, int ovmax, int levl, int levu,
+                                                int maxslots) {
+    return ((tsize * (size_t)maxrows + 15) & ~(size_t)15) + 16 * (size_t)ovmax +
+           ((4 * (size_t)(levl + levu + 2) + 7) & ~(size_t)7) + 8 * (size_t)maxslots;
 }
 
 // rowid: global row of each local row (nullptr: block b's rows are the
@@ -854,7 +866,8 @@ __global__ void __launch_bounds__(64) bj_solve(const int *__restrict__ roff, con
                                                const T *__restrict__ val_u, const unsigned short *__restrict__ col_u,
                                                const T *__restrict__ dinv, const int *__restrict__ ov_ptr,
                                                const long *__restrict__ ov_pos, const unsigned char *__restrict__ ov_tri,
-                                               int maxrows, int ovmax, const double *__restrict__ x,
+                                               int maxrows, int ovmax, int levmax_l, int levmax_u,
+                                               const double *__restrict__ x,
                                                const double *__restrict__ xdiv, double *__restrict__ y,
                                                double *__restrict__ yn2) {
     extern __shared__ double smem[];
@@ -906,11 +919,13 @@ __global__ void __launch_bounds__(64) bj_solve(const int *__restrict__ roff, con
     const int l0 = blev_l[b], nl = blev_l[b + 1] - l0;
     const int u0 = blev_u[b], nu = blev_u[b + 1] - u0;
     int *lpu = lpl + nl + 1;
+    uint2 *ms = reinterpret_cast<uint2 *>(reinterpret_cast<char *>(lpl) +
+                                          ((4 * (size_t)(levmax_l + levmax_u + 2) + 7) & ~(size_t)7));
     for (int k = threadIdx.x; k <= nl; k += 64) lpl[k] = lptr_l[l0 + k];
     for (int k = threadIdx.x; k <= nu; k += 64) lpu[k] = lptr_u[u0 + k];
     wave_lds_order();
-    bj_sweep<T, false, OVG>(v, nl, lpl, meta_l, val_l + ebase_l[b], col_l + ebase_l[b], nullptr, ovp);
-    bj_sweep<T, true, OVG>(v, nu, lpu, meta_u, val_u + ebase_u[b], col_u + ebase_u[b], dinv, ovp);
+    bj_sweep<T, false, OVG>(v, nl, lpl, meta_l, ms, val_l + ebase_l[b], col_l + ebase_l[b], nullptr, ovp);
+    bj_sweep<T, true, OVG>(v, nu, lpu, meta_u, ms, val_u + ebase_u[b], col_u + ebase_u[b], dinv, ovp);
     const int olo = own[2 * b], ohi = own[2 * b + 1];
     double q = 0.;
     for (int r0 = olo + threadIdx.x; r0 < ohi; r0 += 64 * kBjStage) {
@@ -1776,6 +1791,9 @@ static int build_bj(fvp_csr *A) {
         std::vector<int> sbl, spl, sbu, spu;
         split(blev_l, lptr_l, sbl, spl, A->s_levmax_l);
         split(blev_u, lptr_u, sbu, spu, A->s_levmax_u);
+        A->s_maxslots = 0;
+        for (int b = 0; b < nb; ++b)
+            A->s_maxslots = std::max({A->s_maxslots, spl[sbl[b + 1]] - spl[sbl[b]], spu[sbu[b + 1]] - spu[sbu[b]]});
         if (A->meta_l.upload(tr[0].meta.data(), tr[0].meta.size()) ||
             A->meta_u.upload(tr[1].meta.data(), tr[1].meta.size()) ||
             A->ebase_l.upload(tr[0].ebase.data(), nb + 1) || A->ebase_u.upload(tr[1].ebase.data(), nb + 1) ||
@@ -2094,9 +2112,9 @@ static int precon_apply_impl(fvp_csr *A, const double *x, double *y, const doubl
     case FVP_PRECON_BJILU0_F32: {
         const bool f64 = A->ptype == FVP_PRECON_BJILU0;
         const size_t ts = f64 ? sizeof(double) : sizeof(float);
-        size_t lds = bj_solve_lds(ts, A->b_maxrows, A->ov_max, A->s_levmax_l, A->s_levmax_u);
+        size_t lds = bj_solve_lds(ts, A->b_maxrows, A->ov_max, A->s_levmax_l, A->s_levmax_u, A->s_maxslots);
         const bool ovg = lds > (size_t)A->lds_max;  // overflow lists too long for the LDS: read them in memory
-        if (ovg) lds = bj_solve_lds(ts, A->b_maxrows, 0, A->s_levmax_l, A->s_levmax_u);
+        if (ovg) lds = bj_solve_lds(ts, A->b_maxrows, 0, A->s_levmax_l, A->s_levmax_u, A->s_maxslots);
         if (lds > (size_t)A->lds_max) {
             set_error("block-Jacobi ILU(0): a block's solution does not fit the LDS");
             return -1;
@@ -2106,7 +2124,8 @@ static int precon_apply_impl(fvp_csr *A, const double *x, double *y, const doubl
     bj_solve<TT, OV><<<A->nblk, 64, lds, s>>>(A->b_roff.p, A->b_rowid.p, A->b_own.p, A->s_lev_l.p, A->s_lptr_l.p, \
                                               ml, A->ebase_l.p, VL, A->col_l.p, A->s_lev_u.p, A->s_lptr_u.p, mu, \
                                               A->ebase_u.p, VU, A->col_u.p, DI, A->ov_ptr.p, A->ov_pos.p,         \
-                                              A->ov_tri.p, A->b_maxrows, A->ov_max, x, xdiv, y, yn2)
+                                              A->ov_tri.p, A->b_maxrows, A->ov_max, A->s_levmax_l,       \
+                                              A->s_levmax_u, x, xdiv, y, yn2)
         if (f64) {
             if (ovg) FVP_BJ_SOLVE(double, true, A->val_l.p, A->val_u.p, A->dinv.p);
             else FVP_BJ_SOLVE(double, false, A->val_l.p, A->val_u.p, A->dinv.p);
