@@ -69,7 +69,7 @@ struct fvp_csr {
     DevBuf<unsigned short> b_rows_l, b_rows_u;  // local rows grouped by level
     DevBuf<unsigned> fac_rec, fac_pair;     // factorisation: per entry its pair record (2 words), the pairs
     DevBuf<long> fac_pbase;                 // per block: its first pair
-    // the solve's levels: the factor's levels cut into pieces of at most 64 rows
+    // the solve's levels: the factor's levels cut into pieces of at most 32 rows
     DevBuf<int> s_lev_l, s_lev_u, s_lptr_l, s_lptr_u;
     int s_levmax_l = 0, s_levmax_u = 0;
     int s_maxslots = 0;                     // most slots of one block's triangle
@@ -662,16 +662,15 @@ __global__ void __launch_bounds__(kBjFactorThreads) bj_factor(
 // The factor streams from a level-ordered packed copy per triangle: the row
 // parts one after another (each padded to an even length), so the bytes read
 // are the factor's own.  Per row part (a "slot", level order) a meta word pair
-// gives its start and its overflow offset, its row and its length; lane t of
-// a level reads the fixed window of kBjRow entries starting at its slot's
-// start (past the row's length the window runs into the following rows, which
-// the level uses anyway, and is masked), the entries past kBjRow come from
-// LDS.  Two prefetch stages: the meta words of level l + 2 kBjAhead and the
-// windows of level l + kBjAhead (whose meta words arrived kBjAhead levels
-// earlier) are loaded while level l is solved, so the level chain waits only
-// on LDS.  The entries of a row are subtracted in column order, as the
-// sequential solve does.  T is the factor's arithmetic type (double, or float
-// for the FP32 preconditioner); x and y stay FP64.
+// gives its start and its overflow offset, its row and its length.  A lane
+// pair takes a slot: the pair reads the fixed window of kBjRow entries
+// starting at the slot's start, one half each (past the row's length the
+// window runs into the following rows, which the level uses anyway, and is
+// masked); the halves' sums meet by a swap within the pair, and the even lane
+// subtracts the entries past kBjRow (from LDS) and stores the row.  So a level
+// of up to 32 slots costs one half window of instructions per lane.  T is the
+// factor's arithmetic type (double, or float for the FP32 preconditioner); x
+// and y stay FP64.
 constexpr int kBjRow = 12;
 #ifndef BJ_AHEAD_F32
 #define BJ_AHEAD_F32 8
@@ -690,14 +689,29 @@ __global__ void bj_pack_vals(long n, const int *__restrict__ src, const T *__res
     val[k] = q >= 0 ? lu[q] : T(0);
 }
 
+constexpr int kBjHalf = kBjRow / 2;  // window entries per lane of a pair
+
 template <typename T>
 struct BjWin {
-    T a[kBjRow];
-    unsigned c[kBjRow / 2];  // local columns, two per word
-    unsigned m0, m1;         // meta words
-    T d;                     // inverted diagonal (U)
+    T a[kBjHalf];
+    unsigned c[kBjHalf / 2];  // local columns, two per word
+    unsigned m0, m1;          // meta words
+    T d;                      // inverted diagonal (U)
     bool have;
 };
+
+// x plus the other lane of the pair (lanes 2i, 2i+1): a DPP quad permutation
+// [1, 0, 3, 2], so both lanes get the same sum
+__device__ __forceinline__ float pair_sum(float x) {
+    const int y = __builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false);
+    return x + __int_as_float(y);
+}
+__device__ __forceinline__ double pair_sum(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, 0xB1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
+    return x + __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
 struct BjMeta {
     unsigned m0, m1;  // start (low 16) | overflow offset (high 16); row (low 16) | length (high 16)
     int s;            // slot
@@ -705,14 +719,14 @@ struct BjMeta {
 };
 
 template <typename T, bool UPPER>
-__device__ __forceinline__ void bj_win_load(BjWin<T> &w, const BjMeta &m, const T *__restrict__ val,
+__device__ __forceinline__ void bj_win_load(BjWin<T> &w, const BjMeta &m, int half, const T *__restrict__ val,
                                             const unsigned short *__restrict__ col, const T *__restrict__ dinv) {
-    const long k = (long)(m.m0 & 0xffffu);  // even: the window starts 2-entry aligned
-    // values: 8-byte (FP32) or 16-byte (FP64) aligned pairs
+    // this lane's half of the window: even start, so pairs stay aligned
+    const long k = (long)(m.m0 & 0xffffu) + half * kBjHalf;
     if (sizeof(T) == 4) {
         const uint2 *p = reinterpret_cast<const uint2 *>(val + k);
 #pragma unroll
-        for (int i = 0; i < kBjRow / 2; ++i) {
+        for (int i = 0; i < kBjHalf / 2; ++i) {
             const uint2 q = p[i];
             w.a[2 * i] = (T)__uint_as_float(q.x);
             w.a[2 * i + 1] = (T)__uint_as_float(q.y);
@@ -720,7 +734,7 @@ __device__ __forceinline__ void bj_win_load(BjWin<T> &w, const BjMeta &m, const 
     } else {
         const double2 *p = reinterpret_cast<const double2 *>(val + k);
 #pragma unroll
-        for (int i = 0; i < kBjRow / 2; ++i) {
+        for (int i = 0; i < kBjHalf / 2; ++i) {
             const double2 q = p[i];
             w.a[2 * i] = (T)q.x;
             w.a[2 * i + 1] = (T)q.y;
@@ -728,31 +742,37 @@ __device__ __forceinline__ void bj_win_load(BjWin<T> &w, const BjMeta &m, const 
     }
     const unsigned *pc = reinterpret_cast<const unsigned *>(col + k);
 #pragma unroll
-    for (int i = 0; i < kBjRow / 2; ++i) w.c[i] = pc[i];
+    for (int i = 0; i < kBjHalf / 2; ++i) w.c[i] = pc[i];
     w.m0 = m.m0;
     w.m1 = m.m1;
     w.d = UPPER ? dinv[m.s] : T(0);
     w.have = m.have;
 }
 
-// The row's entries are subtracted in column order, the overflow ones (from
-// LDS) after the first kBjRow, as the sequential solve does.
+// Each lane of the pair sums its half window; the even lane then finishes
+// the row: s = v[r] - (both halves) - (entries past kBjRow, in column order),
+// times the inverted diagonal for U.  Every lane takes part in the pair sum
+// (lanes without a slot read a valid clamped slot and store nothing).
 // OVG: the overflow entries are read from the packed arrays in memory (blocks
 // whose overflow lists do not fit the LDS) instead of from LDS
 template <typename T, bool UPPER, bool OVG>
-__device__ __forceinline__ void bj_win_update(const BjWin<T> &w, T *v, const uint4 *ovp, const T *__restrict__ val,
-                                              const unsigned short *__restrict__ col) {
+__device__ __forceinline__ void bj_win_update(const BjWin<T> &w, int half, T *v, const uint4 *ovp,
+                                              const T *__restrict__ val, const unsigned short *__restrict__ col) {
     const int r = (int)(w.m1 & 0xffffu), kn = (int)(w.m1 >> 16);
-    T s = v[r];
+    T p = T(0);
 #pragma unroll
-    for (int u = 0; u < kBjRow; ++u) {
+    for (int u = 0; u < kBjHalf; ++u) {
         // past the row's length the window holds the following rows' entries
         // (possibly of another block): masked to 0 * v[r]
+        const int ug = half * kBjHalf + u;
         const unsigned cw = w.c[u / 2];
-        const int c = u < kn ? (int)((u & 1) ? (cw >> 16) : (cw & 0xffffu)) : r;
-        const T a = u < kn ? w.a[u] : T(0);
-        s -= a * v[c];
+        const int c = ug < kn ? (int)((u & 1) ? (cw >> 16) : (cw & 0xffffu)) : r;
+        const T a = ug < kn ? w.a[u] : T(0);
+        p += a * v[c];
     }
+    p = pair_sum(p);
+    if (!w.have || half) return;
+    T s = v[r] - p;
     if (kn > kBjRow) {
         if (OVG) {  // overflow entries straight after the window in the packed arrays
             const int k0 = (int)(w.m0 & 0xffffu);
@@ -777,8 +797,8 @@ __device__ __forceinline__ void bj_win_update(const BjWin<T> &w, T *v, const uin
 template <typename T>
 constexpr int bj_ahead() { return sizeof(T) == 4 ? BJ_AHEAD_F32 : BJ_AHEAD_F64; }
 
-// One triangle of one block: thread t takes the t-th slot of each level (the
-// solve's levels hold at most 64 slots).  The triangle's meta words are first
+// One triangle of one block: lane pair t/2 takes the (t/2)-th slot of each
+// level (the solve's levels hold at most 32 slots).  The triangle's meta words are first
 // staged in LDS (ms), so a level's window addresses come from LDS and only the
 // windows themselves are loads in flight: kA levels ahead, one window per lane
 // and level, each consumed by the same unrolled stage that issued it kA
@@ -804,31 +824,32 @@ __device__ __forceinline__ void bj_sweep(T *v, int nl, const int *lp, const uint
             if (k0 + 64 * i < ns) ms[k0 + 64 * i] = q[i];
     }
     wave_lds_order();
+    const int q = t >> 1, half = t & 1;
     auto fetch_meta = [&](int l, BjMeta &m) {
         const int lc = min(l, nl - 1);
         const int p0 = lp[lc], p1 = lp[lc + 1];
         const int cnt = p1 - p0;
-        m.s = p0 + min(t, cnt - 1);
-        const uint2 q = ms[m.s - s0];
-        m.m0 = q.x;
-        m.m1 = q.y;
-        m.have = l < nl && t < cnt;
+        m.s = p0 + min(q, cnt - 1);
+        const uint2 mq = ms[m.s - s0];
+        m.m0 = mq.x;
+        m.m1 = mq.y;
+        m.have = l < nl && q < cnt;
     };
     BjWin<T> wr[kA];
 #pragma unroll
     for (int d = 0; d < kA; ++d) {
         BjMeta m;
         fetch_meta(d, m);
-        bj_win_load<T, UPPER>(wr[d], m, val, col, dinv);
+        bj_win_load<T, UPPER>(wr[d], m, half, val, col, dinv);
         wave_lds_order();
     }
     for (int lb = 0; lb < nl; lb += kA) {
 #pragma unroll
         for (int d = 0; d < kA; ++d) {
-            if (wr[d].have) bj_win_update<T, UPPER, OVG>(wr[d], v, ovp, val, col);
+            bj_win_update<T, UPPER, OVG>(wr[d], half, v, ovp, val, col);
             BjMeta m;
             fetch_meta(lb + d + kA, m);
-            bj_win_load<T, UPPER>(wr[d], m, val, col, dinv);  // level lb + d + kA
+            bj_win_load<T, UPPER>(wr[d], m, half, val, col, dinv);  // level lb + d + kA
             // one wavefront: the LDS executes its accesses in order, so only
             // the compiler must keep them in program order
             wave_lds_order();
@@ -851,7 +872,7 @@ __host__ __device__ inline size_t bj_solve_lds(size_t tsize, int maxrows, int ov
 // block, so that every block of the matrix is resident at once.  (Staging the
 // whole block in LDS was measured slower: it limits the resident blocks, and
 // one wavefront stages too slowly.)  Per triangle: solve levels blev (per
-// block) into lptr (per level, into the level-ordered slots, at most 64 per
+// block) into lptr (per level, into the level-ordered slots, at most 32 per
 // level), per slot its meta word pair, the packed values and local columns
 // from the block's entry base ebase, U's inverted diagonals; the block's
 // overflow entries are the packed entries at ov_pos[ov_ptr[b] ..).
@@ -1775,7 +1796,7 @@ static int build_bj(fvp_csr *A) {
             tr[t].ebase[nb] = (long)tr[t].esrc.size();
             for (int k = 0; k < kBjRow; ++k) { tr[t].esrc.push_back(-1); tr[t].col.push_back(0); }
         }
-        // the solve's levels: each factor level cut into pieces of at most 64 slots
+        // the solve's levels: each factor level cut into pieces of at most 32 slots (one per lane pair)
         auto split = [&](const std::vector<int> &blev, const std::vector<int> &lptr, std::vector<int> &sb,
                          std::vector<int> &sp, int &levmax) {
             sb.assign(nb + 1, 0);
@@ -1783,7 +1804,7 @@ static int build_bj(fvp_csr *A) {
             levmax = 0;
             for (int b = 0; b < nb; ++b) {
                 for (int l = blev[b]; l < blev[b + 1]; ++l)
-                    for (int p = lptr[l]; p < lptr[l + 1]; p += 64) sp.push_back(std::min(p + 64, lptr[l + 1]));
+                    for (int p = lptr[l]; p < lptr[l + 1]; p += 32) sp.push_back(std::min(p + 32, lptr[l + 1]));
                 sb[b + 1] = (int)sp.size() - 1;
                 levmax = std::max(levmax, sb[b + 1] - sb[b]);
             }
