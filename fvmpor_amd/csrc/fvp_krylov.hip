@@ -797,33 +797,46 @@ __device__ __forceinline__ void bj_win_update(const BjWin<T> &w, int half, T *v,
 template <typename T>
 constexpr int bj_ahead() { return sizeof(T) == 4 ? BJ_AHEAD_F32 : BJ_AHEAD_F64; }
 
+// Staging a triangle's meta words (its ns slots from meta) in LDS: one round
+// is kBjMeta loads per lane, all in flight together, so a block of up to
+// 64 * kBjMeta slots per triangle waits out one memory latency.
+constexpr int kBjMeta = 32;
+__device__ __forceinline__ void bj_meta_load(uint2 (&q)[kBjMeta], const uint2 *__restrict__ meta, int ns, int k0) {
+#pragma unroll
+    for (int i = 0; i < kBjMeta; ++i) q[i] = meta[min(k0 + 64 * i, ns - 1)];
+}
+__device__ __forceinline__ void bj_meta_store(const uint2 (&q)[kBjMeta], uint2 *ms, int ns, int k0) {
+#pragma unroll
+    for (int i = 0; i < kBjMeta; ++i)
+        if (k0 + 64 * i < ns) ms[k0 + 64 * i] = q[i];
+}
+// the slots from k0 (this lane's first) on, round by round
+__device__ __forceinline__ void bj_meta_stage(const uint2 *__restrict__ meta, uint2 *ms, int ns, int k0) {
+    for (; k0 < ns; k0 += 64 * kBjMeta) {
+        uint2 q[kBjMeta];
+        bj_meta_load(q, meta, ns, k0);
+        bj_meta_store(q, ms, ns, k0);
+    }
+}
+
 // One triangle of one block: lane pair t/2 takes the (t/2)-th slot of each
-// level (the solve's levels hold at most 32 slots).  The triangle's meta words are first
-// staged in LDS (ms), so a level's window addresses come from LDS and only the
-// windows themselves are loads in flight: kA levels ahead, one window per lane
-// and level, each consumed by the same unrolled stage that issued it kA
-// stages earlier (no loop-carried register holds a pending load, so the
-// compiler's wait counts keep all kA levels in flight).  No global load with
-// a data-dependent count is issued inside the level loop, and every pass runs
-// all kA levels (those past the last have no lane with work).
+// level (the solve's levels hold at most 32 slots).  The triangle's meta words
+// are staged in LDS (ms, slot lp[0] first) before the sweep, so a level's
+// window addresses come from LDS and only the windows themselves are loads in
+// flight: kA levels ahead, one window per lane and level, each consumed by the
+// same unrolled stage that issued it kA stages earlier (no loop-carried
+// register holds a pending load, so the compiler's wait counts keep all kA
+// levels in flight).  No global load with a data-dependent count is issued
+// inside the level loop, and every pass runs all kA levels (those past the
+// last have no lane with work).
 template <typename T, bool UPPER, bool OVG>
-__device__ __forceinline__ void bj_sweep(T *v, int nl, const int *lp, const uint2 *__restrict__ meta, uint2 *ms,
-                                         const T *__restrict__ val, const unsigned short *__restrict__ col,
-                                         const T *__restrict__ dinv, const uint4 *ovp) {
+__device__ __forceinline__ void bj_sweep(T *v, int nl, const int *lp, const uint2 *ms, const T *__restrict__ val,
+                                         const unsigned short *__restrict__ col, const T *__restrict__ dinv,
+                                         const uint4 *ovp) {
     const int t = threadIdx.x;
     if (nl <= 0) return;
     constexpr int kA = bj_ahead<T>();
-    const int s0 = lp[0], ns = lp[nl] - s0;
-    // stage the meta words of the triangle's slots, 8 loads per lane in flight
-    for (int k0 = t; k0 < ns; k0 += 64 * 8) {
-        uint2 q[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) q[i] = meta[s0 + min(k0 + 64 * i, ns - 1)];
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-            if (k0 + 64 * i < ns) ms[k0 + 64 * i] = q[i];
-    }
-    wave_lds_order();
+    const int s0 = lp[0];
     const int q = t >> 1, half = t & 1;
     auto fetch_meta = [&](int l, BjMeta &m) {
         const int lc = min(l, nl - 1);
@@ -898,9 +911,16 @@ __global__ void __launch_bounds__(64) bj_solve(const int *__restrict__ roff, con
     char *lds = reinterpret_cast<char *>(smem);
     T *v = reinterpret_cast<T *>(lds);
     uint4 *ovp = reinterpret_cast<uint4 *>(lds + ((sizeof(T) * (size_t)maxrows + 15) & ~(size_t)15));
+    const int l0 = blev_l[b], nl = blev_l[b + 1] - l0;
+    const int u0 = blev_u[b], nu = blev_u[b + 1] - u0;
+    // L's meta words: the first round's loads are issued here, ahead of
+    // staging x, and stored to the LDS after it
+    const int sl0 = nl > 0 ? lptr_l[l0] : 0, nsl = nl > 0 ? lptr_l[l0 + nl] - sl0 : 0;
+    uint2 mq[kBjMeta];
+    if (nsl > 0) bj_meta_load(mq, meta_l + sl0, nsl, threadIdx.x);
     // stage x (optionally scaled by 1 / xdiv) in batches of kBjStage rows per
     // lane, so that a batch's gathers are in flight together
-    constexpr int kBjStage = 8;
+    constexpr int kBjStage = 16;
     for (int r0 = threadIdx.x; r0 < cnt; r0 += 64 * kBjStage) {
         int g[kBjStage];
         double xv[kBjStage], dv[kBjStage];
@@ -937,16 +957,20 @@ __global__ void __launch_bounds__(64) bj_solve(const int *__restrict__ roff, con
         ovp[k] = make_uint4(lo, hi, (unsigned)(up ? col_u[e] : col_l[e]), 0u);
     }
     int *lpl = reinterpret_cast<int *>(ovp + ovmax);
-    const int l0 = blev_l[b], nl = blev_l[b + 1] - l0;
-    const int u0 = blev_u[b], nu = blev_u[b + 1] - u0;
     int *lpu = lpl + nl + 1;
     uint2 *ms = reinterpret_cast<uint2 *>(reinterpret_cast<char *>(lpl) +
                                           ((4 * (size_t)(levmax_l + levmax_u + 2) + 7) & ~(size_t)7));
     for (int k = threadIdx.x; k <= nl; k += 64) lpl[k] = lptr_l[l0 + k];
     for (int k = threadIdx.x; k <= nu; k += 64) lpu[k] = lptr_u[u0 + k];
+    if (nsl > 0) bj_meta_store(mq, ms, nsl, threadIdx.x);
+    bj_meta_stage(meta_l + sl0, ms, nsl, threadIdx.x + 64 * kBjMeta);
     wave_lds_order();
-    bj_sweep<T, false, OVG>(v, nl, lpl, meta_l, ms, val_l + ebase_l[b], col_l + ebase_l[b], nullptr, ovp);
-    bj_sweep<T, true, OVG>(v, nu, lpu, meta_u, ms, val_u + ebase_u[b], col_u + ebase_u[b], dinv, ovp);
+    bj_sweep<T, false, OVG>(v, nl, lpl, ms, val_l + ebase_l[b], col_l + ebase_l[b], nullptr, ovp);
+    // U's meta words replace L's (the sweep's LDS accesses execute in order;
+    // loading them during L's sweep instead was measured no faster)
+    if (nu > 0) bj_meta_stage(meta_u + lpu[0], ms, lpu[nu] - lpu[0], threadIdx.x);
+    wave_lds_order();
+    bj_sweep<T, true, OVG>(v, nu, lpu, ms, val_u + ebase_u[b], col_u + ebase_u[b], dinv, ovp);
     const int olo = own[2 * b], ohi = own[2 * b + 1];
     double q = 0.;
     for (int r0 = olo + threadIdx.x; r0 < ohi; r0 += 64 * kBjStage) {
