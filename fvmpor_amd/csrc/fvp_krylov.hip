@@ -749,15 +749,15 @@ __device__ __forceinline__ void bj_win_load(BjWin<T> &w, const BjMeta &m, int ha
     w.have = m.have;
 }
 
-// Each lane of the pair sums its half window; the even lane then finishes
-// the row: s = v[r] - (both halves) - (entries past kBjRow, in column order),
-// times the inverted diagonal for U.  Every lane takes part in the pair sum
-// (lanes without a slot read a valid clamped slot and store nothing).
-// OVG: the overflow entries are read from the packed arrays in memory (blocks
-// whose overflow lists do not fit the LDS) instead of from LDS
-template <typename T, bool UPPER, bool OVG>
-__device__ __forceinline__ void bj_win_update(const BjWin<T> &w, int half, T *v, const uint4 *ovp,
-                                              const T *__restrict__ val, const unsigned short *__restrict__ col) {
+// Each lane of the pair sums its half window (bj_win_partial); the even lane
+// then finishes the row (bj_win_finish): s = v[r] - (both halves) - (entries
+// past kBjRow, in column order), times the inverted diagonal for U.  Every
+// lane takes part in the pair sum (lanes without a slot read a valid clamped
+// slot and store nothing).
+// The half window's partial sum; vr: v[r], read with the window's columns
+// (rows of one level do not depend on each other, so v[r] is final here).
+template <typename T>
+__device__ __forceinline__ T bj_win_partial(const BjWin<T> &w, int half, const T *v, T &vr) {
     const int r = (int)(w.m1 & 0xffffu), kn = (int)(w.m1 >> 16);
     T p = T(0);
 #pragma unroll
@@ -770,10 +770,22 @@ __device__ __forceinline__ void bj_win_update(const BjWin<T> &w, int half, T *v,
         const T a = ug < kn ? w.a[u] : T(0);
         p += a * v[c];
     }
-    p = pair_sum(p);
-    if (!w.have || half) return;
-    T s = v[r] - p;
-    if (kn > kBjRow) {
+    vr = v[r];
+    return p;
+}
+
+// OVG: the overflow entries are read from the packed arrays in memory (blocks
+// whose overflow lists do not fit the LDS) instead of from LDS
+template <typename T, bool UPPER, bool OVG>
+__device__ __forceinline__ void bj_win_finish(const BjWin<T> &w, int half, T p, T vr, T *v, T *vd,
+                                              const uint4 *ovp, const T *__restrict__ val,
+                                              const unsigned short *__restrict__ col) {
+    const int r = (int)(w.m1 & 0xffffu), kn = (int)(w.m1 >> 16);
+    // every lane forms and stores s (lanes without a row store to their own
+    // spare slot vd), so v[r]'s read stays with the window's reads
+    const bool mine = w.have && !half;
+    T s = vr - pair_sum(p);
+    if (mine && kn > kBjRow) {
         if (OVG) {  // overflow entries straight after the window in the packed arrays
             const int k0 = (int)(w.m0 & 0xffffu);
             for (int k = k0 + kBjRow; k < k0 + kn; ++k) s -= val[k] * v[col[k]];
@@ -789,7 +801,7 @@ __device__ __forceinline__ void bj_win_update(const BjWin<T> &w, int half, T *v,
         }
     }
     if (UPPER) s *= w.d;  // inverted diagonal
-    v[r] = s;
+    *(mine ? v + r : vd) = s;
 }
 
 // levels loaded ahead: deep enough to cover a memory round trip with the short
@@ -830,18 +842,21 @@ __device__ __forceinline__ void bj_meta_stage(const uint2 *__restrict__ meta, ui
 // inside the level loop, and every pass runs all kA levels (those past the
 // last have no lane with work).
 template <typename T, bool UPPER, bool OVG>
-__device__ __forceinline__ void bj_sweep(T *v, int nl, const int *lp, const uint2 *ms, const T *__restrict__ val,
-                                         const unsigned short *__restrict__ col, const T *__restrict__ dinv,
-                                         const uint4 *ovp) {
+__device__ __forceinline__ void bj_sweep(T *v, T *vd, int nl, const int *lp, const uint2 *ms,
+                                         const T *__restrict__ val, const unsigned short *__restrict__ col,
+                                         const T *__restrict__ dinv, const uint4 *ovp) {
     const int t = threadIdx.x;
     if (nl <= 0) return;
     constexpr int kA = bj_ahead<T>();
     const int s0 = lp[0];
     const int q = t >> 1, half = t & 1;
-    auto fetch_meta = [&](int l, BjMeta &m) {
+    // a level's slot range, then this lane's slot and its meta words
+    auto level_range = [&](int l, int &p0, int &cnt) {
         const int lc = min(l, nl - 1);
-        const int p0 = lp[lc], p1 = lp[lc + 1];
-        const int cnt = p1 - p0;
+        p0 = lp[lc];
+        cnt = lp[lc + 1] - p0;
+    };
+    auto slot_meta = [&](int l, int p0, int cnt, BjMeta &m) {
         m.s = p0 + min(q, cnt - 1);
         const uint2 mq = ms[m.s - s0];
         m.m0 = mq.x;
@@ -851,18 +866,28 @@ __device__ __forceinline__ void bj_sweep(T *v, int nl, const int *lp, const uint
     BjWin<T> wr[kA];
 #pragma unroll
     for (int d = 0; d < kA; ++d) {
+        int p0, cnt;
+        level_range(d, p0, cnt);
         BjMeta m;
-        fetch_meta(d, m);
+        slot_meta(d, p0, cnt, m);
         bj_win_load<T, UPPER>(wr[d], m, half, val, col, dinv);
         wave_lds_order();
     }
+    // Per stage three LDS round trips overlap: level lb + d + kA's range is
+    // read with the update's values, its meta words while the update's sums
+    // are formed; the window of that level is issued last.
     for (int lb = 0; lb < nl; lb += kA) {
 #pragma unroll
         for (int d = 0; d < kA; ++d) {
-            bj_win_update<T, UPPER, OVG>(wr[d], half, v, ovp, val, col);
+            const int ln = lb + d + kA;
+            int p0, cnt;
+            level_range(ln, p0, cnt);
+            T vr;
+            const T p = bj_win_partial<T>(wr[d], half, v, vr);
             BjMeta m;
-            fetch_meta(lb + d + kA, m);
-            bj_win_load<T, UPPER>(wr[d], m, half, val, col, dinv);  // level lb + d + kA
+            slot_meta(ln, p0, cnt, m);
+            bj_win_finish<T, UPPER, OVG>(wr[d], half, p, vr, v, vd, ovp, val, col);
+            bj_win_load<T, UPPER>(wr[d], m, half, val, col, dinv);  // level ln
             // one wavefront: the LDS executes its accesses in order, so only
             // the compiler must keep them in program order
             wave_lds_order();
@@ -870,12 +895,13 @@ __device__ __forceinline__ void bj_sweep(T *v, int nl, const int *lp, const uint
     }
 }
 
-// LDS bytes of one block's solve: the solution (T x maxrows), the block's
+// LDS bytes of one block's solve: the solution (T x maxrows) and a spare
+// slot per lane (T x 64), the block's
 // overflow entries (16 bytes each; none with OVG), its L and U level pointers,
 // the meta words of one triangle's slots
 __host__ __device__ inline size_t bj_solve_lds(size_t tsize, int maxrows, int ovmax, int levl, int levu,
                                                 int maxslots) {
-    return ((tsize * (size_t)maxrows + 15) & ~(size_t)15) + 16 * (size_t)ovmax +
+    return ((tsize * (size_t)(maxrows + 64) + 15) & ~(size_t)15) + 16 * (size_t)ovmax +
            ((4 * (size_t)(levl + levu + 2) + 7) & ~(size_t)7) + 8 * (size_t)maxslots;
 }
 
@@ -910,7 +936,9 @@ __global__ void __launch_bounds__(64) bj_solve(const int *__restrict__ roff, con
     const int cnt = roff[b + 1] - base;
     char *lds = reinterpret_cast<char *>(smem);
     T *v = reinterpret_cast<T *>(lds);
-    uint4 *ovp = reinterpret_cast<uint4 *>(lds + ((sizeof(T) * (size_t)maxrows + 15) & ~(size_t)15));
+    // v: the block's rows, then one spare slot per lane
+    T *vd = v + maxrows + threadIdx.x;
+    uint4 *ovp = reinterpret_cast<uint4 *>(lds + ((sizeof(T) * (size_t)(maxrows + 64) + 15) & ~(size_t)15));
     const int l0 = blev_l[b], nl = blev_l[b + 1] - l0;
     const int u0 = blev_u[b], nu = blev_u[b + 1] - u0;
     // L's meta words: the first round's loads are issued here, ahead of
@@ -965,12 +993,12 @@ __global__ void __launch_bounds__(64) bj_solve(const int *__restrict__ roff, con
     if (nsl > 0) bj_meta_store(mq, ms, nsl, threadIdx.x);
     bj_meta_stage(meta_l + sl0, ms, nsl, threadIdx.x + 64 * kBjMeta);
     wave_lds_order();
-    bj_sweep<T, false, OVG>(v, nl, lpl, ms, val_l + ebase_l[b], col_l + ebase_l[b], nullptr, ovp);
+    bj_sweep<T, false, OVG>(v, vd, nl, lpl, ms, val_l + ebase_l[b], col_l + ebase_l[b], nullptr, ovp);
     // U's meta words replace L's (the sweep's LDS accesses execute in order;
     // loading them during L's sweep instead was measured no faster)
     if (nu > 0) bj_meta_stage(meta_u + lpu[0], ms, lpu[nu] - lpu[0], threadIdx.x);
     wave_lds_order();
-    bj_sweep<T, true, OVG>(v, nu, lpu, ms, val_u + ebase_u[b], col_u + ebase_u[b], dinv, ovp);
+    bj_sweep<T, true, OVG>(v, vd, nu, lpu, ms, val_u + ebase_u[b], col_u + ebase_u[b], dinv, ovp);
     const int olo = own[2 * b], ohi = own[2 * b + 1];
     double q = 0.;
     for (int r0 = olo + threadIdx.x; r0 < ohi; r0 += 64 * kBjStage) {
