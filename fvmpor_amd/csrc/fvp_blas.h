@@ -8,6 +8,33 @@
 #include "../../include/fvp.h"
 
 namespace fvp {
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// block sum of v (thread 0 holds the result; lds: one double per wavefront)
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double r = 0.;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += lds[w];
+    __syncthreads();
+    return r;
+}
+
+// FD increment of J v: sigma = sqrt(eps) ||psi|| / ||v|| from the squared norms
+__host__ __device__ inline double jv_sigma_of(double psi_n2, double v_n2) {
+    const double sq_eps = 1.4901161193847656e-08;  // sqrt(2^-52)
+    double np_ = sqrt(psi_n2), nv = sqrt(v_n2);
+    if (np_ == 0.) np_ = 1.;
+    return nv > 0. ? sq_eps * np_ / nv : 1.;
+}
+
 // out[0] (device) = x . y; part must hold 512 doubles.  Asynchronous on s.
 int dot_async(hipStream_t s, long n, const double *x, const double *y, double *part, double *out);
 

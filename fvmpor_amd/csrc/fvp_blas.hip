@@ -13,25 +13,6 @@ namespace fvp {
 constexpr int kRedBlocks = 512;   // partial sums per reduction (>= 2 per CU)
 constexpr int kRedThreads = 256;
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    return v;
-}
-
-// block sum of v into out (thread 0 holds the result); 256 threads = 4 wavefronts
-__device__ __forceinline__ double block_sum(double v, double *lds) {
-    v = wave_sum(v);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) lds[wid] = v;
-    __syncthreads();
-    double r = 0.;
-    if (threadIdx.x == 0)
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += lds[w];
-    __syncthreads();
-    return r;
-}
-
 // partials[b] = sum over this block's grid-stride slice of x*y (x == y gives ||x||^2)
 __global__ void __launch_bounds__(kRedThreads) dot_partial(long n, const double *__restrict__ x,
                                                           const double *__restrict__ y, double *__restrict__ part) {
@@ -61,12 +42,7 @@ __global__ void __launch_bounds__(kRedThreads) jv_sigma(int np, const double *__
     for (int i = threadIdx.x; i < np; i += blockDim.x) { a += part_psi[i]; b += part_v[i]; }
     a = block_sum(a, lds);
     b = block_sum(b, lds);
-    if (threadIdx.x == 0) {
-        const double sq_eps = 1.4901161193847656e-08;  // sqrt(2^-52)
-        double np_ = sqrt(a), nv = sqrt(b);
-        if (np_ == 0.) np_ = 1.;
-        sig[0] = nv > 0. ? sq_eps * np_ / nv : 1.;
-    }
+    if (threadIdx.x == 0) sig[0] = jv_sigma_of(a, b);
 }
 
 // the same from ||psi||^2 already reduced (psi_n2[0]) and the partials of ||v||^2
@@ -76,12 +52,7 @@ __global__ void __launch_bounds__(kRedThreads) jv_sigma_n2(int np, const double 
     double b = 0.;
     for (int i = threadIdx.x; i < np; i += blockDim.x) b += part_v[i];
     b = block_sum(b, lds);
-    if (threadIdx.x == 0) {
-        const double sq_eps = 1.4901161193847656e-08;  // sqrt(2^-52)
-        double np_ = sqrt(psi_n2[0]), nv = sqrt(b);
-        if (np_ == 0.) np_ = 1.;
-        sig[0] = nv > 0. ? sq_eps * np_ / nv : 1.;
-    }
+    if (threadIdx.x == 0) sig[0] = jv_sigma_of(psi_n2[0], b);
 }
 
 __global__ void jv_perturb(long n, const double *__restrict__ sig, double cj, const double *__restrict__ psi,
@@ -122,12 +93,15 @@ int jv_impl(fvp_ctx *c, double t, const double *psi, const double *psip, const d
     hipStream_t s = c->stream;
     const long N = c->N;
     double *part = c->red.p;  // [0, 512) psi partials, [512, 1024) v partials, [1024] sigma
-    if (psi_n2 && v_n2)
+    // partials of ||v||^2 from the solve that wrote v: the fused residual
+    // kernels' van Genuchten pass forms sigma itself
+    const bool fold = psi_n2 && v_n2 && !c->averaging;
+    if (!fold && psi_n2 && v_n2) {
         jv_sigma_n2<<<1, kRedThreads, 0, s>>>(v_count, psi_n2, v_n2, part + 2 * kRedBlocks);
-    else if (psi_n2) {
+    } else if (!fold && psi_n2) {
         dot_partial<<<kRedBlocks, kRedThreads, 0, s>>>(N, v, v, part + kRedBlocks);
         jv_sigma_n2<<<1, kRedThreads, 0, s>>>(kRedBlocks, psi_n2, part + kRedBlocks, part + 2 * kRedBlocks);
-    } else {
+    } else if (!fold) {
         dot_partial<<<kRedBlocks, kRedThreads, 0, s>>>(N, v, v, part + kRedBlocks);
         dot_partial<<<kRedBlocks, kRedThreads, 0, s>>>(N, psi, psi, part);
         jv_sigma<<<1, kRedThreads, 0, s>>>(kRedBlocks, part, part + kRedBlocks, part + 2 * kRedBlocks);
@@ -136,6 +110,9 @@ int jv_impl(fvp_ctx *c, double t, const double *psi, const double *psip, const d
     if (!c->averaging) {  // perturbation and difference quotient fused into the residual kernels
         JvArgs jv;
         jv.v = v; jv.sig = part + 2 * kRedBlocks; jv.F0 = F0; jv.cj = c_j; jv.osc = osc;
+        if (fold) {
+            jv.psi_n2 = psi_n2; jv.part_v = v_n2; jv.np = v_count; jv.sig_out = part + 2 * kRedBlocks;
+        }
         return residual_impl(c, t, psi, psip, Jv, 0, &jv);
     }
     jv_perturb<<<nblk(N, 256), 256, 0, s>>>(N, part + 2 * kRedBlocks, c_j, psi, psip, v, c->tmp1.p, c->tmp2.p);

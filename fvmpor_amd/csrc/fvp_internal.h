@@ -128,6 +128,12 @@ struct JvArgs {
     const double *v = nullptr, *sig = nullptr, *F0 = nullptr;
     const double *osc = nullptr;  // optional: J v scaled row-wise by osc
     double cj = 0.;
+    // optional (np > 0): the van Genuchten pass forms sigma itself from
+    // ||psi||^2 (psi_n2) and np partials of ||v||^2 and writes it to sig_out
+    // (== sig) for the kernels after it
+    const double *psi_n2 = nullptr, *part_v = nullptr;
+    double *sig_out = nullptr;
+    int np = 0;
 };
 // J v with options for the integrator's scaled solve: psi_n2 (device scalar,
 // or null) is ||psi||^2 computed once for the Newton iterate; v_n2 (or null)

@@ -466,18 +466,39 @@ __host__ __device__ inline void drotg_dev(double *da, double *db, double *c, dou
 // applies the previous rotations and the new Givens rotation, rotates the
 // right-hand side g, and writes {|g_{j+1}|, h_{j+1,j}} to stat and, when hstat
 // is given (pinned host memory the device can write), there too.
+// With npart, ||w||^2 is not reduced beforehand: every block sums the nb
+// partials of the single Gram-Schmidt pass itself, in reduce_cols' order (the
+// same value bit for bit, one launch fewer), and the slices are grid-strided.
+// Workgroups of arnoldi_finish (grid-strided): few enough that every one
+// summing the norm's partials costs little.
+constexpr int kFinishBlocks = 512;
 __global__ void __launch_bounds__(256) arnoldi_finish(int j, int m, const double *__restrict__ h1,
                                                       const double *__restrict__ h2, const double *__restrict__ n1,
                                                       const double *__restrict__ n2, const double *__restrict__ flag,
                                                       double *__restrict__ H, double *__restrict__ cs,
                                                       double *__restrict__ sn, double *__restrict__ g,
                                                       double *__restrict__ stat, double *__restrict__ hstat, long n,
-                                                      const double *__restrict__ w, double *__restrict__ q) {
-    const bool two = flag && flag[0] != 0.;
-    const double hn = sqrt(two ? n2[0] : n1[0]);
+                                                      const double *__restrict__ w, double *__restrict__ q,
+                                                      const double *__restrict__ npart = nullptr, int nb = 0) {
+    __shared__ double lds[4];
+    const bool two = !npart && flag && flag[0] != 0.;
+    double nn;
+    if (npart) {
+        double s = 0.;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) s += npart[b];
+        s = wave_reduce(s);
+        if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+        __syncthreads();
+        double v = 0.;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) v += lds[k];
+        nn = v;
+    } else {
+        nn = two ? n2[0] : n1[0];
+    }
+    const double hn = sqrt(nn);
     const double ninv = hn > 0. ? 1. / hn : 0.;
-    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i < n) q[i] = w[i] * ninv;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        q[i] = w[i] * ninv;
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     double *hc = H + (size_t)j * (m + 1);
     for (int k = 0; k <= j; ++k) hc[k] = two ? h1[k] + h2[k] : h1[k];
@@ -1232,6 +1253,7 @@ static void multidot(hipStream_t s, long n, const double *w, const double *Q, lo
 }
 
 // w -= Q[:, 0..nc) coef in chunks of kChunk columns; *norm2 = the updated w . w
+// (norm2 null: left as the gs_blocks(n) partials in part)
 static void multiaxpy(hipStream_t s, long n, double *w, const double *Q, long ldq, int nc, const double *coef,
                       double *part, double *norm2, const double *skip) {
     const int nb = gs_blocks(n);
@@ -1239,7 +1261,7 @@ static void multiaxpy(hipStream_t s, long n, double *w, const double *Q, long ld
         const int k = std::min(kChunk, nc - c0);
         kAxpy[k - 1](s, nb, n, w, Q + c0 * ldq, ldq, coef + c0, part, skip);
     }
-    reduce_cols<<<1, kThreads, 0, s>>>(nb, part, norm2, 0, skip);
+    if (norm2) reduce_cols<<<1, kThreads, 0, s>>>(nb, part, norm2, 0, skip);
 }
 
 using Op = std::function<int(const double *, double *)>;
@@ -1357,7 +1379,8 @@ static int gmres_core(const GmresWork &W, hipStream_t s, long n, const Op &A, co
                                                            mode == 2 ? nullptr : h1 + j + 1, flag);
                     multiaxpy(s, n, w, Q, ldq, j + 1, h2, part, n2, flag);
                 } else {
-                    multiaxpy(s, n, w, Q, ldq, j + 1, h1, part, n1, nullptr);
+                    // single pass: arnoldi_finish reduces ||w||^2 itself
+                    multiaxpy(s, n, w, Q, ldq, j + 1, h1, part, mode ? n1 : nullptr, nullptr);
                     if (mode) {
                         reorth_decide<<<1, 64, 0, s>>>(h1 + j + 1, n1, mode, flag);
                         multidot(s, n, w, Q, ldq, j + 1, part, h2, flag);
@@ -1365,10 +1388,10 @@ static int gmres_core(const GmresWork &W, hipStream_t s, long n, const Op &A, co
                     }
                 }
                 const bool fused = mode && j + 1 <= kChunk;  // ||w||^2 after the first pass in h2[j + 1]
-                arnoldi_finish<<<nblk(n, 256), 256, 0, s>>>(j, m, h1, h2, fused ? h2 + j + 1 : n1, n2,
-                                                            mode ? flag : nullptr, W.Hd, W.csd, W.snd, W.gd, stat,
-                                                            W.hmapped ? W.hstat + 2 * j : nullptr, n, w,
-                                                            Q + (size_t)(j + 1) * ldq);
+                arnoldi_finish<<<std::min(nblk(n, 256), kFinishBlocks), 256, 0, s>>>(
+                    j, m, h1, h2, fused ? h2 + j + 1 : n1, n2, mode ? flag : nullptr, W.Hd, W.csd, W.snd, W.gd, stat,
+                    W.hmapped ? W.hstat + 2 * j : nullptr, n, w, Q + (size_t)(j + 1) * ldq, mode ? nullptr : part,
+                    gs_blocks(n));
                 if (!W.hmapped)
                     FVP_HIP(hipMemcpyAsync(W.hstat + 2 * j, stat, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
                 if (W.ev) FVP_HIP(hipEventRecord(W.ev[j], s));

@@ -24,6 +24,7 @@
 
 #include <cmath>
 
+#include "fvp_blas.h"
 #include "fvp_internal.h"
 
 namespace fvp {
@@ -61,19 +62,36 @@ __device__ __forceinline__ void vg_eval(double h, const double *zp, double &Sw, 
 // One node per thread; the node's zones (its mask bits) are visited in a
 // per-lane loop with the zone parameters in LDS, so a wavefront pays for the
 // largest zone count of its nodes rather than for every zone any of them has.
+// JV with jv.np > 0: every workgroup reduces the np partials of ||v||^2 in the
+// fixed order of jv_sigma_n2 (the same sigma bit for bit) and workgroup 0
+// stores sigma for the finalisation, so no launch of its own is needed.
 template <bool JV>
 __global__ void __launch_bounds__(256) vg_kernel(int N, ZoneTable zt, double rho0, const double *__restrict__ psi,
                                                  const uint8_t *__restrict__ zmask, const double *__restrict__ zw,
-                                                 double *__restrict__ kr, double *__restrict__ cap,
-                                                 const double *__restrict__ v, const double *__restrict__ sig,
+                                                 double *__restrict__ kr, double *__restrict__ cap, JvArgs jv,
                                                  double *__restrict__ nrec) {
     __shared__ double zp[kMaxZones * 10];
+    __shared__ double red[4];
     if (threadIdx.x < 10 * zt.nz) zp[threadIdx.x] = zt.p[threadIdx.x / 10][threadIdx.x % 10];
-    __syncthreads();
+    double sg = 0.;
+    if (JV && jv.np > 0) {
+        double b = 0.;
+        for (int k = threadIdx.x; k < jv.np; k += blockDim.x) b += jv.part_v[k];
+        b = block_sum(b, red);  // ends with a barrier, so the zone table is in place
+        if (threadIdx.x == 0) {
+            red[0] = jv_sigma_of(jv.psi_n2[0], b);
+            if (blockIdx.x == 0) jv.sig_out[0] = red[0];
+        }
+        __syncthreads();
+        sg = red[0];
+    } else {
+        __syncthreads();
+        if (JV) sg = jv.sig[0];
+    }
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     double h = psi[i];
-    if (JV) h = psi[i] + sig[0] * v[i];
+    if (JV) h = psi[i] + sg * jv.v[i];
     // the head of the node record the element kernel gathers (x, y, z are
     // written once at set-up)
     if (nrec) nrec[4 * (size_t)i + 3] = h;
@@ -960,11 +978,11 @@ int fd_jacobian_elem(fvp_ctx *c, double cj, const double *psi, const double *psi
     const int N = c->N, T = c->T;
     // base state: k_r, capacity and the node records' heads
     vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p, c->cap.p,
-                                                     nullptr, nullptr, c->nrec.p);
+                                                     JvArgs(), c->nrec.p);
     // every node moved by its shift (each node is moved once, in its own column)
     fd_add<<<blocks(N, 256), 256, 0, s>>>(N, psi, shift, ys);
-    vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, ys, c->zmask.p, c->zw.p, kr2, cap2, nullptr,
-                                                     nullptr, nullptr);
+    vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, ys, c->zmask.p, c->zw.p, kr2, cap2, JvArgs(),
+                                                     nullptr);
     if (fp32) {
         float *c32 = reinterpret_cast<float *>(contrib);
         if (c->dim == 3)
@@ -1011,10 +1029,10 @@ int residual_impl(fvp_ctx *c, double t, const double *psi, const double *psip, d
     prof_mark(c, 1, 0);
     if (jv)
         vg_kernel<true><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p,
-                                                        c->cap.p, jv->v, jv->sig, c->nrec.p);
+                                                        c->cap.p, *jv, c->nrec.p);
     else
         vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p,
-                                                         c->cap.p, nullptr, nullptr, c->rvariant == FVP_RESIDUAL_OWNER ? c->nrec.p : nullptr);
+                                                         c->cap.p, JvArgs(), c->rvariant == FVP_RESIDUAL_OWNER ? c->nrec.p : nullptr);
     if (!averaging) {
         // tiled paths (element fluxes with the boundary fluxes, then the finalisation)
         const JvArgs none;
@@ -1130,7 +1148,7 @@ extern "C" int fvp_begin_step(fvp_ctx *c, double t, const double *psi, const dou
     const int N = c->N, T = c->T;
     // k_r at the state, then the averaged-weight face fluxes summed per edge
     vg_kernel<false><<<blocks(N, 256), 256, 0, s>>>(N, c->zt, c->rho0, psi, c->zmask.p, c->zw.p, c->kr.p, c->cap.p,
-                                                     nullptr, nullptr, nullptr);
+                                                     JvArgs(), nullptr);
     if (c->dim == 3)
         elem_kernel<3, 2><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, c->conn.p, c->ezone.p, c->kmul.p,
                                                        c->emask.p, c->eorient.p, c->xyz.p, psi, c->kr.p, nullptr,
