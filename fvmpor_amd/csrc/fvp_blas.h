@@ -36,12 +36,20 @@ __host__ __device__ inline double jv_sigma_of(double psi_n2, double v_n2) {
 }
 
 // out[0] (device) = x . y; part must hold 512 doubles.  Asynchronous on s.
-int dot_async(hipStream_t s, long n, const double *x, const double *y, double *part, double *out);
+// hout (optional): pinned host memory that receives the value as well.
+int dot_async(hipStream_t s, long n, const double *x, const double *y, double *part, double *out,
+              double *hout = nullptr);
+// Waits until the work queued on s so far has finished: records ev behind it
+// and polls the event.  Scalars that kernels stored to pinned host memory are
+// then readable; the poll avoids the wake-up latency of a blocking wait.
+int host_wait(hipStream_t s, hipEvent_t ev);
 
 using Op = std::function<int(const double *, double *)>;
 // Restarted right-preconditioned GMRES on A x = b; W supplies the workspace.
+// x_zero: the caller has set x = 0, so the first residual is b itself (no
+// product with A).
 int gmres_device(fvp_csr *W, hipStream_t s, long n, const Op &A, const Op &M, const double *b, double *x,
-                 const fvp_gmres_params *p, fvp_gmres_stats *st);
+                 const fvp_gmres_params *p, fvp_gmres_stats *st, bool x_zero = false);
 // the context's Jacobian handle (built on first use)
 fvp_csr *jacobian_handle(fvp_ctx *c);
 }  // namespace fvp

@@ -24,14 +24,21 @@ __global__ void __launch_bounds__(kRedThreads) dot_partial(long n, const double 
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// out[0] = sum of np partials (fixed order: deterministic)
+// out[0] = sum of np partials (fixed order: deterministic); hout (optional,
+// pinned host memory) gets it too
 __global__ void __launch_bounds__(kRedThreads) sum_partials(int np, const double *__restrict__ part,
-                                                           double *__restrict__ out) {
+                                                           double *__restrict__ out, double *__restrict__ hout) {
     __shared__ double lds[kRedThreads / 64];
     double s = 0.;
     for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
     s = block_sum(s, lds);
-    if (threadIdx.x == 0) out[0] = s;
+    if (threadIdx.x == 0) {
+        out[0] = s;
+        if (hout) {
+            hout[0] = s;
+            __threadfence_system();
+        }
+    }
 }
 
 // FD increment sigma = sqrt(eps) * ||psi|| / ||v|| from two squared norms
@@ -75,10 +82,19 @@ __global__ void jv_finish(long n, const double *__restrict__ sig, const double *
 
 static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
-int dot_async(hipStream_t s, long n, const double *x, const double *y, double *part, double *out) {
+int dot_async(hipStream_t s, long n, const double *x, const double *y, double *part, double *out, double *hout) {
     dot_partial<<<kRedBlocks, kRedThreads, 0, s>>>(n, x, y, part);
-    sum_partials<<<1, kRedThreads, 0, s>>>(kRedBlocks, part, out);
+    sum_partials<<<1, kRedThreads, 0, s>>>(kRedBlocks, part, out, hout);
     FVP_HIP(hipGetLastError());
+    return 0;
+}
+
+int host_wait(hipStream_t s, hipEvent_t ev) {
+    FVP_HIP(hipEventRecord(ev, s));
+    hipError_t e;
+    while ((e = hipEventQuery(ev)) == hipErrorNotReady) {
+    }
+    FVP_HIP(e);
     return 0;
 }
 

@@ -41,6 +41,8 @@ struct fvp_integrator {
     double th[NHIST];
     int nh = 0;
     DevBuf<double> yp, ewt, y, ypn, ypred, ypred2, G, b, delta, beta, tmp, part, scal;
+    double *hscal = nullptr;   // pinned host: the norms the step logic reads
+    hipEvent_t hev = nullptr;  // marks the kernels that wrote them
     double h = 0., ss = 20., cj_setup = -1.;
     int k = 1, steps_at_k = 0, steps_since_setup = 0;
     bool need_setup = true;
@@ -199,9 +201,10 @@ __global__ void __launch_bounds__(256) seg_sq(const int *__restrict__ seg, const
     if (threadIdx.x == 0) out[g] = lds[0] + lds[1] + lds[2] + lds[3];
 }
 
-// out[0] = max over instances of sqrt(sum / size), out[1] = total sum
+// out[0] = max over instances of sqrt(sum / size), out[1] = total sum; the
+// same to hout (pinned host memory)
 __global__ void seg_fold(int nseg, const int *__restrict__ seg, const double *__restrict__ sums,
-                         double *__restrict__ out) {
+                         double *__restrict__ out, double *__restrict__ hout) {
     if (threadIdx.x != 0) return;
     double mx = 0., tot = 0.;
     for (int g = 0; g < nseg; ++g) {
@@ -210,25 +213,28 @@ __global__ void seg_fold(int nseg, const int *__restrict__ seg, const double *__
     }
     out[0] = mx;
     out[1] = tot;
+    hout[0] = mx;
+    hout[1] = tot;
+    __threadfence_system();
 }
 
 // Weighted RMS norm ||v||_WRMS with the error weights (the maximum over the
 // instances of an ensemble); l2 (optional) gets the plain 2-norm of v * ewt.
+// The norm's last kernel stores it to pinned host memory; a failed wait gives
+// NaN (every test on it fails; the error is recorded).
 static double wrms(fvp_integrator *I, const double *v, double *l2 = nullptr) {
     hipStream_t s = I->c->stream;
-    double r[2] = {0., 0.};
+    const double *r = I->hscal;
     if (I->nseg > 1) {
         seg_sq<<<I->nseg, 256, 0, s>>>(I->seg.p, v, I->ewt.p, I->seg_sum.p);
-        seg_fold<<<1, 64, 0, s>>>(I->nseg, I->seg.p, I->seg_sum.p, I->scal.p);
-        (void)hipMemcpyAsync(r, I->scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
-        (void)hipStreamSynchronize(s);
+        seg_fold<<<1, 64, 0, s>>>(I->nseg, I->seg.p, I->seg_sum.p, I->scal.p, I->hscal);
+        if (host_wait(s, I->hev)) return std::nan("");
         if (l2) *l2 = std::sqrt(r[1]);
         return r[0];
     }
     mul_kernel<<<nb(I->N), 256, 0, s>>>(I->N, v, I->ewt.p, I->tmp.p);
-    dot_async(s, I->N, I->tmp.p, I->tmp.p, I->part.p, I->scal.p);
-    (void)hipMemcpyAsync(r, I->scal.p, sizeof(double), hipMemcpyDeviceToHost, s);
-    (void)hipStreamSynchronize(s);
+    dot_async(s, I->N, I->tmp.p, I->tmp.p, I->part.p, I->scal.p, I->hscal);
+    if (host_wait(s, I->hev)) return std::nan("");
     if (l2) *l2 = std::sqrt(r[0]);
     return std::sqrt(r[0] / I->N);
 }
@@ -430,7 +436,7 @@ static int attempt(fvp_integrator *I, double h, int k, double *err_k, double *er
                 div_kernel<<<nb(N), 256, 0, s>>>(N, in, ew, out);
                 return 0;
             };
-        int rc = fvp::gmres_device(W, s, N, A, M, I->b.p, I->delta.p, &gp, &gs);
+        int rc = fvp::gmres_device(W, s, N, A, M, I->b.p, I->delta.p, &gp, &gs, true);
         I->st.gmres_iters += gs.iterations;
         if (rc < 0) return -1;
         // a linear solve that stopped short of its tolerance still gives a Newton
@@ -582,6 +588,18 @@ extern "C" int fvp_integrator_create(fvp_ctx *c, const fvp_integrator_params *p,
         fvp_integrator_destroy(I);
         return -1;
     }
+    if (hipHostMalloc((void **)&I->hscal, sizeof(double) * 8, hipHostMallocDefault) != hipSuccess) {
+        I->hscal = nullptr;
+        fvp_integrator_destroy(I);
+        set_error("pinned host allocation failed");
+        return -1;
+    }
+    if (hipEventCreateWithFlags(&I->hev, hipEventDisableTiming) != hipSuccess) {
+        I->hev = nullptr;
+        fvp_integrator_destroy(I);
+        set_error("event creation failed");
+        return -1;
+    }
     hipStream_t s = c->stream;
     FVP_HIP(hipMemcpyAsync(I->hist[0].p, psi0, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
     I->nh = 1;
@@ -608,6 +626,8 @@ extern "C" int fvp_integrator_destroy(fvp_integrator *I) {
     I->seg.release(); I->seg_sum.release();
     I->stat.release(); I->feat.release(); I->ylearn.release(); I->y2.release(); I->ypn2.release();
     I->G2.release();
+    if (I->hscal) (void)hipHostFree(I->hscal);
+    if (I->hev) (void)hipEventDestroy(I->hev);
     delete I;
     return 0;
 }

@@ -1225,7 +1225,8 @@ static int ensure_work(fvp_csr *A, int m) {
         return -1;
     if (A->hstat) (void)hipHostFree(A->hstat);
     A->hstat = nullptr;
-    if (hipHostMalloc((void **)&A->hstat, sizeof(double) * 2 * (m + 1), hipHostMallocDefault) != hipSuccess) {
+    // per Arnoldi step {estimate, h_{j+1,j}}, then two scalar slots
+    if (hipHostMalloc((void **)&A->hstat, sizeof(double) * (2 * (m + 1) + 2), hipHostMallocDefault) != hipSuccess) {
         A->hstat = nullptr;
         set_error("pinned host allocation failed");
         return -1;
@@ -1287,6 +1288,10 @@ struct GmresWork {
     double *hstat;
     const hipEvent_t *ev;
     bool hmapped;  // hstat is pinned host memory the kernels write directly
+    // with hmapped: two pinned scalar slots for ||r||^2 and ||b||^2, and the
+    // event the host polls for them (hipMemcpyAsync and a blocking wait otherwise)
+    double *hscal;
+    hipEvent_t hev;
 };
 
 // doubles of a GmresWork carved from one device buffer (each piece 256-byte aligned)
@@ -1318,21 +1323,21 @@ static GmresWork gmres_work_carve(double *base, long n, int m) {
 }
 
 static int gmres_core(const GmresWork &W, hipStream_t s, long n, const Op &A, const Op &M, const double *b,
-                      double *x, const fvp_gmres_params *p, fvp_gmres_stats *st);
+                      double *x, const fvp_gmres_params *p, fvp_gmres_stats *st, bool x_zero = false);
 
 int gmres_device(fvp_csr *W, hipStream_t s, long n, const Op &A, const Op &M, const double *b, double *x,
-                 const fvp_gmres_params *p, fvp_gmres_stats *st) {
+                 const fvp_gmres_params *p, fvp_gmres_stats *st, bool x_zero) {
     const int m = p->dim_krylov;
     if (m < 1 || m > 63) { set_error("dim_krylov must be 1..63"); return -1; }
     if (p->reorthogonalise < 0 || p->reorthogonalise > 2) { set_error("reorthogonalise must be 0, 1 or 2"); return -1; }
     if (ensure_work(W, m)) return -1;
     GmresWork wk{W->Q.p, W->w.p, W->r.p, W->z.p, W->part.p, W->scal.p, W->hdev.p, W->Hd.p, W->gd.p, W->csd.p,
-                 W->snd.p, W->yd.p, W->hstat, W->ev.data(), true};
-    return gmres_core(wk, s, n, A, M, b, x, p, st);
+                 W->snd.p, W->yd.p, W->hstat, W->ev.data(), true, W->hstat + 2 * (m + 1), W->ev[m]};
+    return gmres_core(wk, s, n, A, M, b, x, p, st, x_zero);
 }
 
 static int gmres_core(const GmresWork &W, hipStream_t s, long n, const Op &A, const Op &M, const double *b,
-                      double *x, const fvp_gmres_params *p, fvp_gmres_stats *st) {
+                      double *x, const fvp_gmres_params *p, fvp_gmres_stats *st, bool x_zero) {
     const int m = p->dim_krylov;
     double *Q = W.Q, *w = W.w, *r = W.r, *z = W.z, *part = W.part, *sc = W.sc;
     // h1, h2: first / second pass coefficients (j+1 dots, then ||w||^2); stat: {estimate, h_{j+1,j}}
@@ -1345,12 +1350,27 @@ static int gmres_core(const GmresWork &W, hipStream_t s, long n, const Op &A, co
     fvp_gmres_stats stats{0, 0, 0, 0., 0.};
     double hn2[2] = {0., 0.};
     // r = b - A x; ||r||, ||b|| with one synchronisation
-    if (A(x, r)) return -1;
-    vrsub<<<nblk(n, 256), 256, 0, s>>>(n, b, r);
-    dot_async(s, n, r, r, part, sc);
-    dot_async(s, n, b, b, part, sc + 1);
-    FVP_HIP(hipMemcpyAsync(hn2, sc, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-    FVP_HIP(hipStreamSynchronize(s));
+    if (x_zero) {
+        vcopy<<<nblk(n, 256), 256, 0, s>>>(n, b, r);
+    } else {
+        if (A(x, r)) return -1;
+        vrsub<<<nblk(n, 256), 256, 0, s>>>(n, b, r);
+    }
+    // ||r||^2 and ||b||^2 to the host (pinned slots and a polled event when
+    // there are, a copy and a blocking wait otherwise)
+    auto fetch = [&](int cnt) -> int {
+        if (W.hscal) {
+            if (host_wait(s, W.hev)) return -1;
+            for (int k = 0; k < cnt; ++k) hn2[k] = W.hscal[k];
+            return 0;
+        }
+        FVP_HIP(hipMemcpyAsync(hn2, sc, cnt * sizeof(double), hipMemcpyDeviceToHost, s));
+        FVP_HIP(hipStreamSynchronize(s));
+        return 0;
+    };
+    dot_async(s, n, r, r, part, sc, W.hscal);
+    dot_async(s, n, b, b, part, sc + 1, W.hscal ? W.hscal + 1 : nullptr);
+    if (fetch(2)) return -1;
     double beta = std::sqrt(hn2[0]), normb = std::sqrt(hn2[1]);
     double error = normb > 0 ? beta / normb : 0.;
     if (beta == 0. || error < p->tol) {
@@ -1399,7 +1419,12 @@ static int gmres_core(const GmresWork &W, hipStream_t s, long n, const Op &A, co
                 FVP_HIP(hipGetLastError());
             }
             if (j > 0) {  // read step j-1 (while step j runs, when pipelined)
-                if (W.ev) FVP_HIP(hipEventSynchronize(W.ev[j - 1]));
+                if (W.ev) {  // polled: a blocking wait would wake up late
+                    hipError_t e;
+                    while ((e = hipEventQuery(W.ev[j - 1])) == hipErrorNotReady) {
+                    }
+                    FVP_HIP(e);
+                }
                 ++total;
                 const double est = W.hstat[2 * (j - 1)], hn = W.hstat[2 * (j - 1) + 1];
                 if (est < p->tol * normb || !(hn > 1e-16)) { k = j; break; }
@@ -1412,9 +1437,8 @@ static int gmres_core(const GmresWork &W, hipStream_t s, long n, const Op &A, co
         vadd<<<nblk(n, 256), 256, 0, s>>>(n, z, x);
         if (A(x, r)) return -1;
         vrsub<<<nblk(n, 256), 256, 0, s>>>(n, b, r);
-        dot_async(s, n, r, r, part, sc);
-        FVP_HIP(hipMemcpyAsync(hn2, sc, sizeof(double), hipMemcpyDeviceToHost, s));
-        FVP_HIP(hipStreamSynchronize(s));
+        dot_async(s, n, r, r, part, sc, W.hscal);
+        if (fetch(1)) return -1;
         beta = std::sqrt(hn2[0]);
         error = beta / normb;
         if (error < p->tol) converged = 1;
