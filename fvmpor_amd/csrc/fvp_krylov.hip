@@ -246,9 +246,12 @@ __device__ __forceinline__ double wave_reduce(double v) {
     return v;
 }
 
-// Workgroups of the Gram-Schmidt passes: 2048 x 256 threads keep enough
-// 16-byte loads in flight to stream the basis at HBM rate (8 per CU).
-constexpr int kGsBlocks = 2048;
+// Workgroups of the Gram-Schmidt passes: 512 x 256 threads (2 per CU), each
+// thread streaming several row pairs per column.  Measured on the 1M-node
+// box's node graph (30 unpreconditioned GMRES steps): 4.62 ms with 2048
+// workgroups, 4.53 with 1024, 4.42-4.47 with 512, 4.42 with 256; fewer
+// workgroups also mean fewer partials to reduce.
+constexpr int kGsBlocks = 512;
 
 // Leading dimension of the basis: columns padded to an even length, so that
 // every column starts 16-byte aligned for the pairwise loads.

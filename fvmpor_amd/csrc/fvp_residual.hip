@@ -634,14 +634,29 @@ __global__ void dirichlet_kernel(int nd, const int *__restrict__ dnode, const do
 // edge flux = sum over the edge's CV faces of q.n / sum of their areas
 // (flux_lim_matrix, fvmpor.h:1033-1071; the faces in the reference's (edge,
 // element) order), then the upwind test (fvmpor.h:783-794)
+// The edge's slots are taken kEdgeBatch at a time: their indices, then their
+// face values, are all in flight together (an edge has about five), and the
+// sum still runs in slot order.
+constexpr int kEdgeBatch = 8;
 __global__ void edge_flux_kernel(int NE, const int *__restrict__ slot_ptr, const int *__restrict__ slot,
                                  const double *__restrict__ qface, const double *__restrict__ ea,
                                  double *__restrict__ eflux, double *__restrict__ wf) {
     int ed = blockIdx.x * blockDim.x + threadIdx.x;
     if (ed >= NE) return;
     const double w = 1. / ea[ed];
+    const int s0 = slot_ptr[ed], s1 = slot_ptr[ed + 1];
     double sum = 0.;
-    for (int s = slot_ptr[ed]; s < slot_ptr[ed + 1]; ++s) sum += w * qface[slot[s]];
+    for (int sb = s0; sb < s1; sb += kEdgeBatch) {
+        int sl[kEdgeBatch];
+        double q[kEdgeBatch];
+#pragma unroll
+        for (int u = 0; u < kEdgeBatch; ++u) sl[u] = slot[min(sb + u, s1 - 1)];
+#pragma unroll
+        for (int u = 0; u < kEdgeBatch; ++u) q[u] = qface[sl[u]];
+#pragma unroll
+        for (int u = 0; u < kEdgeBatch; ++u)
+            if (sb + u < s1) sum += w * q[u];
+    }
     eflux[ed] = sum;
     wf[ed] = sum < 0. ? 1. : 0.;
 }
@@ -777,7 +792,7 @@ static void launch_tile(fvp_ctx *c, const double *psi, const double *psip, doubl
 // by its shift (k_r from the second van Genuchten pass, kr2), and
 // (f_base - f_moved) V^-1 / shift for (row node i, moved node k) is stored at
 // contrib[srank[16 e + 4 i + k]], where the entries' contributions lie
-// contiguous and in element order; fd_gather sums each entry's range (no
+// contiguous and in element order; fd_gather_t sums each entry's range (no
 // atomics, so the Jacobian is repeatable bit for bit).  The
 // capacity term and the Dirichlet rows are set by fd_node_kernel and
 // fd_dirichlet.  The residual is a sum of element terms, so these are the
@@ -925,23 +940,24 @@ __global__ void __launch_bounds__(256) fd_elem_kernel32(int T, int N, ZoneTable 
     }
 }
 
-// J[p] = sum of entry p's FP32 contributions (element order), accumulated in FP64
-__global__ void fd_gather32(long nnz, const long *__restrict__ cptr, const float *__restrict__ contrib,
+// J[p] = sum of entry p's contributions contrib[cptr[p] .. cptr[p + 1]) in
+// element order, accumulated in FP64 (FP32 or FP64 contributions); the loads
+// go out kEdgeBatch at a time so that they are in flight together.
+template <typename C>
+__global__ void fd_gather_t(long nnz, const long *__restrict__ cptr, const C *__restrict__ contrib,
                             double *__restrict__ J) {
     const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (p >= nnz) return;
+    const long q0 = cptr[p], q1 = cptr[p + 1];
     double s = 0.;
-    for (long q = cptr[p]; q < cptr[p + 1]; ++q) s += (double)contrib[q];
-    J[p] = s;
-}
-
-// J[p] = sum of entry p's contributions contrib[cptr[p] .. cptr[p + 1]) (element order)
-__global__ void fd_gather(long nnz, const long *__restrict__ cptr, const double *__restrict__ contrib,
-                          double *__restrict__ J) {
-    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (p >= nnz) return;
-    double s = 0.;
-    for (long q = cptr[p]; q < cptr[p + 1]; ++q) s += contrib[q];
+    for (long qb = q0; qb < q1; qb += kEdgeBatch) {
+        C c[kEdgeBatch];
+#pragma unroll
+        for (int u = 0; u < kEdgeBatch; ++u) c[u] = qb + u < q1 ? contrib[qb + u] : C(0);
+#pragma unroll
+        for (int u = 0; u < kEdgeBatch; ++u)
+            if (qb + u < q1) s += (double)c[u];
+    }
     J[p] = s;
 }
 
@@ -995,7 +1011,7 @@ int fd_jacobian_elem(fvp_ctx *c, double cj, const double *psi, const double *psi
                                                                 (const uint8_t *)c->ezone.p, c->kmul.p,
                                                                 (const uint8_t *)c->emask.p, c->nrec.p, c->kr.p, kr2,
                                                                 shift, c->invV.p, srank, c32);
-        fd_gather32<<<blocks(nnz, 256), 256, 0, s>>>(nnz, cptr, c32, J);
+        fd_gather_t<float><<<blocks(nnz, 256), 256, 0, s>>>(nnz, cptr, c32, J);
     } else {
         if (c->dim == 3)
             fd_elem_kernel<3><<<blocks(T, 256), 256, 0, s>>>(T, N, c->zt, c->rho0, (const int4 *)c->conn.p,
@@ -1007,7 +1023,7 @@ int fd_jacobian_elem(fvp_ctx *c, double cj, const double *psi, const double *psi
                                                               (const uint8_t *)c->ezone.p, c->kmul.p,
                                                               (const uint8_t *)c->emask.p, c->nrec.p, c->kr.p, kr2,
                                                               shift, c->invV.p, srank, contrib);
-        fd_gather<<<blocks(nnz, 256), 256, 0, s>>>(nnz, cptr, contrib, J);
+        fd_gather_t<double><<<blocks(nnz, 256), 256, 0, s>>>(nnz, cptr, contrib, J);
     }
     fd_node_kernel<<<blocks(N, 256), 256, 0, s>>>(N, cj, psip, shift, c->cap.p, cap2, dpos, J);
     if (c->ndir)

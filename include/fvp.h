@@ -205,7 +205,7 @@ int fvp_gmres_csr_deflated(fvp_csr *A, const double *b, double *x, const fvp_gmr
  * counterpart in the reference: for callers that cannot allocate device or
  * pinned memory (tight serving loops, harnesses that forbid allocation).  The
  * preconditioner is built beforehand (fvp_precon_build allocates).  Limit:
- * the workspace, about n (dim_krylov + 4) + 35 k doubles, must fit 4 Mi
+ * the workspace, about n (dim_krylov + 4) + 10 k doubles, must fit 4 Mi
  * doubles; larger problems return < 0 with a message.  Every call on one GPU
  * shares the array, so calls that overlap on one GPU (other threads or
  * streams) race. */
