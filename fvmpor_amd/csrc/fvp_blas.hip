@@ -29,8 +29,17 @@ __global__ void __launch_bounds__(kRedThreads) dot_partial(long n, const double 
 __global__ void __launch_bounds__(kRedThreads) sum_partials(int np, const double *__restrict__ part,
                                                            double *__restrict__ out, double *__restrict__ hout) {
     __shared__ double lds[kRedThreads / 64];
+    // each lane's partials in order, four loads in flight at a time
     double s = 0.;
-    for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
+    const int bd = blockDim.x;
+    for (int i0 = threadIdx.x; i0 < np; i0 += 4 * bd) {
+        double q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = i0 + u * bd < np ? part[i0 + u * bd] : 0.;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + u * bd < np) s += q[u];
+    }
     s = block_sum(s, lds);
     if (threadIdx.x == 0) {
         out[0] = s;

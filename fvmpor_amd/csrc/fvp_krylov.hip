@@ -487,8 +487,17 @@ __global__ void __launch_bounds__(256) arnoldi_finish(int j, int m, const double
     const bool two = !npart && flag && flag[0] != 0.;
     double nn;
     if (npart) {
+        // each lane's partials in order, four loads in flight at a time
         double s = 0.;
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) s += npart[b];
+        const int bd = blockDim.x;
+        for (int b0 = threadIdx.x; b0 < nb; b0 += 4 * bd) {
+            double q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = b0 + u * bd < nb ? npart[b0 + u * bd] : 0.;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (b0 + u * bd < nb) s += q[u];
+        }
         s = wave_reduce(s);
         if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
         __syncthreads();
@@ -555,8 +564,18 @@ __global__ void __launch_bounds__(kThreads) reduce_cols(int nb, const double *__
     if (skip && skip[0] == 0.) return;
     __shared__ double lds[kThreads / 64];
     int c = blockIdx.x;
+    // each lane's partials in order, four loads in flight at a time
     double s = 0.;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[c * nb + b];
+    const int bd = blockDim.x;
+    const double *pc = part + (size_t)c * nb;
+    for (int b0 = threadIdx.x; b0 < nb; b0 += 4 * bd) {
+        double q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = b0 + u * bd < nb ? pc[b0 + u * bd] : 0.;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (b0 + u * bd < nb) s += q[u];
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
     if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;

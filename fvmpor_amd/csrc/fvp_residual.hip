@@ -73,10 +73,25 @@ __global__ void __launch_bounds__(256) vg_kernel(int N, ZoneTable zt, double rho
     __shared__ double zp[kMaxZones * 10];
     __shared__ double red[4];
     if (threadIdx.x < 10 * zt.nz) zp[threadIdx.x] = zt.p[threadIdx.x / 10][threadIdx.x % 10];
+    // the node's own inputs are loaded first: their latency overlaps sigma's reduction
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < N;
+    double h = live ? psi[i] : 0.;
+    const double vi = JV && live ? jv.v[i] : 0.;
+    unsigned m = live ? zmask[i] : 0u;
     double sg = 0.;
     if (JV && jv.np > 0) {
+        // each lane's partials in order, four loads in flight at a time
         double b = 0.;
-        for (int k = threadIdx.x; k < jv.np; k += blockDim.x) b += jv.part_v[k];
+        const int bd = blockDim.x;
+        for (int k0 = threadIdx.x; k0 < jv.np; k0 += 4 * bd) {
+            double q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = k0 + u * bd < jv.np ? jv.part_v[k0 + u * bd] : 0.;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (k0 + u * bd < jv.np) b += q[u];
+        }
         b = block_sum(b, red);  // ends with a barrier, so the zone table is in place
         if (threadIdx.x == 0) {
             red[0] = jv_sigma_of(jv.psi_n2[0], b);
@@ -88,14 +103,11 @@ __global__ void __launch_bounds__(256) vg_kernel(int N, ZoneTable zt, double rho
         __syncthreads();
         if (JV) sg = jv.sig[0];
     }
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    double h = psi[i];
-    if (JV) h = psi[i] + sg * jv.v[i];
+    if (!live) return;
+    if (JV) h += sg * vi;
     // the head of the node record the element kernel gathers (x, y, z are
     // written once at set-up)
     if (nrec) nrec[4 * (size_t)i + 3] = h;
-    unsigned m = zmask[i];
     double phi = 0., Sw = 0., dSw = 0., Ss = 0.;
     while (m) {
         const int z = __ffs(m) - 1;
