@@ -294,7 +294,10 @@ typedef struct {
 int fvp_integrator_create(fvp_ctx *ctx, const fvp_integrator_params *p, double t0, const double *psi0,
                           fvp_integrator **out);
 int fvp_integrator_destroy(fvp_integrator *I);
-/* One accepted step, never past t_stop (IDASolve IDA_ONE_STEP with a stop time). */
+/* One accepted step, never past t_stop (IDASolve IDA_ONE_STEP with a stop time).
+ * The calling thread waits for the device by polling HIP events (the norms it
+ * reads are written to pinned host memory by the kernels), so it is busy for
+ * the whole step; run one host thread per context that advances at once. */
 int fvp_integrator_step(fvp_integrator *I, double t_stop);
 /* Step until t_out (Solver::advance(t), solver.h:176-180); psi(t_out) into psi_out (device). */
 int fvp_integrator_advance(fvp_integrator *I, double t_out, double *psi_out);
